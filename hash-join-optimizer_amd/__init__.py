@@ -1,0 +1,12 @@
+"""MI355X-native multi-way INT32 equi-join engine: a drop-in for the reference's
+Contest::execute (src/execute.cpp:598) with HIP kernels for gfx950.
+
+Import as importlib.import_module("hash-join-optimizer_amd") with the repository root on sys.path.
+"""
+from . import abi  # noqa: F401
+from .abi import EngineMissing  # noqa: F401
+from .plan import (Codec, Column, ColumnarTable, DataType, JoinNode, Plan,  # noqa: F401
+                   PlanNode, ScanNode, Table, sorted_rows)
+from .engine import Context, DeviceResult, DeviceTable, Directory, EngineError  # noqa: F401
+
+HASH_ALGO = "hip"  # the hash-table implementation this package provides (GPU unchained directory)

@@ -1,0 +1,935 @@
+// engine.cpp - plan execution on one GPU and the extern "C" layer of include/hjoin.h.
+//
+// execute() (src/execute.cpp:584-608) walks the plan post-order like the reference, but
+// intermediate results are late-materialized row-id lists instead of value_t columns:
+//   scan node  -> a lazy relation over the resident base table (identity row ids + filters)
+//   join node  -> build side: key extraction (ballot compaction) -> LSD radix sort by the
+//                 Fibonacci hash -> unchained directory (slot starts + Bloom bits);
+//                 probe side: per-block match counts -> device scan -> write of one row-id
+//                 array per base table the node's output_attrs still reference
+//   root       -> gather of the output columns (INT32 values, VARCHAR bytes) into flat
+//                 HBM buffers; page encoding happens on the host afterwards.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hjoin.h"
+#include "hj_internal.hpp"
+#include "kernels.hpp"
+#include "pagecodec.hpp"
+
+namespace hj {
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string& m) {
+    std::lock_guard<std::mutex> g(g_err_mu);
+    g_err = m;
+}
+
+template <class T>
+T fetch(hj_context* ctx, const T* dptr) {
+    HJ_HIP(hipMemcpyAsync(ctx->pinned, dptr, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    HJ_HIP(hipStreamSynchronize(ctx->stream));
+    T v;
+    std::memcpy(&v, ctx->pinned, sizeof(T));
+    return v;
+}
+
+uint32_t log2_pow2_ceil(uint64_t n) {
+    uint32_t k = 0;
+    while ((1ull << k) < n) ++k;
+    return k;
+}
+
+// ------------------------------------------------------------------ relations
+struct Slot {
+    uint32_t table;
+    const uint32_t* rid;  // device row ids into the base table; nullptr = identity
+};
+struct AttrRef {
+    uint32_t slot;
+    uint32_t col;
+    int32_t type;
+};
+struct Rel {
+    bool is_scan = false;
+    uint64_t n = 0;
+    std::vector<Slot> slots;
+    std::vector<AttrRef> attrs;
+    uint32_t table = 0;                // scan only
+    std::vector<hj_filter> filters;    // scan only
+};
+
+class Executor {
+  public:
+    Executor(hj_context* ctx, const hj_plan* plan, const hj_dtable* const* tables)
+        : ctx_(ctx), plan_(plan), tables_(tables), s_(ctx->stream), A_(ctx->arena) {}
+
+    hj_dresult* run() {
+        if (!plan_->nodes || plan_->root >= plan_->num_nodes)
+            throw Error(HJ_E_INVALID, "plan root out of range");
+        depth_ = 0;
+        Rel r = node(plan_->root);
+        return materialize(r, plan_->nodes[plan_->root]);
+    }
+
+  private:
+    hj_context* ctx_;
+    const hj_plan* plan_;
+    const hj_dtable* const* tables_;
+    hipStream_t s_;
+    Arena& A_;
+    int depth_ = 0;
+
+    template <class T>
+    T* scratch(uint64_t n) {
+        return (T*)A_.scratch(n * sizeof(T));
+    }
+    template <class T>
+    T* result(uint64_t n) {
+        return (T*)A_.result(n * sizeof(T));
+    }
+
+    const hj_dtable* table(uint64_t id) const {
+        if (id >= plan_->num_inputs || !tables_[id])
+            throw Error(HJ_E_INVALID, "scan of an input table that does not exist");
+        return tables_[id];
+    }
+    const DevColumn& column(uint32_t t, uint32_t c) const {
+        const hj_dtable* tb = table(t);
+        if (c >= tb->cols.size()) throw Error(HJ_E_INVALID, "column index out of range");
+        return tb->cols[c];
+    }
+
+    Rel node(uint64_t idx) {
+        if (idx >= plan_->num_nodes) throw Error(HJ_E_INVALID, "plan node index out of range");
+        if (++depth_ > 4096) throw Error(HJ_E_INVALID, "plan is not a tree");
+        const hj_node& nd = plan_->nodes[idx];
+        Rel r;
+        if (nd.kind == HJ_NODE_SCAN) {
+            r = scan(nd);
+        } else if (nd.kind == HJ_NODE_JOIN) {
+            Rel L = node(nd.left);
+            Rel R = node(nd.right);
+            r = join(nd, L, R);
+        } else {
+            throw Error(HJ_E_INVALID, "unknown plan node kind");
+        }
+        --depth_;
+        return r;
+    }
+
+    Rel scan(const hj_node& nd) {
+        const hj_dtable* tb = table(nd.base_table_id);
+        Rel r;
+        r.is_scan = true;
+        r.table = (uint32_t)nd.base_table_id;
+        r.n = tb->num_rows;
+        r.slots.push_back({r.table, nullptr});
+        for (uint64_t i = 0; i < nd.num_output_attrs; ++i) {
+            const hj_attr& a = nd.output_attrs[i];
+            if (a.index >= tb->cols.size()) throw Error(HJ_E_INVALID, "scan attribute out of range");
+            r.attrs.push_back({0, (uint32_t)a.index, tb->cols[a.index].type});
+        }
+        if (nd.num_filters > (uint64_t)kMaxFilters)
+            throw Error(HJ_E_INVALID, "too many filters on one scan");
+        for (uint64_t f = 0; f < nd.num_filters; ++f) {
+            const hj_filter& fl = nd.filters[f];
+            if (fl.column >= tb->cols.size() || tb->cols[fl.column].type != HJ_INT32)
+                throw Error(HJ_E_TYPE, "filter column must be an INT32 column of the scanned table");
+            r.filters.push_back(fl);
+        }
+        return r;
+    }
+
+    ScanSrc scan_src(const Rel& r, const AttrRef& key) const {
+        ScanSrc s;
+        std::memset(&s, 0, sizeof s);
+        s.key = column(r.table, key.col).view();
+        s.nf = (int32_t)r.filters.size();
+        for (size_t f = 0; f < r.filters.size(); ++f) {
+            s.fcol[f] = column(r.table, r.filters[f].column).view();
+            s.flo[f] = r.filters[f].lo;
+            s.fhi[f] = r.filters[f].hi;
+        }
+        return s;
+    }
+
+    RelSrc rel_src(const Rel& r, const AttrRef& key) const {
+        RelSrc s;
+        s.key = column(r.slots[key.slot].table, key.col).view();
+        s.rid = r.slots[key.slot].rid;
+        s.n = r.n;
+        return s;
+    }
+
+    const AttrRef& key_attr(const Rel& r, uint64_t idx) const {
+        if (idx >= r.attrs.size()) throw Error(HJ_E_INVALID, "join attribute out of range");
+        const AttrRef& a = r.attrs[idx];
+        const Slot& sl = r.slots[a.slot];
+        if (column(sl.table, a.col).type != HJ_INT32)
+            throw Error(HJ_E_TYPE, "Join key isn't of type int32_t");
+        return a;
+    }
+
+    Rel join(const hj_node& nd, Rel& L, Rel& R) {
+        const bool bl = nd.build_left != 0;
+        Rel& B = bl ? L : R;
+        Rel& P = bl ? R : L;
+        const AttrRef bkey = key_attr(B, bl ? nd.left_attr : nd.right_attr);
+        const AttrRef pkey = key_attr(P, bl ? nd.right_attr : nd.left_attr);
+
+        // output slots: one per (relation, slot) pair the output attributes reference
+        Rel out;
+        std::vector<std::pair<int, uint32_t>> used;  // (0 = L, 1 = R), slot
+        for (uint64_t i = 0; i < nd.num_output_attrs; ++i) {
+            uint64_t idx = nd.output_attrs[i].index;
+            int side;
+            AttrRef a;
+            if (idx < L.attrs.size()) {
+                side = 0;
+                a = L.attrs[idx];
+            } else if (idx < L.attrs.size() + R.attrs.size()) {
+                side = 1;
+                a = R.attrs[idx - L.attrs.size()];
+            } else {
+                throw Error(HJ_E_INVALID, "output attribute out of range");
+            }
+            auto key = std::make_pair(side, a.slot);
+            auto it = std::find(used.begin(), used.end(), key);
+            uint32_t ns;
+            if (it == used.end()) {
+                ns = (uint32_t)used.size();
+                used.push_back(key);
+            } else {
+                ns = (uint32_t)(it - used.begin());
+            }
+            out.attrs.push_back({ns, a.col, a.type});
+        }
+        if (used.size() > (size_t)kMaxSlots)
+            throw Error(HJ_E_INVALID, "a join output references more than 16 base tables");
+
+        const size_t mark = A_.mark();
+        // ---------------- build
+        uint32_t *bh = nullptr, *brow = nullptr;
+        uint64_t nb = 0;
+        {
+            const bool sc = B.is_scan;
+            ScanSrc ss;
+            RelSrc rs;
+            uint32_t nblk;
+            if (sc) {
+                ss = scan_src(B, bkey);
+                nblk = extract_scan_blocks(ss);
+            } else {
+                rs = rel_src(B, bkey);
+                nblk = rel_blocks(rs.n);
+            }
+            if (nblk) {
+                uint32_t* counts = scratch<uint32_t>(nblk);
+                uint32_t* offs = scratch<uint32_t>(nblk);
+                uint32_t* part = scratch<uint32_t>(scan_partials_needed(nblk));
+                uint32_t* tot = scratch<uint32_t>(1);
+                if (sc)
+                    extract_scan_count(ss, counts, s_);
+                else
+                    extract_rel_count(rs, counts, s_);
+                scan_excl_sum_u32(counts, offs, nblk, part, tot, s_);
+                nb = fetch(ctx_, tot);
+                if (nb) {
+                    bh = scratch<uint32_t>(nb);
+                    brow = scratch<uint32_t>(nb);
+                    if (sc)
+                        extract_scan_write(ss, offs, bh, brow, s_);
+                    else
+                        extract_rel_write(rs, offs, bh, brow, s_);
+                }
+            }
+        }
+        if (nb == 0) {
+            A_.reset(mark);
+            return empty_like(out, used, L, R);
+        }
+        {
+            uint32_t* th = scratch<uint32_t>(nb);
+            uint32_t* tr = scratch<uint32_t>(nb);
+            void* sortscr = A_.scratch(radix_sort_scratch_bytes(nb));
+            radix_sort_pairs(bh, brow, th, tr, nb, sortscr, s_);
+        }
+        const uint32_t k = log2_pow2_ceil(nb);
+        const uint64_t D = 1ull << k;
+        uint64_t* dir = scratch<uint64_t>(D + 1);
+        {
+            uint32_t* e = scratch<uint32_t>(D);
+            uint32_t* bloom = scratch<uint32_t>(D);
+            uint32_t* part = scratch<uint32_t>(scan_partials_needed(D));
+            build_directory(bh, nb, k, dir, e, bloom, part, s_);
+        }
+        DirView dv;
+        dv.dir = dir;
+        dv.sh = bh;
+        dv.srow = brow;
+        dv.k = k;
+        dv.pad = 0;
+        dv.n = nb;
+
+        // ---------------- probe
+        const bool psc = P.is_scan;
+        ScanSrc pss;
+        RelSrc prs;
+        uint32_t pblk;
+        if (psc) {
+            pss = scan_src(P, pkey);
+            pblk = pss.key.num_pages;
+        } else {
+            prs = rel_src(P, pkey);
+            pblk = rel_blocks(prs.n);
+        }
+        uint64_t total = 0;
+        uint64_t* poffs = nullptr;
+        if (pblk) {
+            uint64_t* counts = scratch<uint64_t>(pblk);
+            poffs = scratch<uint64_t>(pblk);
+            uint64_t* part = scratch<uint64_t>(scan_partials_needed(pblk));
+            uint64_t* tot = scratch<uint64_t>(1);
+            if (psc)
+                probe_scan_count(pss, dv, counts, s_);
+            else
+                probe_rel_count(prs, dv, counts, s_);
+            scan_excl_sum_u64(counts, poffs, pblk, part, tot, s_);
+            total = fetch(ctx_, tot);
+        }
+        if (total >= (1ull << 32))
+            throw Error(HJ_E_INVALID, "a join produced 2^32 or more rows (row ids are 32-bit)");
+        if (total == 0) {
+            A_.reset(mark);
+            return empty_like(out, used, L, R);
+        }
+        OutSlots o;
+        std::memset(&o, 0, sizeof o);
+        o.n = (int32_t)used.size();
+        for (size_t i = 0; i < used.size(); ++i) {
+            const Rel& src = used[i].first == 0 ? L : R;
+            const bool from_build = (used[i].first == 0) == bl;
+            uint32_t* dst = result<uint32_t>(total);
+            o.dst[i] = dst;
+            o.src[i] = src.slots[used[i].second].rid;
+            o.side[i] = from_build ? 1 : 0;
+            out.slots.push_back({src.slots[used[i].second].table, dst});
+        }
+        out.n = total;
+        HeavyEntry* heavy = scratch<HeavyEntry>(total / kLightMax + 1);
+        uint32_t* hcount = scratch<uint32_t>(1);
+        HJ_HIP(hipMemsetAsync(hcount, 0, sizeof(uint32_t), s_));
+        if (psc)
+            probe_scan_write(pss, dv, poffs, o, heavy, hcount, s_);
+        else
+            probe_rel_write(prs, dv, poffs, o, heavy, hcount, s_);
+        expand_heavy(dv, o, heavy, hcount, s_);
+        // the scratch (build side, directory, counts) is reused by the next operator; the
+        // stream orders the reuse after these kernels
+        A_.reset(mark);
+        return out;
+    }
+
+    Rel empty_like(Rel out, const std::vector<std::pair<int, uint32_t>>& used, const Rel& L,
+                   const Rel& R) {
+        out.n = 0;
+        for (auto& u : used) {
+            const Rel& src = u.first == 0 ? L : R;
+            out.slots.push_back({src.slots[u.second].table, nullptr});
+        }
+        return out;
+    }
+
+    hj_dresult* materialize(Rel& r, const hj_node& root) {
+        // a filtered scan at the root is compacted to a row-id list first
+        if (r.is_scan && !r.filters.empty()) {
+            AttrRef any{0, r.filters[0].column, HJ_INT32};
+            ScanSrc ss = scan_src(r, any);
+            uint32_t nblk = ss.key.num_pages;
+            uint64_t n = 0;
+            uint32_t* rows = nullptr;
+            if (nblk) {
+                uint32_t* counts = scratch<uint32_t>(nblk);
+                uint32_t* offs = scratch<uint32_t>(nblk);
+                uint32_t* part = scratch<uint32_t>(scan_partials_needed(nblk));
+                uint32_t* tot = scratch<uint32_t>(1);
+                select_scan_count(ss, counts, s_);
+                scan_excl_sum_u32(counts, offs, nblk, part, tot, s_);
+                n = fetch(ctx_, tot);
+                rows = result<uint32_t>(n ? n : 1);
+                if (n) select_scan_write(ss, offs, rows, s_);
+            }
+            r.n = n;
+            r.slots[0].rid = rows;
+            r.is_scan = false;
+        }
+        auto* res = new hj_dresult();
+        res->num_rows = r.n;
+        const uint64_t n = r.n;
+        for (size_t i = 0; i < root.num_output_attrs; ++i) {
+            const AttrRef& a = r.attrs[i];
+            const Slot& sl = r.slots[a.slot];
+            const DevColumn& dc = column(sl.table, a.col);
+            hj_dresult::Col c;
+            c.type = dc.type;
+            if (root.output_attrs[i].type != dc.type)
+                throw Error(HJ_E_TYPE, "output attribute type does not match its column");
+            const uint64_t vbytes = ((n + 63) / 64) * 8;
+            if (dc.type == HJ_INT32) {
+                c.values = result<int32_t>(n ? n : 1);
+                c.validity = result<uint8_t>(vbytes ? vbytes : 8);
+                gather_int32(dc.view(), sl.rid, n, c.values, c.validity, s_);
+            } else if (dc.type == HJ_VARCHAR) {
+                c.offsets = result<uint64_t>(n + 1);
+                c.validity = result<uint8_t>(vbytes ? vbytes : 8);
+                if (n) {
+                    const size_t mark = A_.mark();
+                    uint64_t* lens = scratch<uint64_t>(n);
+                    uint64_t* part = scratch<uint64_t>(scan_partials_needed(n));
+                    gather_strlen(dc.view(), sl.rid, n, lens, c.validity, s_);
+                    scan_excl_sum_u64(lens, c.offsets, n, part, c.offsets + n, s_);
+                    c.num_chars = fetch(ctx_, c.offsets + n);
+                    A_.reset(mark);
+                } else {
+                    HJ_HIP(hipMemsetAsync(c.offsets, 0, sizeof(uint64_t), s_));
+                }
+                c.chars = result<char>(c.num_chars ? c.num_chars : 1);
+                gather_strcopy(dc.view(), sl.rid, n, c.offsets, c.chars, s_);
+            } else {
+                throw Error(HJ_E_TYPE, "unsupported output column type (only INT32 and VARCHAR)");
+            }
+            res->cols.push_back(c);
+        }
+        res->live = true;
+        return res;
+    }
+};
+
+// ------------------------------------------------------------------ upload
+uint32_t rd16h(const uint8_t* p) {
+    uint16_t v;
+    std::memcpy(&v, p, 2);
+    return v;
+}
+
+void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj_dtable* t,
+                   DevColumn& dc) {
+    dc.type = hc.type;
+    dc.num_pages = hc.num_pages;
+    if (hc.num_pages && !hc.pages) throw Error(HJ_E_INVALID, "column pages missing");
+    dc.h_row_start.assign((size_t)hc.num_pages + 1, 0);
+    std::vector<uint32_t> group_start(hc.num_pages ? hc.num_pages : 1, 0);
+    uint64_t rows = 0, groups = 0;
+    uint32_t first_rows = 0;
+    bool uniform = true;
+    dc.all_dense = 1;
+    for (uint32_t p = 0; p < hc.num_pages; ++p) {
+        const uint8_t* pg = hc.pages[p];
+        if (!pg) throw Error(HJ_E_INVALID, "null page pointer");
+        const uint32_t nr = rd16h(pg), nn = rd16h(pg + 2);
+        dc.h_row_start[p] = (uint32_t)rows;
+        group_start[p] = (uint32_t)groups;
+        uint32_t prow = nr;
+        if (hc.type == HJ_VARCHAR && (nr == 0xffffu || nr == 0xfffeu)) {
+            if (nr == 0xfffeu && (p == 0)) throw Error(HJ_E_INVALID, "0xfffe page without a long string");
+            if (nn > HJ_PAGE_SIZE - 4) throw Error(HJ_E_INVALID, "corrupt long-string page");
+            prow = nr == 0xffffu ? 1 : 0;
+            dc.num_chars += nn;
+            dc.all_dense = 0;  // long-string pages have no bitmap but break uniformity
+            uniform = false;
+        } else if (hc.type == HJ_INT32) {
+            if (nn > nr || 4ull + 4ull * nn + (nr + 7) / 8 > HJ_PAGE_SIZE)
+                throw Error(HJ_E_INVALID, "corrupt INT32 page");
+            if (nn != nr) {
+                dc.all_dense = 0;
+                groups += (nr + 63) / 64;
+            }
+        } else if (hc.type == HJ_VARCHAR) {
+            if (nn > nr || 4ull + 2ull * nn + (nr + 7) / 8 > HJ_PAGE_SIZE)
+                throw Error(HJ_E_INVALID, "corrupt VARCHAR page");
+            uint32_t prev = 0;
+            for (uint32_t i = 0; i < nn; ++i) {
+                uint32_t e = rd16h(pg + 4 + 2 * i);
+                if (e < prev) throw Error(HJ_E_INVALID, "corrupt VARCHAR offsets");
+                dc.max_len = std::max(dc.max_len, e - prev);
+                prev = e;
+            }
+            if (4ull + 2ull * nn + prev + (nr + 7) / 8 > HJ_PAGE_SIZE)
+                throw Error(HJ_E_INVALID, "corrupt VARCHAR page");
+            dc.num_chars += prev;
+            if (nn != nr) {
+                dc.all_dense = 0;
+                groups += (nr + 63) / 64;
+            }
+        } else {
+            prow = nr;  // INT64/FP64 are kept resident but never decoded
+        }
+        if (p == 0) first_rows = prow;
+        if (p + 1 < hc.num_pages && prow != first_rows) uniform = false;
+        rows += prow;
+        if (rows >= (1ull << 32)) throw Error(HJ_E_INVALID, "a table must have fewer than 2^32 rows");
+    }
+    dc.h_row_start[hc.num_pages] = (uint32_t)rows;
+    if (rows != table_rows && !(hc.num_pages == 0 && table_rows == 0))
+        throw Error(HJ_E_INVALID, "column row count (" + std::to_string(rows) +
+                                      ") differs from the table's num_rows (" +
+                                      std::to_string(table_rows) + ")");
+    dc.num_rows = rows;
+    dc.uniform_rows = (uniform && hc.num_pages && first_rows) ? first_rows : 0;
+    if (hc.type == HJ_VARCHAR && dc.uniform_rows) {
+        // VARCHAR pages hold a varying number of rows; only use the division shortcut when
+        // every page really is the same size (checked above)
+    }
+    const size_t bytes = (size_t)hc.num_pages * HJ_PAGE_SIZE;
+    auto dev_alloc = [&](size_t n) -> void* {
+        void* p = nullptr;
+        HJ_HIP(hipMalloc(&p, n ? n : 256));
+        t->allocations.push_back(p);
+        return p;
+    };
+    dc.pages = (uint8_t*)dev_alloc(bytes);
+    if (bytes) {
+        std::vector<uint8_t> staging(bytes);
+        for (uint32_t p = 0; p < hc.num_pages; ++p)
+            std::memcpy(staging.data() + (size_t)p * HJ_PAGE_SIZE, hc.pages[p], HJ_PAGE_SIZE);
+        HJ_HIP(hipMemcpyAsync(dc.pages, staging.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+        HJ_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    dc.row_start = (uint32_t*)dev_alloc(dc.h_row_start.size() * 4);
+    HJ_HIP(hipMemcpyAsync(dc.row_start, dc.h_row_start.data(), dc.h_row_start.size() * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    dc.group_start = (uint32_t*)dev_alloc(group_start.size() * 4);
+    HJ_HIP(hipMemcpyAsync(dc.group_start, group_start.data(), group_start.size() * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    dc.rank = (uint16_t*)dev_alloc(groups * 2);
+    if (!dc.all_dense && groups) launch_rank_index(dc.view(), dc.rank, ctx->stream);
+    HJ_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// ------------------------------------------------------------------ error plumbing
+template <class F>
+int guarded(hj_context* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->last_error.clear();
+        return HJ_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->last_error = e.what();
+        set_global_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->last_error = "host out of memory";
+        set_global_error("host out of memory");
+        return HJ_E_OOM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->last_error = e.what();
+        set_global_error(e.what());
+        return HJ_E_INVALID;
+    }
+}
+
+void free_dresult(hj_context* ctx, hj_dresult* r) {
+    if (!r) return;
+    if (ctx && ctx->live_result == r) {
+        ctx->live_result = nullptr;
+        ctx->arena.clear();
+    }
+    delete r;
+}
+
+void to_host(hj_context* ctx, const hj_dresult* r, hj_result* out) {
+    HJ_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t n = r->num_rows;
+    out->num_rows = n;
+    out->num_columns = (uint32_t)r->cols.size();
+    out->columns = (hj_out_column*)std::calloc(r->cols.size() ? r->cols.size() : 1,
+                                               sizeof(hj_out_column));
+    if (!out->columns) throw std::bad_alloc();
+    const uint64_t vbytes = ((n + 63) / 64) * 8;
+    for (size_t c = 0; c < r->cols.size(); ++c) {
+        const auto& col = r->cols[c];
+        std::vector<uint8_t> valid(vbytes ? vbytes : 8);
+        if (vbytes) HJ_HIP(hipMemcpy(valid.data(), col.validity, vbytes, hipMemcpyDeviceToHost));
+        auto vf = [&valid](uint64_t i) { return ((valid[i >> 3] >> (i & 7)) & 1) != 0; };
+        if (col.type == HJ_INT32) {
+            std::vector<int32_t> vals(n ? n : 1);
+            if (n) HJ_HIP(hipMemcpy(vals.data(), col.values, n * 4, hipMemcpyDeviceToHost));
+            encode_int32(vals.data(), vf, n, &out->columns[c]);
+        } else {
+            std::vector<uint64_t> offs(n + 1);
+            HJ_HIP(hipMemcpy(offs.data(), col.offsets, (n + 1) * 8, hipMemcpyDeviceToHost));
+            std::vector<char> chars(col.num_chars ? col.num_chars : 1);
+            if (col.num_chars)
+                HJ_HIP(hipMemcpy(chars.data(), col.chars, col.num_chars, hipMemcpyDeviceToHost));
+            encode_varchar(offs.data(), chars.data(), vf, n, &out->columns[c]);
+        }
+    }
+}
+
+}  // namespace
+}  // namespace hj
+
+// ====================================================================== directory object
+struct hj_directory {
+    uint64_t n = 0;
+    uint32_t k = 0;
+    uint64_t max_slot = 0;
+    std::vector<void*> allocs;
+    hj::DirView view{};
+};
+
+using namespace hj;
+
+extern "C" {
+
+const char* hj_global_error(void) {
+    std::lock_guard<std::mutex> g(g_err_mu);
+    static thread_local std::string copy;
+    copy = g_err;
+    return copy.c_str();
+}
+
+const char* hj_last_error(const hj_context* ctx) {
+    return ctx ? ctx->last_error.c_str() : hj_global_error();
+}
+
+int hj_context_create(uint64_t arena_bytes, hj_context** out) {
+    if (!out) return HJ_E_INVALID;
+    *out = nullptr;
+    auto* ctx = new (std::nothrow) hj_context();
+    if (!ctx) return HJ_E_OOM;
+    int rc = guarded(nullptr, [&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw Error(HJ_E_NODEV, "no HIP device available");
+        HJ_HIP(hipGetDevice(&ctx->device));
+        hipDeviceProp_t prop;
+        HJ_HIP(hipGetDeviceProperties(&prop, ctx->device));
+        ctx->num_cus = prop.multiProcessorCount;
+        HJ_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        uint64_t bytes = arena_bytes;
+        if (!bytes) {
+            const char* e = std::getenv("HJ_ARENA_BYTES");
+            bytes = e ? std::strtoull(e, nullptr, 10) : 0;
+        }
+        if (!bytes) bytes = 16ull << 30;
+        HJ_HIP(hipMalloc(&ctx->arena_mem, bytes));
+        ctx->arena.init(ctx->arena_mem, bytes);
+        HJ_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+    });
+    if (rc) {
+        if (ctx->arena_mem) (void)hipFree(ctx->arena_mem);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return HJ_OK;
+}
+
+int hj_context_destroy(hj_context* ctx) {
+    if (!ctx) return HJ_E_INVALID;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->live_result) delete ctx->live_result;
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->arena_mem) (void)hipFree(ctx->arena_mem);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HJ_OK;
+}
+
+void* hj_context_stream(hj_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int hj_synchronize(hj_context* ctx) {
+    if (!ctx) return HJ_E_INVALID;
+    return guarded(ctx, [&] { HJ_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int hj_table_upload(hj_context* ctx, const hj_table* table, hj_dtable** out) {
+    if (!ctx || !table || !out) return HJ_E_INVALID;
+    *out = nullptr;
+    auto* t = new hj_dtable();
+    int rc = guarded(ctx, [&] {
+        if (table->num_columns && !table->columns) throw Error(HJ_E_INVALID, "columns missing");
+        t->num_rows = table->num_rows;
+        t->cols.resize(table->num_columns);
+        for (uint32_t c = 0; c < table->num_columns; ++c)
+            upload_column(ctx, table->columns[c], table->num_rows, t, t->cols[c]);
+    });
+    if (rc) {
+        for (void* p : t->allocations) (void)hipFree(p);
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return HJ_OK;
+}
+
+int hj_table_free(hj_context* ctx, hj_dtable* t) {
+    if (!t) return HJ_E_INVALID;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : t->allocations) (void)hipFree(p);
+    delete t;
+    return HJ_OK;
+}
+
+uint64_t hj_table_num_rows(const hj_dtable* t) { return t ? t->num_rows : 0; }
+
+int hj_execute_device(hj_context* ctx, const hj_plan* plan, const hj_dtable* const* tables,
+                      hj_dresult** out) {
+    if (!ctx || !plan || !out || (!tables && plan->num_inputs)) return HJ_E_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        if (ctx->live_result)
+            throw Error(HJ_E_BUSY, "free the previous device result before the next execute");
+        ctx->arena.clear();
+        Executor ex(ctx, plan, tables);
+        hj_dresult* r = nullptr;
+        try {
+            r = ex.run();
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->arena.clear();
+            throw;
+        }
+        ctx->live_result = r;
+        *out = r;
+    });
+}
+
+uint64_t hj_dresult_num_rows(const hj_dresult* r) { return r ? r->num_rows : 0; }
+
+int hj_dresult_column(const hj_dresult* r, uint32_t c, int32_t* type, const void** values,
+                      const void** chars, const uint8_t** validity) {
+    if (!r || c >= r->cols.size()) return HJ_E_INVALID;
+    const auto& col = r->cols[c];
+    if (type) *type = col.type;
+    if (values) *values = col.type == HJ_INT32 ? (const void*)col.values : (const void*)col.offsets;
+    if (chars) *chars = col.chars;
+    if (validity) *validity = col.validity;
+    return HJ_OK;
+}
+
+int hj_dresult_to_host(hj_context* ctx, const hj_dresult* r, hj_result* result) {
+    if (!ctx || !r || !result) return HJ_E_INVALID;
+    std::memset(result, 0, sizeof *result);
+    int rc = guarded(ctx, [&] { to_host(ctx, r, result); });
+    if (rc) hj_result_free(result);
+    return rc;
+}
+
+int hj_dresult_free(hj_context* ctx, hj_dresult* r) {
+    if (!r) return HJ_E_INVALID;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    free_dresult(ctx, r);
+    return HJ_OK;
+}
+
+void hj_result_free(hj_result* r) {
+    if (!r || !r->columns) return;
+    for (uint32_t c = 0; c < r->num_columns; ++c) std::free(r->columns[c].pages);
+    std::free(r->columns);
+    r->columns = nullptr;
+    r->num_columns = 0;
+    r->num_rows = 0;
+}
+
+int hj_execute(hj_context* ctx, const hj_plan* plan, hj_result* result) {
+    if (!ctx || !plan || !result) return HJ_E_INVALID;
+    std::memset(result, 0, sizeof *result);
+    std::vector<hj_dtable*> dts(plan->num_inputs, nullptr);
+    int rc = HJ_OK;
+    for (uint64_t i = 0; i < plan->num_inputs && rc == HJ_OK; ++i)
+        rc = hj_table_upload(ctx, &plan->inputs[i], &dts[i]);
+    hj_dresult* dr = nullptr;
+    if (rc == HJ_OK) rc = hj_execute_device(ctx, plan, dts.data(), &dr);
+    if (rc == HJ_OK) rc = hj_dresult_to_host(ctx, dr, result);
+    std::string err = ctx->last_error;
+    if (dr) hj_dresult_free(ctx, dr);
+    for (auto* t : dts)
+        if (t) hj_table_free(ctx, t);
+    ctx->last_error = err;
+    return rc;
+}
+
+// ---------------------------------------------------------------- page codec
+int hj_encode_int32(const int32_t* values, const uint8_t* validity, uint64_t n, hj_out_column* out) {
+    if (!out || (n && !values)) return HJ_E_INVALID;
+    return guarded(nullptr, [&] {
+        encode_int32(values, [validity](uint64_t i) { return !validity || validity[i] != 0; }, n,
+                     out);
+    });
+}
+
+int hj_encode_varchar(const uint64_t* offsets, const char* chars, const uint8_t* validity,
+                      uint64_t n, hj_out_column* out) {
+    if (!out || !offsets) return HJ_E_INVALID;
+    return guarded(nullptr, [&] {
+        encode_varchar(offsets, chars, [validity](uint64_t i) { return !validity || validity[i] != 0; },
+                       n, out);
+    });
+}
+
+void hj_out_column_free(hj_out_column* col) {
+    if (!col) return;
+    std::free(col->pages);
+    col->pages = nullptr;
+    col->num_pages = 0;
+}
+
+uint64_t hj_column_num_rows(const hj_column* col) { return col ? column_rows(col) : 0; }
+uint64_t hj_column_num_chars(const hj_column* col) { return col ? column_chars(col) : 0; }
+int hj_decode_int32(const hj_column* col, int32_t* values, uint8_t* validity) {
+    if (!col) return HJ_E_INVALID;
+    return decode_int32(col, values, validity);
+}
+int hj_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint8_t* validity) {
+    if (!col) return HJ_E_INVALID;
+    return decode_varchar(col, offsets, chars, validity);
+}
+
+// ---------------------------------------------------------------- directory test surface
+int hj_directory_build(hj_context* ctx, const uint32_t* keys, uint64_t n, hj_directory** out) {
+    if (!ctx || !out || (n && !keys)) return HJ_E_INVALID;
+    *out = nullptr;
+    auto* d = new hj_directory();
+    int rc = guarded(ctx, [&] {
+        if (n >= (1ull << 32)) throw Error(HJ_E_INVALID, "too many keys");
+        auto alloc = [&](size_t b) -> void* {
+            void* p = nullptr;
+            HJ_HIP(hipMalloc(&p, b ? b : 256));
+            d->allocs.push_back(p);
+            return p;
+        };
+        d->n = n;
+        d->k = log2_pow2_ceil(n ? n : 1);
+        const uint64_t D = 1ull << d->k;
+        uint32_t* dk = (uint32_t*)alloc(n * 4);
+        uint32_t* h = (uint32_t*)alloc(n * 4);
+        uint32_t* row = (uint32_t*)alloc(n * 4);
+        uint32_t* th = (uint32_t*)alloc(n * 4);
+        uint32_t* tr = (uint32_t*)alloc(n * 4);
+        void* scr = alloc(radix_sort_scratch_bytes(n ? n : 1));
+        uint64_t* dir = (uint64_t*)alloc((D + 1) * 8);
+        uint32_t* e = (uint32_t*)alloc(D * 4);
+        uint32_t* bloom = (uint32_t*)alloc(D * 4);
+        uint32_t* part = (uint32_t*)alloc(scan_partials_needed(D) * 4);
+        if (n) HJ_HIP(hipMemcpyAsync(dk, keys, n * 4, hipMemcpyHostToDevice, ctx->stream));
+        hash_keys(dk, n, h, row, ctx->stream);
+        radix_sort_pairs(h, row, th, tr, n, scr, ctx->stream);
+        build_directory(h, n, d->k, dir, e, bloom, part, ctx->stream);
+        std::vector<uint64_t> hd(D + 1);
+        HJ_HIP(hipMemcpyAsync(hd.data(), dir, (D + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HJ_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint64_t s = 0; s < D; ++s)
+            d->max_slot = std::max<uint64_t>(d->max_slot, (hd[s + 1] >> 16) - (hd[s] >> 16));
+        d->view.dir = dir;
+        d->view.sh = h;
+        d->view.srow = row;
+        d->view.k = d->k;
+        d->view.pad = 0;
+        d->view.n = n;
+    });
+    if (rc) {
+        for (void* p : d->allocs) (void)hipFree(p);
+        delete d;
+        return rc;
+    }
+    *out = d;
+    return HJ_OK;
+}
+
+int hj_directory_probe(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
+                       uint64_t m, uint64_t* counts, uint64_t* first_row) {
+    if (!ctx || !dir || (m && (!probe_keys || !counts))) return HJ_E_INVALID;
+    return guarded(ctx, [&] {
+        if (!m) return;
+        uint32_t* dk = nullptr;
+        uint64_t *dc = nullptr, *df = nullptr;
+        HJ_HIP(hipMalloc(&dk, m * 4));
+        HJ_HIP(hipMalloc(&dc, m * 8));
+        HJ_HIP(hipMalloc(&df, m * 8));
+        HJ_HIP(hipMemcpyAsync(dk, probe_keys, m * 4, hipMemcpyHostToDevice, ctx->stream));
+        dir_probe_keys(dir->view, dk, m, dc, df, ctx->stream);
+        HJ_HIP(hipMemcpyAsync(counts, dc, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (first_row)
+            HJ_HIP(hipMemcpyAsync(first_row, df, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HJ_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(dk);
+        (void)hipFree(dc);
+        (void)hipFree(df);
+    });
+}
+
+int hj_directory_probe_pairs(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
+                             uint64_t m, uint64_t** pairs, uint64_t* num_pairs) {
+    if (!ctx || !dir || !pairs || !num_pairs || (m && !probe_keys)) return HJ_E_INVALID;
+    *pairs = nullptr;
+    *num_pairs = 0;
+    return guarded(ctx, [&] {
+        std::vector<uint64_t> counts(m);
+        if (m) {
+            int rc = hj_directory_probe(ctx, dir, probe_keys, m, counts.data(), nullptr);
+            if (rc) throw Error(rc, ctx->last_error);
+        }
+        std::vector<uint64_t> offs(m + 1, 0);
+        for (uint64_t i = 0; i < m; ++i) offs[i + 1] = offs[i] + counts[i];
+        const uint64_t total = offs[m];
+        uint64_t* host = (uint64_t*)std::malloc((total ? total : 1) * 16);
+        if (!host) throw std::bad_alloc();
+        if (total) {
+            uint32_t* dk = nullptr;
+            uint64_t *doff = nullptr, *dp = nullptr;
+            HJ_HIP(hipMalloc(&dk, m * 4));
+            HJ_HIP(hipMalloc(&doff, (m + 1) * 8));
+            HJ_HIP(hipMalloc(&dp, total * 16));
+            HJ_HIP(hipMemcpyAsync(dk, probe_keys, m * 4, hipMemcpyHostToDevice, ctx->stream));
+            HJ_HIP(hipMemcpyAsync(doff, offs.data(), (m + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+            dir_probe_pairs(dir->view, dk, m, doff, dp, ctx->stream);
+            HJ_HIP(hipMemcpyAsync(host, dp, total * 16, hipMemcpyDeviceToHost, ctx->stream));
+            HJ_HIP(hipStreamSynchronize(ctx->stream));
+            (void)hipFree(dk);
+            (void)hipFree(doff);
+            (void)hipFree(dp);
+        }
+        *pairs = host;
+        *num_pairs = total;
+    });
+}
+
+int hj_directory_info(const hj_directory* dir, uint64_t* num_slots, uint64_t* num_tuples,
+                      uint64_t* max_slot_len) {
+    if (!dir) return HJ_E_INVALID;
+    if (num_slots) *num_slots = 1ull << dir->k;
+    if (num_tuples) *num_tuples = dir->n;
+    if (max_slot_len) *max_slot_len = dir->max_slot;
+    return HJ_OK;
+}
+
+int hj_directory_free(hj_context* ctx, hj_directory* dir) {
+    if (!dir) return HJ_E_INVALID;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : dir->allocs) (void)hipFree(p);
+    delete dir;
+    return HJ_OK;
+}
+
+void hj_free(void* p) { std::free(p); }
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// hj_device.hpp - device-side data layout and helpers shared by the join kernels (gfx950, wave64).
+//
+// HBM layout of a resident input column (one per ColumnarTable column, include/plan.h:106-166):
+//   pages       num_pages * 8192 B, the reference's page bytes unchanged
+//   row_start   u32[num_pages + 1]   first global row of each page (0xffff page = 1 row,
+//                                    0xfffe continuation page = 0 rows), last = row count
+//   rank        u16[groups]          per 64-row group of a page with NULLs: the number of
+//                                    non-NULL rows of the page before the group
+//   group_start u32[num_pages]       index of the page's first group in rank
+// A row r is found by page search (or r / uniform_rows when every page but the last holds the
+// same number of rows, the dense 1984-row case of include/column_t.h:14), then its value index
+// among the page's packed non-NULL values is rank + popcount of the bitmap bits below it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HJ_PAGE_BYTES 8192u
+#define HJ_WAVE 64
+
+namespace hj {
+
+struct DCol {
+    const uint8_t* pages;
+    const uint32_t* row_start;
+    const uint16_t* rank;
+    const uint32_t* group_start;
+    uint32_t num_pages;
+    int32_t type;
+    uint32_t uniform_rows;
+    uint32_t all_dense;
+    uint64_t num_rows;
+};
+
+// ---------------------------------------------------------------- hashing
+// Directory prefix: 32-bit Fibonacci (multiplicative) hash, h = key * floor(2^32/phi). Odd
+// multiplier => a bijection on u32, so sorting tuples by h orders them by directory slot (the
+// top bits) and keeps equal keys adjacent; equal h <=> equal key, so the adjacency array stores
+// h alone. The Bloom filter follows include/unchained.h:107-128 (16-bit mask, one bit per nibble
+// of a 16-bit tag); the tag is bits 40..55 of h * floor(2^64/phi), because the low bits of a
+// multiplicative hash depend only on the low key bits (unchained.h:75-77 takes the low 16).
+__host__ __device__ inline uint32_t fib32(uint32_t key) { return key * 0x9E3779B9u; }
+__host__ __device__ inline uint16_t bloom_tag(uint32_t h) {
+    return (uint16_t)(((uint64_t)h * 0x9E3779B97F4A7C15ull) >> 40);
+}
+__host__ __device__ inline uint32_t bloom_mask(uint16_t s) {
+    return (1u << (s & 0xF)) | (1u << ((s >> 4) & 0xF)) | (1u << ((s >> 8) & 0xF)) |
+           (1u << ((s >> 12) & 0xF));
+}
+__host__ __device__ inline uint32_t slot_of(uint32_t h, uint32_t k) {
+    return k ? (h >> (32u - k)) : 0u;
+}
+
+// ---------------------------------------------------------------- page access
+__device__ inline const uint8_t* page_ptr(const DCol& c, uint32_t p) {
+    return c.pages + (size_t)p * HJ_PAGE_BYTES;
+}
+
+// last page p with row_start[p] <= r (never a 0-row page, see the layout note above)
+__device__ inline uint32_t find_page(const DCol& c, uint32_t r) {
+    if (c.uniform_rows) {
+        uint32_t p = r / c.uniform_rows;
+        return p < c.num_pages ? p : c.num_pages - 1;
+    }
+    uint32_t lo = 0, hi = c.num_pages;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (c.row_start[mid] <= r)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// the up-to-8 bitmap bytes [off, off+8) of a bitmap of nb bytes, little-endian
+__device__ inline uint64_t load_bits64(const uint8_t* bm, uint32_t nb, uint32_t off) {
+    uint64_t v = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+        if (off + k < nb) v |= (uint64_t)bm[off + k] << (8 * k);
+    return v;
+}
+
+// value index of local row `local` in page p (header word hdr = nrows | nnonnull << 16);
+// returns false for a NULL row
+__device__ inline bool value_index(const DCol& c, uint32_t p, const uint8_t* pg, uint32_t hdr,
+                                   uint32_t local, uint32_t* vi) {
+    uint32_t nr = hdr & 0xffffu, nn = hdr >> 16;
+    if (nn == nr) {
+        *vi = local;
+        return true;
+    }
+    uint32_t nb = (nr + 7) >> 3;
+    const uint8_t* bm = pg + HJ_PAGE_BYTES - nb;
+    if (!((bm[local >> 3] >> (local & 7)) & 1)) return false;
+    uint32_t g = local >> 6;
+    uint64_t bits = load_bits64(bm, nb, g * 8);
+    *vi = (uint32_t)c.rank[c.group_start[p] + g] +
+          (uint32_t)__popcll(bits & ((1ull << (local & 63)) - 1ull));
+    return true;
+}
+
+// INT32 value of global row r; false for NULL
+__device__ inline bool get_int32(const DCol& c, uint32_t r, int32_t* v) {
+    uint32_t p = find_page(c, r);
+    const uint8_t* pg = page_ptr(c, p);
+    uint32_t hdr = *(const uint32_t*)pg;
+    uint32_t vi;
+    if (!value_index(c, p, pg, hdr, r - c.row_start[p], &vi)) return false;
+    *v = *(const int32_t*)(pg + 4 + 4 * vi);
+    return true;
+}
+
+// INT32 value of local row `local` of a known page
+__device__ inline bool get_int32_in_page(const DCol& c, uint32_t p, const uint8_t* pg,
+                                         uint32_t hdr, uint32_t local, int32_t* v) {
+    uint32_t vi;
+    if (!value_index(c, p, pg, hdr, local, &vi)) return false;
+    *v = *(const int32_t*)(pg + 4 + 4 * vi);
+    return true;
+}
+
+// VARCHAR string of global row r: returns false for NULL; for a normal page *src points at the
+// bytes and *len is their count; for a long string (0xffff page) *first_page is set and *len is
+// the total over the page and its 0xfffe continuation pages.
+__device__ inline bool get_string(const DCol& c, uint32_t r, const uint8_t** src, uint32_t* len,
+                                  uint32_t* long_page) {
+    uint32_t p = find_page(c, r);
+    const uint8_t* pg = page_ptr(c, p);
+    uint32_t hdr = *(const uint32_t*)pg;
+    uint32_t nr = hdr & 0xffffu;
+    if (nr == 0xffffu) {
+        uint32_t total = hdr >> 16;
+        for (uint32_t q = p + 1; q < c.num_pages; ++q) {
+            uint32_t h2 = *(const uint32_t*)page_ptr(c, q);
+            if ((h2 & 0xffffu) != 0xfffeu) break;
+            total += h2 >> 16;
+        }
+        *src = pg + 4;
+        *len = total;
+        *long_page = p;
+        return true;
+    }
+    uint32_t vi;
+    if (!value_index(c, p, pg, hdr, r - c.row_start[p], &vi)) return false;
+    uint32_t nn = hdr >> 16;
+    const uint16_t* offs = (const uint16_t*)(pg + 4);
+    uint32_t start = vi ? offs[vi - 1] : 0u;
+    *src = pg + 4 + 2 * nn + start;
+    *len = (uint32_t)offs[vi] - start;
+    *long_page = 0xffffffffu;
+    return true;
+}
+
+// ---------------------------------------------------------------- wave / block primitives
+__device__ inline uint32_t lane_id() { return threadIdx.x & (HJ_WAVE - 1); }
+__device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+template <typename T>
+__device__ inline T wave_incl_scan(T x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < HJ_WAVE; d <<= 1) {
+        T y = __shfl_up(x, d, HJ_WAVE);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+template <typename T>
+__device__ inline T wave_sum(T x) {
+#pragma unroll
+    for (int d = HJ_WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, HJ_WAVE);
+    return x;
+}
+
+// exclusive scan over a block of BLOCK threads; lds needs BLOCK/64 entries; returns the
+// exclusive prefix and the block total. Contains two __syncthreads.
+template <int BLOCK, typename T>
+__device__ inline T block_excl_scan(T v, T* lds, T* total) {
+    constexpr int NW = BLOCK / HJ_WAVE;
+    const uint32_t w = threadIdx.x / HJ_WAVE;
+    T incl = wave_incl_scan(v);
+    if (lane_id() == HJ_WAVE - 1) lds[w] = incl;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        T t = lds[i];
+        off += (i < (int)w) ? t : (T)0;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+// block sum, result valid in every thread
+template <int BLOCK, typename T>
+__device__ inline T block_sum(T v, T* lds) {
+    constexpr int NW = BLOCK / HJ_WAVE;
+    T s = wave_sum(v);
+    if (lane_id() == 0) lds[threadIdx.x / HJ_WAVE] = s;
+    __syncthreads();
+    T tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) tot += lds[i];
+    __syncthreads();
+    return tot;
+}
+
+}  // namespace hj

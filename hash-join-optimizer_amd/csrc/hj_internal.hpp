@@ -1,0 +1,138 @@
+// hj_internal.hpp - host-side engine state: context, HBM arena, resident tables, device results.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hjoin.h"
+#include "hj_device.hpp"
+
+namespace hj {
+
+// Error carried through the engine; the C ABI turns it into a return code + message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HJ_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::hj::Error(e_ == hipErrorOutOfMemory ? HJ_E_OOM : HJ_E_HIP,               \
+                              std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+// One contiguous HBM region with two stacks: scratch grows up from the bottom (released by
+// mark/reset after each operator), results grow down from the top (live until the query ends).
+// This replaces the reference's three-level slab allocator (include/slab_allocator.h:15-107):
+// one hipMalloc per context, no per-join allocation, everything released at query end.
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t cap = 0;
+    size_t lo = 0;  // scratch top
+    size_t hi = 0;  // results bottom (offset from base)
+    size_t peak = 0;
+
+    void init(uint8_t* b, size_t c) {
+        base = b;
+        cap = c;
+        lo = 0;
+        hi = c;
+        peak = 0;
+    }
+    static size_t align(size_t n) { return (n + 255) & ~(size_t)255; }
+    void* scratch(size_t n) {
+        n = align(n ? n : 1);
+        if (lo + n > hi)
+            throw Error(HJ_E_OOM, "HBM arena exhausted (scratch " + std::to_string(n) +
+                                      " B, free " + std::to_string(hi - lo) + " B)");
+        void* p = base + lo;
+        lo += n;
+        if (lo + (cap - hi) > peak) peak = lo + (cap - hi);
+        return p;
+    }
+    void* result(size_t n) {
+        n = align(n ? n : 1);
+        if (lo + n > hi)
+            throw Error(HJ_E_OOM, "HBM arena exhausted (result " + std::to_string(n) +
+                                      " B, free " + std::to_string(hi - lo) + " B)");
+        hi -= n;
+        if (lo + (cap - hi) > peak) peak = lo + (cap - hi);
+        return base + hi;
+    }
+    size_t mark() const { return lo; }
+    void reset(size_t m) { lo = m; }
+    void clear() {
+        lo = 0;
+        hi = cap;
+    }
+    size_t free_bytes() const { return hi - lo; }
+};
+
+struct DevColumn {
+    int32_t type = HJ_INT32;
+    uint32_t num_pages = 0;
+    uint64_t num_rows = 0;
+    uint64_t num_chars = 0;        // VARCHAR: total string bytes
+    uint32_t max_len = 0;          // VARCHAR: longest string
+    uint8_t* pages = nullptr;      // device
+    uint32_t* row_start = nullptr; // device, num_pages + 1
+    uint16_t* rank = nullptr;      // device
+    uint32_t* group_start = nullptr;
+    uint32_t uniform_rows = 0;
+    uint32_t all_dense = 1;
+    std::vector<uint32_t> h_row_start;
+    DCol view() const {
+        DCol d;
+        d.pages = pages;
+        d.row_start = row_start;
+        d.rank = rank;
+        d.group_start = group_start;
+        d.num_pages = num_pages;
+        d.type = type;
+        d.uniform_rows = uniform_rows;
+        d.all_dense = all_dense;
+        d.num_rows = num_rows;
+        return d;
+    }
+};
+
+}  // namespace hj
+
+struct hj_dtable {
+    uint64_t num_rows = 0;
+    std::vector<hj::DevColumn> cols;
+    std::vector<void*> allocations;
+};
+
+// A device result: flat output columns in the arena's result stack.
+struct hj_dresult {
+    uint64_t num_rows = 0;
+    struct Col {
+        int32_t type;
+        int32_t* values = nullptr;    // INT32
+        uint64_t* offsets = nullptr;  // VARCHAR, n+1
+        char* chars = nullptr;        // VARCHAR
+        uint8_t* validity = nullptr;  // bitmap, bit i of byte i/8
+        uint64_t num_chars = 0;
+    };
+    std::vector<Col> cols;
+    bool live = false;
+};
+
+struct hj_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hj::Arena arena;
+    uint8_t* arena_mem = nullptr;
+    std::string last_error;
+    hj_dresult* live_result = nullptr;
+    // small pinned host staging for device->host scalars
+    uint64_t* pinned = nullptr;
+    int num_cus = 256;
+};

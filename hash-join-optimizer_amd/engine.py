@@ -1,0 +1,155 @@
+"""Host-side driver of the engine (libhjoin.so): the drop-in execute() and resident tables.
+
+    ctx = Context()                        # Contest::build_context (src/execute.cpp:610-612)
+    result = ctx.execute(plan)             # Contest::execute (src/execute.cpp:598-608)
+    ctx.close()                            # Contest::destroy_context (src/execute.cpp:614)
+
+Errors raise EngineError with the engine's message (e.g. "Join key isn't of type int32_t",
+the reference's message at src/execute.cpp:81). There is no CPU fallback: without the built
+library every call raises abi.EngineMissing.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .plan import ColumnarTable, result_from_c, table_to_c
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class DeviceTable:
+    """A ColumnarTable resident in HBM (hj_table_upload)."""
+
+    def __init__(self, ctx, handle, num_rows):
+        self.ctx = ctx
+        self.handle = handle
+        self.num_rows = num_rows
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.hj_table_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+class DeviceResult:
+    """A join result in HBM (hj_execute_device); valid until free()."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    @property
+    def num_rows(self):
+        return int(self.ctx.lib.hj_dresult_num_rows(self.handle))
+
+    def to_host(self):
+        res = abi.hj_result()
+        self.ctx._check(self.ctx.lib.hj_dresult_to_host(self.ctx.handle, self.handle, C.byref(res)))
+        return result_from_c(res, self.ctx.lib.hj_result_free)
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.hj_dresult_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+class Context:
+    def __init__(self, arena_bytes=0):
+        self.lib = abi.load()
+        h = C.c_void_p()
+        rc = self.lib.hj_context_create(arena_bytes, C.byref(h))
+        if rc:
+            raise EngineError(rc, self.lib.hj_global_error().decode())
+        self.handle = h
+
+    def _check(self, rc):
+        if rc:
+            raise EngineError(rc, self.lib.hj_last_error(self.handle).decode())
+
+    @property
+    def stream(self):
+        return self.lib.hj_context_stream(self.handle)
+
+    def execute(self, plan):
+        p, keep = plan.to_c()
+        res = abi.hj_result()
+        self._check(self.lib.hj_execute(self.handle, C.byref(p), C.byref(res)))
+        return result_from_c(res, self.lib.hj_result_free)
+
+    def upload(self, table: ColumnarTable):
+        keep = []
+        ht = table_to_c(table, keep)
+        h = C.c_void_p()
+        self._check(self.lib.hj_table_upload(self.handle, C.byref(ht), C.byref(h)))
+        return DeviceTable(self, h, table.num_rows)
+
+    def execute_device(self, plan, dtables):
+        p, keep = plan.to_c(with_inputs=False)
+        arr = (C.c_void_p * max(1, len(dtables)))(*[t.handle for t in dtables])
+        h = C.c_void_p()
+        self._check(self.lib.hj_execute_device(self.handle, C.byref(p), arr, C.byref(h)))
+        return DeviceResult(self, h)
+
+    def synchronize(self):
+        self._check(self.lib.hj_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.hj_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Directory:
+    """The GPU unchained directory of one key array (the design of include/unchained.h)."""
+
+    def __init__(self, ctx, keys):
+        self.ctx = ctx
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        h = C.c_void_p()
+        ctx._check(ctx.lib.hj_directory_build(ctx.handle, k.ctypes.data if len(k) else None,
+                                              len(k), C.byref(h)))
+        self.handle = h
+
+    def probe(self, keys):
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        cnt = np.zeros(max(1, len(k)), dtype=np.uint64)
+        first = np.zeros(max(1, len(k)), dtype=np.uint64)
+        self.ctx._check(self.ctx.lib.hj_directory_probe(
+            self.ctx.handle, self.handle, k.ctypes.data if len(k) else None, len(k),
+            cnt.ctypes.data, first.ctypes.data))
+        return cnt[:len(k)], first[:len(k)]
+
+    def probe_pairs(self, keys):
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        ptr = C.c_void_p()
+        n = C.c_uint64()
+        self.ctx._check(self.ctx.lib.hj_directory_probe_pairs(
+            self.ctx.handle, self.handle, k.ctypes.data if len(k) else None, len(k),
+            C.byref(ptr), C.byref(n)))
+        out = np.zeros((n.value, 2), dtype=np.uint64)
+        if n.value:
+            C.memmove(out.ctypes.data, ptr, n.value * 16)
+        self.ctx.lib.hj_free(ptr)
+        return out
+
+    def info(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.lib.hj_directory_info(self.handle, C.byref(a), C.byref(b), C.byref(c))
+        return {"num_slots": a.value, "num_tuples": b.value, "max_slot_len": c.value}
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.hj_directory_free(self.ctx.handle, self.handle)
+            self.handle = None

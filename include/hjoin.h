@@ -1,0 +1,195 @@
+/*
+ * hjoin.h - C ABI of the MI355X multi-way INT32 equi-join engine.
+ *
+ * The engine is a drop-in for the reference's `Contest::execute(const Plan&, void*)`
+ * (include/plan.h:398-405, src/execute.cpp:598-614). Every type below is a plain-C
+ * restatement of the reference's data model (include/plan.h:73-210, include/attribute.h:8-13)
+ * so that C, C++, Python (ctypes) or any FFI can drive it. No torch types, no C++ types.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative HJ_E* code. The message of the
+ *     last failure on a context is returned by hj_last_error(ctx) (or hj_global_error()
+ *     when no context exists yet). Messages match the reference's where one exists
+ *     ("Join key isn't of type int32_t", src/execute.cpp:81).
+ *   - Pages are the reference's 8192-byte pages (include/plan.h:100-104), given as an
+ *     array of page pointers exactly like Column::pages (include/plan.h:106-109).
+ *   - All sizes are 64-bit. Row ids inside the engine are 32-bit (a base table holds < 2^32 rows).
+ *   - One context = one HIP stream + one HBM arena on the current device. Calls on one
+ *     context must not overlap; different contexts may be driven from different threads.
+ */
+#ifndef HJOIN_H
+#define HJOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HJ_PAGE_SIZE 8192u
+
+/* DataType, same order as include/attribute.h:8-13 */
+enum { HJ_INT32 = 0, HJ_INT64 = 1, HJ_FP64 = 2, HJ_VARCHAR = 3 };
+
+/* PlanNode kind (std::variant<ScanNode, JoinNode> index order of include/plan.h:91) */
+enum { HJ_NODE_SCAN = 0, HJ_NODE_JOIN = 1 };
+
+/* error codes */
+enum {
+    HJ_OK = 0,
+    HJ_E_INVALID = -1,     /* malformed plan / argument */
+    HJ_E_TYPE = -2,        /* non-INT32 join key, unsupported column type */
+    HJ_E_OOM = -3,         /* arena / device allocation exhausted */
+    HJ_E_HIP = -4,         /* a HIP runtime call failed */
+    HJ_E_RANGE = -5,       /* StrRef-style field overflow (include/value_t.h:36-47) */
+    HJ_E_BUSY = -6,        /* serving admission refused (memory reservation) */
+    HJ_E_NODEV = -7        /* no usable GPU */
+};
+
+/* Column (include/plan.h:106-161): type + pages. */
+typedef struct hj_column {
+    int32_t type;
+    uint32_t num_pages;
+    const uint8_t* const* pages; /* num_pages pointers to HJ_PAGE_SIZE-byte pages */
+} hj_column;
+
+/* ColumnarTable (include/plan.h:163-166) */
+typedef struct hj_table {
+    uint64_t num_rows;
+    uint32_t num_columns;
+    uint32_t reserved;
+    const hj_column* columns;
+} hj_table;
+
+/* one entry of PlanNode::output_attrs (include/plan.h:92): (index, DataType) */
+typedef struct hj_attr {
+    uint64_t index;
+    int32_t type;
+    int32_t reserved;
+} hj_attr;
+
+/* Engine extension with no counterpart in Plan: a load-time range filter pushed into a scan.
+ * Keeps a row iff column `column` (index into the base table) is non-NULL and lo <= v <= hi,
+ * the semantics of `attr BETWEEN lo AND hi` evaluated by the reference before execute()
+ * (tests/read_sql.cpp:585-591, include/inner_column.h:258-259, src/build_table.cpp:319-371). */
+typedef struct hj_filter {
+    uint32_t column;
+    int32_t lo;
+    int32_t hi;
+    int32_t reserved;
+} hj_filter;
+
+/* PlanNode (include/plan.h:78-98). kind selects ScanNode or JoinNode fields. */
+typedef struct hj_node {
+    int32_t kind;
+    int32_t build_left;          /* JoinNode::build_left */
+    uint64_t base_table_id;      /* ScanNode::base_table_id */
+    uint64_t left, right;        /* JoinNode::left/right (node indices) */
+    uint64_t left_attr, right_attr;
+    uint64_t num_output_attrs;
+    const hj_attr* output_attrs;
+    uint64_t num_filters;        /* scan only; 0 for a plain reference plan */
+    const hj_filter* filters;
+} hj_node;
+
+/* Plan (include/plan.h:173-210) */
+typedef struct hj_plan {
+    uint64_t num_nodes;
+    const hj_node* nodes;
+    uint64_t num_inputs;
+    const hj_table* inputs;
+    uint64_t root;
+} hj_plan;
+
+/* Result column/table: pages are one contiguous malloc'd block of num_pages*HJ_PAGE_SIZE bytes. */
+typedef struct hj_out_column {
+    int32_t type;
+    uint32_t num_pages;
+    uint8_t* pages;
+} hj_out_column;
+
+typedef struct hj_result {
+    uint64_t num_rows;
+    uint32_t num_columns;
+    uint32_t reserved;
+    hj_out_column* columns;
+} hj_result;
+
+typedef struct hj_context hj_context;     /* HIP stream + HBM arena + resident tables */
+typedef struct hj_dtable hj_dtable;       /* a ColumnarTable resident in HBM */
+typedef struct hj_dresult hj_dresult;     /* a join result resident in HBM (flat columns) */
+
+/* ---- context: replaces Contest::build_context / destroy_context (src/execute.cpp:610-614) ---- */
+/* arena_bytes = 0 picks the default (env HJ_ARENA_BYTES, else 16 GiB). */
+int hj_context_create(uint64_t arena_bytes, hj_context** out);
+int hj_context_destroy(hj_context* ctx);
+const char* hj_last_error(const hj_context* ctx);
+const char* hj_global_error(void);
+
+/* ---- one-call drop-in: replaces Contest::execute (include/plan.h:403, src/execute.cpp:598-608).
+ * Uploads the plan's inputs, runs every scan/join on the GPU, materializes the root's
+ * output_attrs into pages. result must be released with hj_result_free. ---- */
+int hj_execute(hj_context* ctx, const hj_plan* plan, hj_result* result);
+void hj_result_free(hj_result* result);
+
+/* ---- resident inputs: the ColumnarTable pages of plan.inputs kept in HBM across calls.
+ * Replaces the reference's borrowed, read-only plan.inputs (include/plan.h:175). ---- */
+int hj_table_upload(hj_context* ctx, const hj_table* table, hj_dtable** out);
+int hj_table_free(hj_context* ctx, hj_dtable* table);
+uint64_t hj_table_num_rows(const hj_dtable* table);
+
+/* Run the plan over resident tables (tables[i] stands for plan->inputs[i]; plan->inputs may be
+ * NULL). The result stays in HBM as flat columns; it is valid until hj_dresult_free. The
+ * device part of execute(): scan/filter, build, probe, late materialization (src/execute.cpp:35-185,
+ * 265-369, 431-573 minus the page encoding). */
+int hj_execute_device(hj_context* ctx, const hj_plan* plan, const hj_dtable* const* tables,
+                      hj_dresult** out);
+uint64_t hj_dresult_num_rows(const hj_dresult* r);
+/* copy the device result to host and encode it into pages with the reference's writer rules
+ * (src/execute.cpp:431-573) */
+int hj_dresult_to_host(hj_context* ctx, const hj_dresult* r, hj_result* result);
+/* device pointers of output column c: INT32 -> values (int32[n]) and validity bitmap (bit i of
+ * byte i/8); VARCHAR -> values = uint64 offsets[n+1] into chars, chars, validity. */
+int hj_dresult_column(const hj_dresult* r, uint32_t c, int32_t* type, const void** values,
+                      const void** chars, const uint8_t** validity);
+int hj_dresult_free(hj_context* ctx, hj_dresult* r);
+/* stream the context launches on (hipStream_t), for event timing by the caller */
+void* hj_context_stream(hj_context* ctx);
+int hj_synchronize(hj_context* ctx);
+
+/* ---- page codec: ColumnInserter / Table::to_columnar writer rules (include/plan.h:212-396,
+ * src/build_table.cpp:657-882) and Table::from_columnar reader rules (src/build_table.cpp:513-637).
+ * validity: NULL means all rows valid; otherwise byte i != 0 means row i is valid. ---- */
+int hj_encode_int32(const int32_t* values, const uint8_t* validity, uint64_t n, hj_out_column* out);
+/* offsets: n+1 entries into chars (row i = chars[offsets[i], offsets[i+1])) */
+int hj_encode_varchar(const uint64_t* offsets, const char* chars, const uint8_t* validity,
+                      uint64_t n, hj_out_column* out);
+void hj_out_column_free(hj_out_column* col);
+/* decode: rows counted over the column's pages; values/validity sized by the caller from
+ * hj_column_num_rows. VARCHAR: offsets (n+1) and chars sized by hj_column_num_chars. */
+uint64_t hj_column_num_rows(const hj_column* col);
+uint64_t hj_column_num_chars(const hj_column* col);
+int hj_decode_int32(const hj_column* col, int32_t* values, uint8_t* validity);
+int hj_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint8_t* validity);
+
+/* ---- directory (hash index) test surface: the GPU unchained directory of one key column,
+ * the design of include/unchained.h:14-247. Used by the ported unchained_tests. ---- */
+typedef struct hj_directory hj_directory;
+int hj_directory_build(hj_context* ctx, const uint32_t* keys, uint64_t n, hj_directory** out);
+/* for each probe key: count of build rows with an equal key and the smallest build row id among
+ * them (UINT64_MAX when count == 0); row ids are positions in the build key array */
+int hj_directory_probe(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
+                       uint64_t m, uint64_t* counts, uint64_t* first_row);
+/* all matches of probe key i as (probe index, build row id) pairs, sorted by probe index */
+int hj_directory_probe_pairs(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
+                             uint64_t m, uint64_t** pairs, uint64_t* num_pairs);
+int hj_directory_info(const hj_directory* dir, uint64_t* num_slots, uint64_t* num_tuples,
+                      uint64_t* max_slot_len);
+int hj_directory_free(hj_context* ctx, hj_directory* dir);
+void hj_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HJOIN_H */

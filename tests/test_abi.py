@@ -1,0 +1,50 @@
+"""The C ABI library loads on a machine without a GPU and exports every entry point that
+include/*.h declares; the Python ctypes layouts match the C struct sizes."""
+import ctypes as C
+import os
+import re
+
+import hjtest
+
+ROOT = hjtest.ROOT
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[\w\*]+\s+)+\**(hj_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_declared_symbols(hjo):
+    lib = hjo.abi.load()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(hjo.abi.EXPORTED) == names
+
+
+def test_struct_layouts(hjo):
+    a = hjo.abi
+    assert C.sizeof(a.hj_column) == 16
+    assert C.sizeof(a.hj_table) == 24
+    assert C.sizeof(a.hj_attr) == 16
+    assert C.sizeof(a.hj_filter) == 16
+    assert C.sizeof(a.hj_node) == 80
+    assert C.sizeof(a.hj_plan) == 40
+    assert C.sizeof(a.hj_out_column) == 16
+    assert C.sizeof(a.hj_result) == 24
+
+
+def test_no_engine_fallback(hjo, tmp_path):
+    """loading a missing library fails loudly instead of falling back to anything"""
+    import pytest
+    with pytest.raises(hjo.EngineMissing):
+        hjo.abi.load(str(tmp_path / "absent.so"))
