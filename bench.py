@@ -1,0 +1,220 @@
+"""Benchmark: one step = one execute of the 4-way star join (fact 36M rows x 3 Zipf(1.1) FKs,
+dimensions 2.5M / 4M / 100K rows with range filters pushed into their scans, VARCHAR output)
+on one MI355X, inputs resident in HBM, output materialized in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload star4|two_way]
+
+Prints one JSON line. value = join tuples/s = sum over joins of (|build_j| + |probe_j|) / step
+time; pct_hbm_peak = algorithmic bytes / step time / 8 TB/s (keys and filter attributes read once,
+validity bits, output columns gathered and written once). The roofline object is the dominant
+kernel timed with HIP events on the engine's stream during the timed steps. cpu_baseline is the
+oracle (a CPU restatement of the reference's execute()) on the host, timed around execute() with
+pre-filtered dimensions, as the reference filters before execute().
+"""
+import argparse
+import ctypes as C
+import importlib
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+hjo = importlib.import_module("hash-join-optimizer_amd")
+dg = importlib.import_module("hash-join-optimizer_amd.datagen")
+
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ workloads
+def star4_workload(codec, scale):
+    n_fact = int(36_000_000 * scale)
+    dims = tuple(max(1000, int(x * scale)) for x in dg.STAR_DIMS)
+    t = time.time()
+    d = dg.star4(seed=43, n_fact=n_fact, dims=dims)
+    tables = dg.star4_tables(d, codec)
+    plan = dg.star4_plan(d, tables)
+    log(f"star4 generated+encoded in {time.time() - t:.1f}s")
+    # per-join sizes (exact, from numpy lookups)
+    n = n_fact
+    ok = np.ones(n, dtype=bool)
+    builds, probes, outs = [], [n], []
+    strlens = []
+    for fk, v, dm in zip(d["fact"], d["fact_valid"], d["dims"]):
+        inv = np.empty(len(dm["key"]), dtype=np.int64)
+        inv[dm["key"]] = np.arange(len(dm["key"]))
+        r = inv[fk]
+        passed = (dm["attr"][r] >= dm["lo"]) & (dm["attr"][r] <= dm["hi"])
+        ok &= (v == 1) & passed
+        builds.append(int(((dm["attr"] >= dm["lo"]) & (dm["attr"] <= dm["hi"])).sum()))
+        outs.append(int(ok.sum()))
+        strlens.append((r, dm))
+    probes += outs[:-1]
+    rows = np.nonzero(ok)[0]
+    out_rows = len(rows)
+    str_bytes = 0
+    for r, dm in strlens:
+        rr = r[rows]
+        str_bytes += int((dm["soff"][rr + 1] - dm["soff"][rr]).sum()) + 2 * out_rows
+    key_bytes = sum(4 * int(v.sum()) + (n + 7) // 8 for v in d["fact_valid"])  # packed values+bits
+    dim_bytes = sum(8 * len(dm["key"]) for dm in d["dims"])                      # key + attr
+    gathered = out_rows * 3 * 4 + str_bytes
+    b_alg = key_bytes + dim_bytes + 2 * gathered
+    info = dict(builds=builds, probes=probes, outs=outs, out_rows=out_rows, b_alg=b_alg,
+                fk_bytes=[4 * int(v.sum()) + (n + 7) // 8 for v in d["fact_valid"]])
+    workload = {"workload": "4-way star join ((F x sD1) x sD2) x sD3",
+                "fact_rows": n_fact, "dim_rows": list(dims), "fk_dist": "zipf(1.1)",
+                "fk_null": 0.03, "filters": [[dm["lo"], dm["hi"]] for dm in d["dims"]],
+                "filtered_dim_rows": builds, "output_rows": out_rows,
+                "output": "3 fact INT32 FKs + 3 dimension VARCHARs", "seed": 43}
+    return d, tables, plan, info, workload
+
+
+def two_way_workload(codec, scale):
+    d = dg.two_way(seed=42, n_build=int(1_000_000 * scale), n_probe=int(10_000_000 * scale))
+    plan = dg.two_way_plan(d, codec)
+    out_rows = dg.two_way_expected_count(d)
+    nb, np_ = len(d["bk"]), len(d["pk"])
+    b_alg = 4 * nb + 4 * int(d["pv"].sum()) + (np_ + 7) // 8 + 2 * out_rows * 8
+    info = dict(builds=[nb], probes=[np_], outs=[out_rows], out_rows=out_rows, b_alg=b_alg,
+                fk_bytes=[4 * int(d["pv"].sum()) + (np_ + 7) // 8])
+    workload = {"workload": "2-way INT32 join 1M unique x 10M FK (5% NULL)", "seed": 42,
+                "output_rows": out_rows}
+    return d, plan.inputs, plan, info, workload
+
+
+# ------------------------------------------------------------------ kernel byte model
+def kernel_bytes(name, info, launches):
+    """algorithmic HBM bytes of all launches of one step of kernel `name` (None: no model)"""
+    fk1 = info["fk_bytes"][0]
+    outs, probes = info["outs"], info["probes"]
+    if name == "probe_scan_count":
+        return fk1
+    if name == "probe_scan_write":
+        return fk1 + outs[0] * 2 * 4
+    if name == "probe_rel_count":
+        return sum(p * 8 for p in probes[1:])          # row id + gathered key per probe row
+    if name == "probe_rel_write":
+        return sum(p * 8 for p in probes[1:]) + sum(o * 4 * (j + 2) for j, o in enumerate(outs[1:]))
+    return None
+
+
+# ------------------------------------------------------------------ CPU baseline (oracle)
+def cpu_baseline(d, workload_name, tuples, threads):
+    """the oracle (CPU restatement of the reference's execute()), used only as the baseline"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import hjtest  # test infrastructure: loads oracle/liboracle.so
+    o = hjtest.Oracle()
+    if workload_name == "star4":
+        tabs = dg.star4_prefiltered_tables(d, o.codec)
+        plan = dg.star4_plan(d, tabs, prefiltered=True)
+        sample = "full star4 workload; dimensions pre-filtered (untimed, as the reference filters at load time)"
+    else:
+        plan = dg.two_way_plan(d, o.codec)
+        sample = "full two_way workload"
+    times = []
+    for _ in range(3):
+        ct, ns = o.execute(plan, threads=threads, timed=True)
+        times.append(ns / 1e9)
+        log(f"cpu oracle execute: {ns / 1e6:.1f} ms, {ct.num_rows} rows")
+    t = statistics.median(times)
+    return {"value": tuples / t, "unit": "join tuples/s", "cores": threads, "kind": "port",
+            "sample": sample + f"; median of 3 execute() calls, {t * 1e3:.1f} ms each",
+            "ms": t * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="star4", choices=["star4", "two_way"])
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the join engine has no "
+              f"multi-GPU path in this build", file=sys.stderr)
+        sys.exit(2)
+
+    codec = hjo.Codec()
+    if args.workload == "star4":
+        d, tables, plan, info, workload = star4_workload(codec, args.scale)
+    else:
+        d, tables, plan, info, workload = two_way_workload(codec, args.scale)
+    tuples = sum(info["builds"]) + sum(info["probes"])
+
+    ctx = hjo.Context()
+    t = time.time()
+    dts = [ctx.upload(tb) for tb in tables]
+    log(f"uploaded {len(dts)} tables in {time.time() - t:.1f}s")
+
+    # warmup with every kernel timed, to find the dominant one
+    ctx.profile("")
+    for _ in range(args.warmup):
+        r = ctx.execute_device(plan, dts)
+        got = r.num_rows
+        r.free()
+    assert got == info["out_rows"], f"output rows {got} != expected {info['out_rows']}"
+    prof = ctx.profile_report()
+    per_step = {k: (n / args.warmup, ms / args.warmup) for k, (n, ms) in prof.items()}
+    for k, (n, ms) in sorted(per_step.items(), key=lambda x: -x[1][1]):
+        log(f"  {k:22s} {n:5.1f} launches/step {ms:8.3f} ms/step")
+    modeled = [k for k in per_step if kernel_bytes(k, info, 1) is not None]
+    dom = max(modeled or per_step, key=lambda k: per_step[k][1])
+
+    # timed region: only the dominant kernel is bracketed by events
+    ctx.profile(dom)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.execute_device(plan, dts)
+        r.free()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    prof = ctx.profile_report()
+    ctx.profile(None)
+    step_s = (t1 - t0) / args.steps
+
+    launches, dom_ms = prof.get(dom, (0, 0.0))
+    roofline = None
+    kb = kernel_bytes(dom, info, launches / max(1, args.steps))
+    if launches and kb is not None:
+        per_launch_bytes = kb / (launches / args.steps)
+        avg_s = dom_ms / launches / 1e3
+        achieved = per_launch_bytes / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
+                    "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_s * 1e3}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(d, args.workload, tuples, threads)
+
+    out = {
+        "metric": "join tuples/s", "value": tuples / step_s, "unit": "tuples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic, seeded", "config": workload,
+        "pct_hbm_peak": 100.0 * info["b_alg"] / step_s / HBM_PEAK,
+        "alg_bytes_per_step": info["b_alg"], "output_rows_per_s": info["out_rows"] / step_s,
+        "kernel_ms_per_step": {k: v[1] for k, v in per_step.items()},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    for t_ in dts:
+        t_.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -245,18 +245,18 @@ class Executor {
                 uint32_t* part = scratch<uint32_t>(scan_partials_needed(nblk));
                 uint32_t* tot = scratch<uint32_t>(1);
                 if (sc)
-                    extract_scan_count(ss, counts, s_);
+                    { Prof p_(ctx_, "extract_scan_count"); extract_scan_count(ss, counts, s_); }
                 else
-                    extract_rel_count(rs, counts, s_);
-                scan_excl_sum_u32(counts, offs, nblk, part, tot, s_);
+                    { Prof p_(ctx_, "extract_rel_count"); extract_rel_count(rs, counts, s_); }
+                { Prof p_(ctx_, "scan"); scan_excl_sum_u32(counts, offs, nblk, part, tot, s_); }
                 nb = fetch(ctx_, tot);
                 if (nb) {
                     bh = scratch<uint32_t>(nb);
                     brow = scratch<uint32_t>(nb);
                     if (sc)
-                        extract_scan_write(ss, offs, bh, brow, s_);
+                        { Prof p_(ctx_, "extract_scan_write"); extract_scan_write(ss, offs, bh, brow, s_); }
                     else
-                        extract_rel_write(rs, offs, bh, brow, s_);
+                        { Prof p_(ctx_, "extract_rel_write"); extract_rel_write(rs, offs, bh, brow, s_); }
                 }
             }
         }
@@ -268,7 +268,7 @@ class Executor {
             uint32_t* th = scratch<uint32_t>(nb);
             uint32_t* tr = scratch<uint32_t>(nb);
             void* sortscr = A_.scratch(radix_sort_scratch_bytes(nb));
-            radix_sort_pairs(bh, brow, th, tr, nb, sortscr, s_);
+            { Prof p_(ctx_, "radix_sort_pairs"); radix_sort_pairs(bh, brow, th, tr, nb, sortscr, s_); }
         }
         const uint32_t k = log2_pow2_ceil(nb);
         const uint64_t D = 1ull << k;
@@ -277,7 +277,7 @@ class Executor {
             uint32_t* e = scratch<uint32_t>(D);
             uint32_t* bloom = scratch<uint32_t>(D);
             uint32_t* part = scratch<uint32_t>(scan_partials_needed(D));
-            build_directory(bh, nb, k, dir, e, bloom, part, s_);
+            { Prof p_(ctx_, "build_directory"); build_directory(bh, nb, k, dir, e, bloom, part, s_); }
         }
         DirView dv;
         dv.dir = dir;
@@ -307,10 +307,10 @@ class Executor {
             uint64_t* part = scratch<uint64_t>(scan_partials_needed(pblk));
             uint64_t* tot = scratch<uint64_t>(1);
             if (psc)
-                probe_scan_count(pss, dv, counts, s_);
+                { Prof p_(ctx_, "probe_scan_count"); probe_scan_count(pss, dv, counts, s_); }
             else
-                probe_rel_count(prs, dv, counts, s_);
-            scan_excl_sum_u64(counts, poffs, pblk, part, tot, s_);
+                { Prof p_(ctx_, "probe_rel_count"); probe_rel_count(prs, dv, counts, s_); }
+            { Prof p_(ctx_, "scan"); scan_excl_sum_u64(counts, poffs, pblk, part, tot, s_); }
             total = fetch(ctx_, tot);
         }
         if (total >= (1ull << 32))
@@ -336,10 +336,10 @@ class Executor {
         uint32_t* hcount = scratch<uint32_t>(1);
         HJ_HIP(hipMemsetAsync(hcount, 0, sizeof(uint32_t), s_));
         if (psc)
-            probe_scan_write(pss, dv, poffs, o, heavy, hcount, s_);
+            { Prof p_(ctx_, "probe_scan_write"); probe_scan_write(pss, dv, poffs, o, heavy, hcount, s_); }
         else
-            probe_rel_write(prs, dv, poffs, o, heavy, hcount, s_);
-        expand_heavy(dv, o, heavy, hcount, s_);
+            { Prof p_(ctx_, "probe_rel_write"); probe_rel_write(prs, dv, poffs, o, heavy, hcount, s_); }
+        { Prof p_(ctx_, "expand_heavy"); expand_heavy(dv, o, heavy, hcount, s_); }
         // the scratch (build side, directory, counts) is reused by the next operator; the
         // stream orders the reuse after these kernels
         A_.reset(mark);
@@ -369,11 +369,11 @@ class Executor {
                 uint32_t* offs = scratch<uint32_t>(nblk);
                 uint32_t* part = scratch<uint32_t>(scan_partials_needed(nblk));
                 uint32_t* tot = scratch<uint32_t>(1);
-                select_scan_count(ss, counts, s_);
-                scan_excl_sum_u32(counts, offs, nblk, part, tot, s_);
+                { Prof p_(ctx_, "select_scan_count"); select_scan_count(ss, counts, s_); }
+                { Prof p_(ctx_, "scan"); scan_excl_sum_u32(counts, offs, nblk, part, tot, s_); }
                 n = fetch(ctx_, tot);
                 rows = result<uint32_t>(n ? n : 1);
-                if (n) select_scan_write(ss, offs, rows, s_);
+                if (n) { Prof p_(ctx_, "select_scan_write"); select_scan_write(ss, offs, rows, s_); }
             }
             r.n = n;
             r.slots[0].rid = rows;
@@ -394,7 +394,7 @@ class Executor {
             if (dc.type == HJ_INT32) {
                 c.values = result<int32_t>(n ? n : 1);
                 c.validity = result<uint8_t>(vbytes ? vbytes : 8);
-                gather_int32(dc.view(), sl.rid, n, c.values, c.validity, s_);
+                { Prof p_(ctx_, "gather_int32"); gather_int32(dc.view(), sl.rid, n, c.values, c.validity, s_); }
             } else if (dc.type == HJ_VARCHAR) {
                 c.offsets = result<uint64_t>(n + 1);
                 c.validity = result<uint8_t>(vbytes ? vbytes : 8);
@@ -402,15 +402,15 @@ class Executor {
                     const size_t mark = A_.mark();
                     uint64_t* lens = scratch<uint64_t>(n);
                     uint64_t* part = scratch<uint64_t>(scan_partials_needed(n));
-                    gather_strlen(dc.view(), sl.rid, n, lens, c.validity, s_);
-                    scan_excl_sum_u64(lens, c.offsets, n, part, c.offsets + n, s_);
+                    { Prof p_(ctx_, "gather_strlen"); gather_strlen(dc.view(), sl.rid, n, lens, c.validity, s_); }
+                    { Prof p_(ctx_, "scan"); scan_excl_sum_u64(lens, c.offsets, n, part, c.offsets + n, s_); }
                     c.num_chars = fetch(ctx_, c.offsets + n);
                     A_.reset(mark);
                 } else {
                     HJ_HIP(hipMemsetAsync(c.offsets, 0, sizeof(uint64_t), s_));
                 }
                 c.chars = result<char>(c.num_chars ? c.num_chars : 1);
-                gather_strcopy(dc.view(), sl.rid, n, c.offsets, c.chars, s_);
+                { Prof p_(ctx_, "gather_strcopy"); gather_strcopy(dc.view(), sl.rid, n, c.offsets, c.chars, s_); }
             } else {
                 throw Error(HJ_E_TYPE, "unsupported output column type (only INT32 and VARCHAR)");
             }
@@ -647,6 +647,11 @@ int hj_context_destroy(hj_context* ctx) {
     if (!ctx) return HJ_E_INVALID;
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->live_result) delete ctx->live_result;
+    for (auto& r : ctx->prof) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->arena_mem) (void)hipFree(ctx->arena_mem);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -655,6 +660,45 @@ int hj_context_destroy(hj_context* ctx) {
 }
 
 void* hj_context_stream(hj_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int hj_profile_enable(hj_context* ctx, const char* prefix) {
+    if (!ctx) return HJ_E_INVALID;
+    ctx->prof_on = prefix != nullptr;
+    ctx->prof_prefix = prefix ? prefix : "";
+    return HJ_OK;
+}
+
+int hj_profile_report(hj_context* ctx, char* buf, uint64_t len) {
+    if (!ctx || !buf || !len) return HJ_E_INVALID;
+    return guarded(ctx, [&] {
+        HJ_HIP(hipStreamSynchronize(ctx->stream));
+        std::map<std::string, std::pair<uint64_t, double>> agg;
+        std::vector<std::string> order;
+        for (auto& r : ctx->prof) {
+            float ms = 0.f;
+            HJ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            auto it = agg.find(r.name);
+            if (it == agg.end()) {
+                order.push_back(r.name);
+                agg[r.name] = {1, ms};
+            } else {
+                it->second.first += 1;
+                it->second.second += ms;
+            }
+            ctx->event_pool.push_back(r.a);
+            ctx->event_pool.push_back(r.b);
+        }
+        ctx->prof.clear();
+        std::string out;
+        char line[256];
+        for (auto& n : order) {
+            std::snprintf(line, sizeof line, "%s %llu %.6f\n", n.c_str(),
+                          (unsigned long long)agg[n].first, agg[n].second);
+            out += line;
+        }
+        std::snprintf(buf, len, "%s", out.c_str());
+    });
+}
 
 int hj_synchronize(hj_context* ctx) {
     if (!ctx) return HJ_E_INVALID;

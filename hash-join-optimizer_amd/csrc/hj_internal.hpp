@@ -135,4 +135,44 @@ struct hj_context {
     // small pinned host staging for device->host scalars
     uint64_t* pinned = nullptr;
     int num_cus = 256;
+    // kernel timing (hj_profile_enable)
+    bool prof_on = false;
+    std::string prof_prefix;
+    struct ProfRec {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> event_pool;
 };
+
+namespace hj {
+// brackets the launches issued during its lifetime with events when the name is profiled
+struct Prof {
+    hj_context* ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    static hipEvent_t get(hj_context* c) {
+        if (!c->event_pool.empty()) {
+            hipEvent_t e = c->event_pool.back();
+            c->event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    Prof(hj_context* c, const char* n) : ctx(c), name(n) {
+        if (!ctx->prof_on || std::strncmp(n, ctx->prof_prefix.c_str(), ctx->prof_prefix.size()))
+            return;
+        a = get(ctx);
+        b = get(ctx);
+        if (a) (void)hipEventRecord(a, ctx->stream);
+    }
+    ~Prof() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->prof.push_back({name, a, b});
+    }
+};
+}  // namespace hj

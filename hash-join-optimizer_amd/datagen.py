@@ -1,0 +1,189 @@
+"""Seeded synthetic inputs for the benchmark configurations (numpy PCG64 generators).
+
+  two_way      1M unique shuffled build keys + INT32 payload  JOIN  10M probe rows, FK ~ U[0, 2M),
+               5 % NULL FKs, INT32 payload; output (build payload, probe payload). Seed 42.
+  star4        fact of 36M rows with three INT32 FKs ~ Zipf(1.1) over dimensions of 2.5M / 4M /
+               100K rows (3 % NULL each); each dimension: unique shuffled keys, an INT32 attribute
+               ~ U[0, 10000) with a range filter of 1-10 % selectivity, a VARCHAR of 8-64 printable
+               ASCII chars. Plan ((F x sD1) x sD2) x sD3, builds on the filtered dimensions,
+               output = the 3 fact FKs + the 3 dimension strings.
+  many_to_many 50M build keys ~ Zipf(1.3) over [0, 5M) and 200M probe keys ~ Zipf(1.3) with an
+               independent rank->key permutation, 10 % NULL probe keys, INT32 payloads.
+
+Zipf(s) over a finite domain [0, N) is sampled by inverse CDF over ranks 1..N (weights r^-s),
+followed by a seeded Fisher-Yates rank->key permutation per column; NULLs are Bernoulli per row.
+Every generator returns plain numpy arrays; build_*_plan turns them into a Plan.
+"""
+import numpy as np
+
+from .plan import ColumnarTable, Codec, DataType, Plan
+
+I32 = DataType.INT32
+STR = DataType.VARCHAR
+
+
+def zipf_ranks(rng, n_domain, s, size, chunk=1 << 22):
+    w = np.arange(1, n_domain + 1, dtype=np.float64) ** (-s)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    out = np.empty(size, dtype=np.int64)
+    for i in range(0, size, chunk):
+        u = rng.random(min(chunk, size - i))
+        out[i:i + len(u)] = np.searchsorted(cdf, u, side="right")
+    np.minimum(out, n_domain - 1, out=out)
+    return out
+
+
+def zipf_keys(rng, n_domain, s, size, perm=None):
+    if perm is None:
+        perm = rng.permutation(n_domain).astype(np.int32)
+    return perm[zipf_ranks(rng, n_domain, s, size)], perm
+
+
+def random_strings(rng, n, lo=8, hi=64):
+    lens = rng.integers(lo, hi + 1, n).astype(np.uint64)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    chars = rng.integers(33, 127, int(off[-1]), dtype=np.uint8)
+    return off, chars
+
+
+# ------------------------------------------------------------------ two_way (2-way INT32 join)
+def two_way(seed=42, n_build=1_000_000, n_probe=10_000_000, null_p=0.05):
+    rng = np.random.default_rng(seed)
+    bk = rng.permutation(n_build).astype(np.int32)
+    bp = rng.integers(0, 2**31, n_build).astype(np.int32)
+    pk = rng.integers(0, 2 * n_build, n_probe).astype(np.int32)
+    pv = (rng.random(n_probe) >= null_p).astype(np.uint8)
+    pp = rng.integers(0, 2**31, n_probe).astype(np.int32)
+    return dict(bk=bk, bp=bp, pk=pk, pv=pv, pp=pp)
+
+
+def two_way_plan(d, codec=None):
+    codec = codec or Codec()
+    plan = Plan()
+    plan.inputs.append(ColumnarTable(len(d["bk"]), [codec.encode_int32(d["bk"]),
+                                                    codec.encode_int32(d["bp"])]))
+    plan.inputs.append(ColumnarTable(len(d["pk"]), [codec.encode_int32(d["pk"], d["pv"]),
+                                                    codec.encode_int32(d["pp"])]))
+    plan.new_scan_node(0, [(0, I32), (1, I32)])
+    plan.new_scan_node(1, [(0, I32), (1, I32)])
+    plan.new_join_node(True, 0, 1, 0, 0, [(1, I32), (3, I32)])
+    plan.root = 2
+    return plan
+
+
+def two_way_expected_count(d):
+    nb = len(d["bk"])
+    hit = (d["pv"] == 1) & (d["pk"] < nb)  # build keys are exactly 0..nb-1
+    return int(hit.sum())
+
+
+# ------------------------------------------------------------------ star4 (4-way star join)
+STAR_DIMS = (2_500_000, 4_000_000, 100_000)
+
+
+def star4(seed=43, n_fact=36_000_000, dims=STAR_DIMS, s=1.1, null_p=0.03, sel=None):
+    rng = np.random.default_rng(seed)
+    out = {"fact": [], "fact_valid": [], "dims": []}
+    for i, nd in enumerate(dims):
+        key = rng.permutation(nd).astype(np.int32)
+        attr = rng.integers(0, 10000, nd).astype(np.int32)
+        soff, schars = random_strings(rng, nd)
+        w = int(round(10000 * (sel[i] if sel else rng.uniform(0.01, 0.10))))
+        lo = int(rng.integers(0, 10000 - w + 1))
+        out["dims"].append(dict(key=key, attr=attr, soff=soff, schars=schars,
+                                lo=lo, hi=lo + w - 1))
+    for i, nd in enumerate(dims):
+        fk, _ = zipf_keys(rng, nd, s, n_fact)
+        out["fact"].append(fk.astype(np.int32))
+        out["fact_valid"].append((rng.random(n_fact) >= null_p).astype(np.uint8))
+    return out
+
+
+def star4_tables(d, codec=None):
+    codec = codec or Codec()
+    n = len(d["fact"][0])
+    fact = ColumnarTable(n, [codec.encode_int32(fk, v)
+                             for fk, v in zip(d["fact"], d["fact_valid"])])
+    dims = []
+    for dm in d["dims"]:
+        dims.append(ColumnarTable(len(dm["key"]), [codec.encode_int32(dm["key"]),
+                                                   codec.encode_int32(dm["attr"]),
+                                                   codec.encode_varchar(dm["soff"], dm["schars"])]))
+    return [fact] + dims
+
+
+def star4_plan(d, tables=None, codec=None, prefiltered=False):
+    """((F x sD1) x sD2) x sD3 with the range filters pushed into the dimension scans (or, with
+    prefiltered=True, over already-filtered dimension tables, the reference's load-time filter)."""
+    plan = Plan()
+    plan.inputs = list(tables if tables is not None else star4_tables(d, codec))
+    f = plan.new_scan_node(0, [(0, I32), (1, I32), (2, I32)])
+    scans = []
+    for i, dm in enumerate(d["dims"]):
+        flt = () if prefiltered else [(1, dm["lo"], dm["hi"])]
+        scans.append(plan.new_scan_node(1 + i, [(0, I32), (2, STR)], flt))
+    j1 = plan.new_join_node(False, f, scans[0], 0, 0, [(0, I32), (1, I32), (2, I32), (4, STR)])
+    j2 = plan.new_join_node(False, j1, scans[1], 1, 0,
+                            [(0, I32), (1, I32), (2, I32), (3, STR), (5, STR)])
+    j3 = plan.new_join_node(False, j2, scans[2], 2, 0,
+                            [(0, I32), (1, I32), (2, I32), (3, STR), (4, STR), (6, STR)])
+    plan.root = j3
+    return plan
+
+
+def star4_prefiltered_tables(d, codec=None):
+    """the dimension tables after the load-time filter (what the reference's execute() gets)"""
+    codec = codec or Codec()
+    tabs = star4_tables(dict(fact=d["fact"], fact_valid=d["fact_valid"], dims=[]), codec)
+    for dm in d["dims"]:
+        m = (dm["attr"] >= dm["lo"]) & (dm["attr"] <= dm["hi"])
+        idx = np.nonzero(m)[0]
+        lens = (dm["soff"][idx + 1] - dm["soff"][idx]).astype(np.uint64)
+        off = np.zeros(len(idx) + 1, np.uint64)
+        np.cumsum(lens, out=off[1:])
+        starts = dm["soff"][idx].astype(np.int64)
+        sel = np.concatenate([np.arange(s0, s0 + l) for s0, l in zip(starts, lens.astype(np.int64))]) \
+            if len(idx) else np.zeros(0, np.int64)
+        chars = dm["schars"][sel] if len(sel) else np.zeros(0, np.uint8)
+        tabs.append(ColumnarTable(len(idx), [codec.encode_int32(dm["key"][idx]),
+                                             codec.encode_int32(dm["attr"][idx]),
+                                             codec.encode_varchar(off, chars)]))
+    return tabs
+
+
+def star4_expected(d):
+    """exact output row count and the matched rows (fact row, dim rows) by numpy lookups"""
+    n = len(d["fact"][0])
+    ok = np.ones(n, dtype=bool)
+    drow = []
+    for fk, v, dm in zip(d["fact"], d["fact_valid"], d["dims"]):
+        inv = np.empty(len(dm["key"]), dtype=np.int64)
+        inv[dm["key"]] = np.arange(len(dm["key"]))
+        r = inv[fk]
+        passed = (dm["attr"][r] >= dm["lo"]) & (dm["attr"][r] <= dm["hi"])
+        ok &= (v == 1) & passed
+        drow.append(r)
+    rows = np.nonzero(ok)[0]
+    return rows, [r[rows] for r in drow]
+
+
+# ------------------------------------------------------------------ many_to_many (skew stress)
+def many_to_many(seed=44, n_build=50_000_000, n_probe=200_000_000, domain=5_000_000, s=1.3,
+                 null_p=0.10):
+    rng = np.random.default_rng(seed)
+    bk, _ = zipf_keys(rng, domain, s, n_build)
+    pk, _ = zipf_keys(rng, domain, s, n_probe)
+    pv = (rng.random(n_probe) >= null_p).astype(np.uint8)
+    bp = rng.integers(0, 2**31, n_build).astype(np.int32)
+    pp = rng.integers(0, 2**31, n_probe).astype(np.int32)
+    return dict(bk=bk.astype(np.int32), bp=bp, pk=pk.astype(np.int32), pv=pv, pp=pp,
+                domain=domain)
+
+
+def many_to_many_expected_count(d):
+    dom = d["domain"]
+    b = np.bincount(d["bk"], minlength=dom).astype(np.int64)
+    p = np.bincount(d["pk"][d["pv"] == 1], minlength=dom).astype(np.int64)
+    return int((b * p).sum())

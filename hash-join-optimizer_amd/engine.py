@@ -96,6 +96,21 @@ class Context:
         self._check(self.lib.hj_execute_device(self.handle, C.byref(p), arr, C.byref(h)))
         return DeviceResult(self, h)
 
+    def profile(self, prefix):
+        """time engine kernels whose name starts with prefix ("" = all, None = off)"""
+        self._check(self.lib.hj_profile_enable(
+            self.handle, None if prefix is None else prefix.encode()))
+
+    def profile_report(self):
+        """{kernel name: (launches, total ms)} since the last report"""
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self.lib.hj_profile_report(self.handle, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
+
     def synchronize(self):
         self._check(self.lib.hj_synchronize(self.handle))
 

@@ -173,6 +173,13 @@ uint64_t hj_column_num_chars(const hj_column* col);
 int hj_decode_int32(const hj_column* col, int32_t* values, uint8_t* validity);
 int hj_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint8_t* validity);
 
+/* ---- kernel timing (no counterpart in the reference): when enabled, every engine kernel launch
+ * whose name starts with `prefix` ("" = all, NULL = off) is bracketed by HIP events on the
+ * context's stream. hj_profile_report synchronizes and writes one line per kernel name,
+ * "name launches total_ms\n", then clears the records. ---- */
+int hj_profile_enable(hj_context* ctx, const char* prefix);
+int hj_profile_report(hj_context* ctx, char* buf, uint64_t len);
+
 /* ---- directory (hash index) test surface: the GPU unchained directory of one key column,
  * the design of include/unchained.h:14-247. Used by the ported unchained_tests. ---- */
 typedef struct hj_directory hj_directory;
