@@ -69,7 +69,7 @@ def star4_workload(codec, scale):
     gathered = out_rows * 3 * 4 + str_bytes
     b_alg = key_bytes + dim_bytes + 2 * gathered
     info = dict(builds=builds, probes=probes, outs=outs, out_rows=out_rows, b_alg=b_alg,
-                fk_bytes=[4 * int(v.sum()) + (n + 7) // 8 for v in d["fact_valid"]])
+                fk_bytes=[4 * int(v.sum()) + (n + 7) // 8 for v in d["fact_valid"]], out_slots=4)
     workload = {"workload": "4-way star join ((F x sD1) x sD2) x sD3",
                 "fact_rows": n_fact, "dim_rows": list(dims), "fk_dist": "zipf(1.1)",
                 "fk_null": 0.03, "filters": [[dm["lo"], dm["hi"]] for dm in d["dims"]],
@@ -85,7 +85,7 @@ def two_way_workload(codec, scale):
     nb, np_ = len(d["bk"]), len(d["pk"])
     b_alg = 4 * nb + 4 * int(d["pv"].sum()) + (np_ + 7) // 8 + 2 * out_rows * 8
     info = dict(builds=[nb], probes=[np_], outs=[out_rows], out_rows=out_rows, b_alg=b_alg,
-                fk_bytes=[4 * int(d["pv"].sum()) + (np_ + 7) // 8])
+                fk_bytes=[4 * int(d["pv"].sum()) + (np_ + 7) // 8], out_slots=2)
     workload = {"workload": "2-way INT32 join 1M unique x 10M FK (5% NULL)", "seed": 42,
                 "output_rows": out_rows}
     return d, plan.inputs, plan, info, workload
@@ -96,6 +96,10 @@ def kernel_bytes(name, info, launches):
     """algorithmic HBM bytes of all launches of one step of kernel `name` (None: no model)"""
     fk1 = info["fk_bytes"][0]
     outs, probes = info["outs"], info["probes"]
+    if name == "pipe_probe":
+        # every probe-side key column read once in full (packed values + validity bits), plus
+        # the surviving rows' ids written once per output slot
+        return sum(info["fk_bytes"]) + info["out_rows"] * info["out_slots"] * 4
     if name == "probe_scan_count":
         return fk1
     if name == "probe_scan_write":

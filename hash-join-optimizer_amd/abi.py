@@ -96,6 +96,7 @@ _SIGS = {
     "hj_directory_info": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
     "hj_directory_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hj_free": (None, [C.c_void_p]),
+    "hj_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "hj_profile_enable": (C.c_int, [C.c_void_p, C.c_char_p]),
     "hj_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64]),
 }

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +28,7 @@
 #include "hj_internal.hpp"
 #include "kernels.hpp"
 #include "pagecodec.hpp"
+#include "pipeline.hpp"
 
 namespace hj {
 
@@ -83,6 +85,11 @@ class Executor {
         if (!plan_->nodes || plan_->root >= plan_->num_nodes)
             throw Error(HJ_E_INVALID, "plan root out of range");
         depth_ = 0;
+        if (ctx_->use_pipeline) {
+            Rel pr;
+            if (try_pipeline(pr)) return materialize(pr, plan_->nodes[plan_->root]);
+            A_.clear();  // not a chain, or a build side with duplicate keys: join at a time
+        }
         Rel r = node(plan_->root);
         return materialize(r, plan_->nodes[plan_->root]);
     }
@@ -346,6 +353,210 @@ class Executor {
         return out;
     }
 
+    // ---------------------------------------------------------------- fused pipeline
+    // A left-deep chain: every join's build child is a base scan, the probe side descends to a
+    // base scan (the source). All build sides are built in one batch; one kernel streams the
+    // source pages and runs every join per row. Requires unique build keys: a duplicate hit or
+    // an over-long slot is reported by the kernels and the plan is re-run join at a time.
+    struct AttrSlot {
+        int slot;
+        uint32_t col;
+    };
+
+    bool chain(uint64_t idx, uint64_t& source, std::vector<uint64_t>& joins, int depth) {
+        if (idx >= plan_->num_nodes || depth > kMaxStages + 1) return false;
+        const hj_node& nd = plan_->nodes[idx];
+        if (nd.kind == HJ_NODE_SCAN) {
+            source = idx;
+            return true;
+        }
+        if (nd.kind != HJ_NODE_JOIN) return false;
+        const uint64_t b = nd.build_left ? nd.left : nd.right;
+        const uint64_t pr = nd.build_left ? nd.right : nd.left;
+        if (b >= plan_->num_nodes || plan_->nodes[b].kind != HJ_NODE_SCAN) return false;
+        if (!chain(pr, source, joins, depth + 1)) return false;
+        joins.push_back(idx);
+        return true;
+    }
+
+    static uint32_t lg2ceil(uint64_t n) {
+        uint32_t k = 0;
+        while ((1ull << k) < n) ++k;
+        return k;
+    }
+
+    bool try_pipeline(Rel& out) {
+        uint64_t source = 0;
+        std::vector<uint64_t> joins;
+        if (!chain(plan_->root, source, joins, 0) || joins.empty() ||
+            joins.size() > (size_t)kMaxStages)
+            return false;
+        const Rel srel = scan(plan_->nodes[source]);
+        std::vector<AttrSlot> cur;
+        for (auto& a : srel.attrs) cur.push_back({0, a.col});
+        std::vector<uint32_t> slot_table{srel.table};
+
+        BuildBatch hb;
+        PipeDesc hp;
+        std::memset(&hb, 0, sizeof hb);
+        std::memset(&hp, 0, sizeof hp);
+        hb.njobs = (int32_t)joins.size();
+        hp.nst = (int32_t)joins.size();
+        uint32_t page_b = 0;
+        for (size_t i = 0; i < joins.size(); ++i) {
+            const hj_node& jn = plan_->nodes[joins[i]];
+            const bool bl = jn.build_left != 0;
+            const Rel brel = scan(plan_->nodes[bl ? jn.left : jn.right]);
+            std::vector<AttrSlot> bat;
+            for (auto& a : brel.attrs) bat.push_back({(int)i + 1, a.col});
+            slot_table.push_back(brel.table);
+            const uint64_t battr = bl ? jn.left_attr : jn.right_attr;
+            const uint64_t pattr = bl ? jn.right_attr : jn.left_attr;
+            if (battr >= bat.size() || pattr >= cur.size())
+                throw Error(HJ_E_INVALID, "join attribute out of range");
+            const AttrSlot bk = bat[battr], pk = cur[pattr];
+            const DevColumn& bcol = column(slot_table[bk.slot], bk.col);
+            const DevColumn& pcol = column(slot_table[pk.slot], pk.col);
+            if (bcol.type != HJ_INT32 || pcol.type != HJ_INT32)
+                throw Error(HJ_E_TYPE, "Join key isn't of type int32_t");
+            if (i == 0 && pk.slot != 0) return false;
+
+            BuildJob& jb = hb.job[i];
+            jb.key = bcol.view();
+            jb.nf = (int32_t)brel.filters.size();
+            for (size_t f = 0; f < brel.filters.size(); ++f) {
+                jb.fcol[f] = column(brel.table, brel.filters[f].column).view();
+                jb.flo[f] = brel.filters[f].lo;
+                jb.fhi[f] = brel.filters[f].hi;
+            }
+            const uint64_t rows = brel.n;
+            jb.rows = rows;
+            jb.kmax = lg2ceil(rows ? rows : 1);
+            const uint64_t dmax = 1ull << jb.kmax;
+            jb.page_block0 = page_b;
+            page_b += jb.key.num_pages;
+            jb.meta = scratch<uint32_t>(4);
+            jb.blk = scratch<uint32_t>(jb.key.num_pages ? jb.key.num_pages : 1);
+            jb.blk_off = scratch<uint32_t>(jb.key.num_pages ? jb.key.num_pages : 1);
+            jb.cnt = scratch<uint32_t>(dmax);
+            jb.bloom = scratch<uint32_t>(dmax);
+            jb.dir = scratch<uint64_t>(dmax + 1);
+            jb.part = scratch<uint32_t>((dmax + 4095) / 4096 + 1);
+            jb.st_h = scratch<uint32_t>(rows ? rows : 1);
+            jb.st_row = scratch<uint32_t>(rows ? rows : 1);
+            jb.st_pos = scratch<uint32_t>(rows ? rows : 1);
+            jb.adj = scratch<uint64_t>(rows ? rows : 1);
+            jb.bf = scratch<uint32_t>(kBloomWords);
+
+            PipeStage& st = hp.st[i];
+            st.job = (int32_t)i;
+            st.key_slot = pk.slot;
+            st.keycol = pcol.view();
+            if (i == 0) hp.driver = pcol.view();
+
+            const std::vector<AttrSlot>& L = bl ? bat : cur;
+            const std::vector<AttrSlot>& R = bl ? cur : bat;
+            std::vector<AttrSlot> nxt;
+            for (uint64_t a = 0; a < jn.num_output_attrs; ++a) {
+                const uint64_t idx = jn.output_attrs[a].index;
+                if (idx < L.size())
+                    nxt.push_back(L[idx]);
+                else if (idx < L.size() + R.size())
+                    nxt.push_back(R[idx - L.size()]);
+                else
+                    throw Error(HJ_E_INVALID, "output attribute out of range");
+            }
+            cur.swap(nxt);
+        }
+        hb.page_blocks = page_b;
+        hb.grid = (uint32_t)ctx_->num_cus * 8;
+
+        hp.nf = (int32_t)srel.filters.size();
+        for (size_t f = 0; f < srel.filters.size(); ++f) {
+            hp.fcol[f] = column(srel.table, srel.filters[f].column).view();
+            hp.flo[f] = srel.filters[f].lo;
+            hp.fhi[f] = srel.filters[f].hi;
+        }
+        // output slots referenced by the root's attributes
+        std::vector<int> oslots;
+        Rel res;
+        for (auto& a : cur) {
+            auto it = std::find(oslots.begin(), oslots.end(), a.slot);
+            uint32_t q;
+            if (it == oslots.end()) {
+                q = (uint32_t)oslots.size();
+                oslots.push_back(a.slot);
+            } else {
+                q = (uint32_t)(it - oslots.begin());
+            }
+            res.attrs.push_back({q, a.col, column(slot_table[a.slot], a.col).type});
+        }
+        const uint64_t nsrc = srel.n;
+        const uint32_t npages = hp.driver.num_pages;
+        hp.nout = (int32_t)oslots.size();
+        const uint64_t ncap = nsrc ? nsrc : 1;
+        for (size_t q = 0; q < oslots.size(); ++q) {
+            hp.out_slot[q] = oslots[q];
+            hp.final_out[q] = result<uint32_t>(ncap);
+            res.slots.push_back({slot_table[oslots[q]], hp.final_out[q]});
+        }
+        hp.s0_row = scratch<uint32_t>(ncap);
+        hp.s0_b = scratch<uint32_t>(ncap);
+        hp.page_cnt = scratch<uint32_t>(npages ? npages : 1);
+        hp.page_off = scratch<uint32_t>(npages ? npages : 1);
+        if (hp.nst > 1) {
+            hp.l_row = scratch<uint32_t>(ncap);
+            for (int st = 0; st < hp.nst; ++st) hp.l_b[st] = scratch<uint32_t>(ncap);
+            hp.tile_cnt = scratch<uint32_t>(ncap / 4096 + 2);
+            hp.tile_off = scratch<uint32_t>(ncap / 4096 + 2);
+            // joins keyed by the source stream over their key column, smallest build first
+            for (int st = 0; st < hp.nst; ++st)
+                if (hp.st[st].key_slot == 0) hp.order[hp.nstream++] = st;
+            std::stable_sort(hp.order, hp.order + hp.nstream, [&](int a, int b) {
+                return hb.job[hp.st[a].job].rows < hb.job[hp.st[b].job].rows;
+            });
+            // one bit per source row, plus two words for the shifted ballot of the last rows
+            hp.bm_words = (uint32_t)((nsrc + 31) / 32 + 2);
+            hp.bm[0] = scratch<uint32_t>(hp.bm_words);
+            hp.bm[1] = scratch<uint32_t>(hp.bm_words);
+        }
+        hp.meta = scratch<uint32_t>(4);
+        hp.grid = (uint32_t)ctx_->num_cus * 8;
+
+        // descriptors -> pinned staging -> device
+        uint8_t* pin = (uint8_t*)ctx_->pinned + hj_context::kDescOffset;
+        const size_t bb_bytes = sizeof(BuildBatch), pd_off = (bb_bytes + 255) & ~(size_t)255;
+        if (pd_off + sizeof(PipeDesc) > hj_context::kPinnedBytes - hj_context::kDescOffset)
+            throw Error(HJ_E_INVALID, "pipeline descriptors exceed the staging buffer");
+        uint8_t* ddesc = (uint8_t*)A_.scratch(pd_off + sizeof(PipeDesc));
+        hp.jobs = (const BuildJob*)(ddesc + offsetof(BuildBatch, job));
+        std::memcpy(pin, &hb, bb_bytes);
+        std::memcpy(pin + pd_off, &hp, sizeof(PipeDesc));
+        HJ_HIP(hipMemcpyAsync(ddesc, pin, pd_off + sizeof(PipeDesc), hipMemcpyHostToDevice, s_));
+        zero_u32(hp.meta, 4, s_);
+        const BuildBatch* dbb = (const BuildBatch*)ddesc;
+        const PipeDesc* dpd = (const PipeDesc*)(ddesc + pd_off);
+        { Prof p_(ctx_, "pipe_build"); pipe_build(hb, dbb, s_); }
+        { Prof p_(ctx_, "pipe_probe"); pipe_probe(hp, dpd, s_); }
+        struct F4 {
+            uint32_t v[4];
+        };
+        const F4 fl = fetch(ctx_, (const F4*)hp.meta);
+        if (std::getenv("HJ_DEBUG_PIPE")) {
+            for (int j = 0; j < hb.njobs; ++j) {
+                const F4 jm = fetch(ctx_, (const F4*)hb.job[j].meta);
+                std::fprintf(stderr, "[pipe] job %d: tuples %u longest slot %u\n", j, jm.v[0],
+                             jm.v[2]);
+            }
+            std::fprintf(stderr, "[pipe] dup %u long %u stage0 %u out %u\n", fl.v[0], fl.v[1],
+                         fl.v[2], fl.v[3]);
+        }
+        if (fl.v[0] || fl.v[1]) return false;
+        res.n = fl.v[3];
+        out = std::move(res);
+        return true;
+    }
+
     Rel empty_like(Rel out, const std::vector<std::pair<int, uint32_t>>& used, const Rel& L,
                    const Rel& R) {
         out.n = 0;
@@ -517,6 +728,22 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
     dc.group_start = (uint32_t*)dev_alloc(group_start.size() * 4);
     HJ_HIP(hipMemcpyAsync(dc.group_start, group_start.data(), group_start.size() * 4,
                           hipMemcpyHostToDevice, ctx->stream));
+    {
+        // chunk index: page of row c * 2048 for every chunk, plus one extra entry
+        const uint64_t nch = (rows >> HJ_CHUNK_SHIFT) + 2;
+        std::vector<uint32_t> cp(nch, hc.num_pages ? hc.num_pages - 1 : 0);
+        uint32_t p = 0;
+        for (uint64_t c = 0; c < nch; ++c) {
+            const uint64_t r = c << HJ_CHUNK_SHIFT;
+            if (r >= rows) break;
+            while (p + 1 < hc.num_pages && dc.h_row_start[p + 1] <= r) ++p;
+            cp[c] = p;
+        }
+        dc.chunk_page = (uint32_t*)dev_alloc(nch * 4);
+        HJ_HIP(hipMemcpyAsync(dc.chunk_page, cp.data(), nch * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
+        HJ_HIP(hipStreamSynchronize(ctx->stream));
+    }
     dc.rank = (uint16_t*)dev_alloc(groups * 2);
     if (!dc.all_dense && groups) launch_rank_index(dc.view(), dc.rank, ctx->stream);
     HJ_HIP(hipStreamSynchronize(ctx->stream));
@@ -631,7 +858,8 @@ int hj_context_create(uint64_t arena_bytes, hj_context** out) {
         if (!bytes) bytes = 16ull << 30;
         HJ_HIP(hipMalloc(&ctx->arena_mem, bytes));
         ctx->arena.init(ctx->arena_mem, bytes);
-        HJ_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+        HJ_HIP(hipHostMalloc(&ctx->pinned, hj_context::kPinnedBytes, hipHostMallocDefault));
+        if (const char* e = std::getenv("HJ_PIPELINE")) ctx->use_pipeline = std::atoi(e) != 0;
     });
     if (rc) {
         if (ctx->arena_mem) (void)hipFree(ctx->arena_mem);
@@ -660,6 +888,16 @@ int hj_context_destroy(hj_context* ctx) {
 }
 
 void* hj_context_stream(hj_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int hj_set_option(hj_context* ctx, const char* name, int64_t value) {
+    if (!ctx || !name) return HJ_E_INVALID;
+    if (std::strcmp(name, "pipeline") == 0) {
+        ctx->use_pipeline = value != 0;
+        return HJ_OK;
+    }
+    ctx->last_error = std::string("unknown option ") + name;
+    return HJ_E_INVALID;
+}
 
 int hj_profile_enable(hj_context* ctx, const char* prefix) {
     if (!ctx) return HJ_E_INVALID;

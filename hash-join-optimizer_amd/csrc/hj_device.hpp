@@ -7,6 +7,7 @@
 //   rank        u16[groups]          per 64-row group of a page with NULLs: the number of
 //                                    non-NULL rows of the page before the group
 //   group_start u32[num_pages]       index of the page's first group in rank
+//   chunk_page  u32[rows/2048 + 2]   page holding the first row of each 2048-row chunk
 // A row r is found by page search (or r / uniform_rows when every page but the last holds the
 // same number of rows, the dense 1984-row case of include/column_t.h:14), then its value index
 // among the page's packed non-NULL values is rank + popcount of the bitmap bits below it.
@@ -24,6 +25,7 @@ struct DCol {
     const uint32_t* row_start;
     const uint16_t* rank;
     const uint32_t* group_start;
+    const uint32_t* chunk_page;  // page holding row c * 2048, for c = 0 .. ceil(rows/2048)
     uint32_t num_pages;
     int32_t type;
     uint32_t uniform_rows;
@@ -55,13 +57,17 @@ __device__ inline const uint8_t* page_ptr(const DCol& c, uint32_t p) {
     return c.pages + (size_t)p * HJ_PAGE_BYTES;
 }
 
-// last page p with row_start[p] <= r (never a 0-row page, see the layout note above)
+#define HJ_CHUNK_SHIFT 11
+// last page p with row_start[p] <= r (never a 0-row page, see the layout note above): the
+// chunk index narrows the search to the pages overlapping r's 2048-row chunk
 __device__ inline uint32_t find_page(const DCol& c, uint32_t r) {
     if (c.uniform_rows) {
         uint32_t p = r / c.uniform_rows;
         return p < c.num_pages ? p : c.num_pages - 1;
     }
-    uint32_t lo = 0, hi = c.num_pages;
+    const uint32_t ch = r >> HJ_CHUNK_SHIFT;
+    uint32_t lo = c.chunk_page[ch], hi = c.chunk_page[ch + 1] + 1;
+    if (hi > c.num_pages) hi = c.num_pages;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
         if (c.row_start[mid] <= r)
@@ -72,31 +78,34 @@ __device__ inline uint32_t find_page(const DCol& c, uint32_t r) {
     return lo;
 }
 
-// the up-to-8 bitmap bytes [off, off+8) of a bitmap of nb bytes, little-endian
-__device__ inline uint64_t load_bits64(const uint8_t* bm, uint32_t nb, uint32_t off) {
-    uint64_t v = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 8; ++k)
-        if (off + k < nb) v |= (uint64_t)bm[off + k] << (8 * k);
-    return v;
+// the 64 validity bits of 64-row group g of a page whose bitmap (nb bytes) ends at the page end:
+// two aligned 8-byte loads and a funnel shift (bits past the page's last row are garbage and are
+// never used: callers test bit (local & 63) and count bits below it). The second word is only
+// loaded while it lies inside the page.
+__device__ inline uint64_t group_bits(const uint8_t* pg, uint32_t nb, uint32_t g) {
+    const uintptr_t a = (uintptr_t)(pg + HJ_PAGE_BYTES - nb + 8u * g);
+    const uint64_t* w = (const uint64_t*)(a & ~(uintptr_t)7);
+    const uint32_t sh = (uint32_t)(a & 7) * 8u;
+    const uint64_t w0 = w[0];
+    if (sh == 0) return w0;
+    const uint64_t w1 = ((uintptr_t)(w + 1) < (uintptr_t)(pg + HJ_PAGE_BYTES)) ? w[1] : 0ull;
+    return (w0 >> sh) | (w1 << (64u - sh));
 }
 
 // value index of local row `local` in page p (header word hdr = nrows | nnonnull << 16);
-// returns false for a NULL row
+// returns false for a NULL row. In a sequential scan the 64 lanes of a wave share one group,
+// so the group word and its rank are one broadcast load each.
 __device__ inline bool value_index(const DCol& c, uint32_t p, const uint8_t* pg, uint32_t hdr,
                                    uint32_t local, uint32_t* vi) {
-    uint32_t nr = hdr & 0xffffu, nn = hdr >> 16;
+    const uint32_t nr = hdr & 0xffffu, nn = hdr >> 16;
     if (nn == nr) {
         *vi = local;
         return true;
     }
-    uint32_t nb = (nr + 7) >> 3;
-    const uint8_t* bm = pg + HJ_PAGE_BYTES - nb;
-    if (!((bm[local >> 3] >> (local & 7)) & 1)) return false;
-    uint32_t g = local >> 6;
-    uint64_t bits = load_bits64(bm, nb, g * 8);
-    *vi = (uint32_t)c.rank[c.group_start[p] + g] +
-          (uint32_t)__popcll(bits & ((1ull << (local & 63)) - 1ull));
+    const uint32_t g = local >> 6, b = local & 63u;
+    const uint64_t bits = group_bits(pg, (nr + 7) >> 3, g);
+    if (!((bits >> b) & 1ull)) return false;
+    *vi = (uint32_t)c.rank[c.group_start[p] + g] + (uint32_t)__popcll(bits & ((1ull << b) - 1ull));
     return true;
 }
 

@@ -84,6 +84,7 @@ struct DevColumn {
     uint32_t* row_start = nullptr; // device, num_pages + 1
     uint16_t* rank = nullptr;      // device
     uint32_t* group_start = nullptr;
+    uint32_t* chunk_page = nullptr;
     uint32_t uniform_rows = 0;
     uint32_t all_dense = 1;
     std::vector<uint32_t> h_row_start;
@@ -93,6 +94,7 @@ struct DevColumn {
         d.row_start = row_start;
         d.rank = rank;
         d.group_start = group_start;
+        d.chunk_page = chunk_page;
         d.num_pages = num_pages;
         d.type = type;
         d.uniform_rows = uniform_rows;
@@ -132,8 +134,11 @@ struct hj_context {
     uint8_t* arena_mem = nullptr;
     std::string last_error;
     hj_dresult* live_result = nullptr;
-    // small pinned host staging for device->host scalars
+    // pinned host staging: [0, 4 KiB) device->host scalars, [4 KiB, 64 KiB) kernel descriptors
     uint64_t* pinned = nullptr;
+    static constexpr size_t kPinnedBytes = 64 * 1024;
+    static constexpr size_t kDescOffset = 4096;
+    bool use_pipeline = true;  // fused probe pipeline for left-deep chains (hj_set_option)
     int num_cus = 256;
     // kernel timing (hj_profile_enable)
     bool prof_on = false;

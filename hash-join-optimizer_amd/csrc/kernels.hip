@@ -91,6 +91,8 @@ void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
 
 }  // namespace
 
+void zero_u32(uint32_t* p, uint64_t n, hipStream_t s) { fill_u32(p, n, 0u, s); }
+
 // =====================================================================================
 // Page index of a resident column: rank[group_start[p] + g] = non-NULL rows of page p before
 // 64-row group g. One wave per page; reads the bitmaps (1 bit per row).
@@ -103,13 +105,12 @@ __global__ __launch_bounds__(64) void k_rank_index(DCol c, uint16_t* rank_out) {
     if (c.type == 3 /*VARCHAR*/ && (nr == 0xffffu || nr == 0xfffeu)) return;
     if (nr == nn) return;
     const uint32_t nb = (nr + 7) >> 3;
-    const uint8_t* bm = pg + HJ_PAGE_BYTES - nb;
     const uint32_t ngroups = (nr + 63) >> 6;
     const uint32_t gs = c.group_start[p];
     uint32_t carry = 0;
     for (uint32_t g0 = 0; g0 < ngroups; g0 += HJ_WAVE) {
         uint32_t g = g0 + lane_id();
-        uint32_t pc = g < ngroups ? (uint32_t)__popcll(load_bits64(bm, nb, g * 8)) : 0u;
+        uint32_t pc = g < ngroups ? (uint32_t)__popcll(group_bits(pg, nb, g)) : 0u;
         uint32_t incl = wave_incl_scan(pc);
         if (g < ngroups) rank_out[gs + g] = (uint16_t)(carry + incl - pc);
         carry += __shfl(incl, HJ_WAVE - 1, HJ_WAVE);

@@ -59,6 +59,9 @@ struct HeavyEntry {
     uint64_t out;
 };
 
+// ---- zero fill by kernel (safe to capture in a HIP graph, unlike hipMemsetAsync)
+void zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
+
 // ---- resident-table page index
 void launch_rank_index(const DCol& c, uint16_t* rank_out, hipStream_t s);
 

@@ -96,6 +96,10 @@ class Context:
         self._check(self.lib.hj_execute_device(self.handle, C.byref(p), arr, C.byref(h)))
         return DeviceResult(self, h)
 
+    def set_option(self, name, value):
+        """engine options: "pipeline" (1 = fused probe pipeline for left-deep chains, 0 = off)"""
+        self._check(self.lib.hj_set_option(self.handle, name.encode(), int(value)))
+
     def profile(self, prefix):
         """time engine kernels whose name starts with prefix ("" = all, None = off)"""
         self._check(self.lib.hj_profile_enable(

@@ -173,6 +173,11 @@ uint64_t hj_column_num_chars(const hj_column* col);
 int hj_decode_int32(const hj_column* col, int32_t* values, uint8_t* validity);
 int hj_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint8_t* validity);
 
+/* ---- engine options (no counterpart in the reference):
+ *   "pipeline" 1 (default) / 0: run left-deep chains whose build sides are base scans as one
+ *              fused probe pipeline; 0 forces the join-at-a-time path for every plan. ---- */
+int hj_set_option(hj_context* ctx, const char* name, int64_t value);
+
 /* ---- kernel timing (no counterpart in the reference): when enabled, every engine kernel launch
  * whose name starts with `prefix` ("" = all, NULL = off) is bracketed by HIP events on the
  * context's stream. hj_profile_report synchronizes and writes one line per kernel name,
