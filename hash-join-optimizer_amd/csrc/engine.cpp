@@ -402,7 +402,9 @@ class Executor {
         std::memset(&hp, 0, sizeof hp);
         hb.njobs = (int32_t)joins.size();
         hp.nst = (int32_t)joins.size();
-        uint32_t page_b = 0;
+        // one zeroed block for the pipeline's and every job's counters
+        uint32_t* metas = scratch<uint32_t>(4 * (joins.size() + 1));
+        hp.meta = metas;
         for (size_t i = 0; i < joins.size(); ++i) {
             const hj_node& jn = plan_->nodes[joins[i]];
             const bool bl = jn.build_left != 0;
@@ -420,6 +422,7 @@ class Executor {
             if (bcol.type != HJ_INT32 || pcol.type != HJ_INT32)
                 throw Error(HJ_E_TYPE, "Join key isn't of type int32_t");
             if (i == 0 && pk.slot != 0) return false;
+            if (brel.n >= (1ull << 30)) return false;  // table index bits
 
             BuildJob& jb = hb.job[i];
             jb.key = bcol.view();
@@ -429,30 +432,22 @@ class Executor {
                 jb.flo[f] = brel.filters[f].lo;
                 jb.fhi[f] = brel.filters[f].hi;
             }
-            const uint64_t rows = brel.n;
+            const uint32_t rows = (uint32_t)brel.n;
             jb.rows = rows;
-            jb.kmax = lg2ceil(rows ? rows : 1);
-            const uint64_t dmax = 1ull << jb.kmax;
-            jb.page_block0 = page_b;
-            page_b += jb.key.num_pages;
-            jb.meta = scratch<uint32_t>(4);
-            jb.blk = scratch<uint32_t>(jb.key.num_pages ? jb.key.num_pages : 1);
-            jb.blk_off = scratch<uint32_t>(jb.key.num_pages ? jb.key.num_pages : 1);
-            jb.cnt = scratch<uint32_t>(dmax);
-            jb.bloom = scratch<uint32_t>(dmax);
-            jb.dir = scratch<uint64_t>(dmax + 1);
-            jb.part = scratch<uint32_t>((dmax + 4095) / 4096 + 1);
-            jb.st_h = scratch<uint32_t>(rows ? rows : 1);
-            jb.st_row = scratch<uint32_t>(rows ? rows : 1);
-            jb.st_pos = scratch<uint32_t>(rows ? rows : 1);
-            jb.adj = scratch<uint64_t>(rows ? rows : 1);
+            jb.kmax = table_bits(rows);
+            jb.meta = metas + 4 * (i + 1);
+            const uint64_t bgrid = (uint64_t)ctx_->num_cus * 2;
+            const uint64_t scap = std::max<uint64_t>(1, build_range(rows, (uint32_t)bgrid) * bgrid);
+            jb.st_key = scratch<uint32_t>(scap);
+            jb.st_row = scratch<uint32_t>(scap);
+            jb.blk = scratch<uint32_t>(bgrid);
+            jb.table = scratch<uint64_t>(1ull << jb.kmax);
             jb.bf = scratch<uint32_t>(kBloomWords);
 
             PipeStage& st = hp.st[i];
             st.job = (int32_t)i;
             st.key_slot = pk.slot;
             st.keycol = pcol.view();
-            if (i == 0) hp.driver = pcol.view();
 
             const std::vector<AttrSlot>& L = bl ? bat : cur;
             const std::vector<AttrSlot>& R = bl ? cur : bat;
@@ -468,8 +463,7 @@ class Executor {
             }
             cur.swap(nxt);
         }
-        hb.page_blocks = page_b;
-        hb.grid = (uint32_t)ctx_->num_cus * 8;
+        hb.grid = (uint32_t)ctx_->num_cus * 2;
 
         hp.nf = (int32_t)srel.filters.size();
         for (size_t f = 0; f < srel.filters.size(); ++f) {
@@ -492,7 +486,7 @@ class Executor {
             res.attrs.push_back({q, a.col, column(slot_table[a.slot], a.col).type});
         }
         const uint64_t nsrc = srel.n;
-        const uint32_t npages = hp.driver.num_pages;
+        hp.src_rows = (uint32_t)nsrc;
         hp.nout = (int32_t)oslots.size();
         const uint64_t ncap = nsrc ? nsrc : 1;
         for (size_t q = 0; q < oslots.size(); ++q) {
@@ -500,27 +494,40 @@ class Executor {
             hp.final_out[q] = result<uint32_t>(ncap);
             res.slots.push_back({slot_table[oslots[q]], hp.final_out[q]});
         }
-        hp.s0_row = scratch<uint32_t>(ncap);
-        hp.s0_b = scratch<uint32_t>(ncap);
-        hp.page_cnt = scratch<uint32_t>(npages ? npages : 1);
-        hp.page_off = scratch<uint32_t>(npages ? npages : 1);
-        if (hp.nst > 1) {
+        // joins keyed by the source run fused, smallest build table first (its Bloom filter
+        // is the one held in LDS); the others run afterwards over the survivors
+        for (int st = 0; st < hp.nst; ++st) {
+            hp.qpos[st] = -1;
+            if (hp.st[st].key_slot == 0)
+                hp.order[hp.nstream++] = st;
+            else
+                ++hp.ndep;
+        }
+        std::stable_sort(hp.order, hp.order + hp.nstream, [&](int a, int b) {
+            return hb.job[hp.st[a].job].rows < hb.job[hp.st[b].job].rows;
+        });
+        for (int q = 0; q < hp.nstream; ++q) hp.qpos[hp.order[q]] = q;
+        {
+            // fused pass: contiguous ranges of 64-row groups per block
+            const int U = probe_unroll(hp.nstream);
+            const uint64_t groups = (nsrc + 63) / 64;
+            const uint64_t maxb = (uint64_t)ctx_->num_cus * 8;
+            const uint64_t per = (uint64_t)U * 4;  // groups per block and iteration
+            uint64_t gpb = std::max<uint64_t>(per, (groups + maxb - 1) / maxb);
+            gpb = (gpb + per - 1) / per * per;
+            hp.probe_gpb = (uint32_t)gpb;
+            hp.probe_blocks = (uint32_t)std::max<uint64_t>(1, (groups + gpb - 1) / gpb);
+            const uint64_t cap = (uint64_t)hp.probe_blocks * gpb * 64;
+            for (int q = 0; q <= hp.nstream; ++q) hp.stage[q] = scratch<uint32_t>(cap);
+            hp.blk_cnt = scratch<uint32_t>(hp.probe_blocks);
+            hp.blk_off = scratch<uint32_t>(hp.probe_blocks);
+        }
+        if (hp.ndep) {
             hp.l_row = scratch<uint32_t>(ncap);
             for (int st = 0; st < hp.nst; ++st) hp.l_b[st] = scratch<uint32_t>(ncap);
             hp.tile_cnt = scratch<uint32_t>(ncap / 4096 + 2);
             hp.tile_off = scratch<uint32_t>(ncap / 4096 + 2);
-            // joins keyed by the source stream over their key column, smallest build first
-            for (int st = 0; st < hp.nst; ++st)
-                if (hp.st[st].key_slot == 0) hp.order[hp.nstream++] = st;
-            std::stable_sort(hp.order, hp.order + hp.nstream, [&](int a, int b) {
-                return hb.job[hp.st[a].job].rows < hb.job[hp.st[b].job].rows;
-            });
-            // one bit per source row, plus two words for the shifted ballot of the last rows
-            hp.bm_words = (uint32_t)((nsrc + 31) / 32 + 2);
-            hp.bm[0] = scratch<uint32_t>(hp.bm_words);
-            hp.bm[1] = scratch<uint32_t>(hp.bm_words);
         }
-        hp.meta = scratch<uint32_t>(4);
         hp.grid = (uint32_t)ctx_->num_cus * 8;
 
         // descriptors -> pinned staging -> device
@@ -533,7 +540,7 @@ class Executor {
         std::memcpy(pin, &hb, bb_bytes);
         std::memcpy(pin + pd_off, &hp, sizeof(PipeDesc));
         HJ_HIP(hipMemcpyAsync(ddesc, pin, pd_off + sizeof(PipeDesc), hipMemcpyHostToDevice, s_));
-        zero_u32(hp.meta, 4, s_);
+        zero_u32(metas, 4 * (joins.size() + 1), s_);
         const BuildBatch* dbb = (const BuildBatch*)ddesc;
         const PipeDesc* dpd = (const PipeDesc*)(ddesc + pd_off);
         { Prof p_(ctx_, "pipe_build"); pipe_build(hb, dbb, s_); }
@@ -545,13 +552,13 @@ class Executor {
         if (std::getenv("HJ_DEBUG_PIPE")) {
             for (int j = 0; j < hb.njobs; ++j) {
                 const F4 jm = fetch(ctx_, (const F4*)hb.job[j].meta);
-                std::fprintf(stderr, "[pipe] job %d: tuples %u longest slot %u\n", j, jm.v[0],
-                             jm.v[2]);
+                std::fprintf(stderr, "[pipe] job %d: tuples %u duplicate %u\n", j, jm.v[0],
+                             jm.v[1]);
             }
-            std::fprintf(stderr, "[pipe] dup %u long %u stage0 %u out %u\n", fl.v[0], fl.v[1],
-                         fl.v[2], fl.v[3]);
+            std::fprintf(stderr, "[pipe] fallback %u survivors %u out %u\n", fl.v[0], fl.v[2],
+                         fl.v[3]);
         }
-        if (fl.v[0] || fl.v[1]) return false;
+        if (fl.v[0]) return false;
         res.n = fl.v[3];
         out = std::move(res);
         return true;
@@ -746,6 +753,19 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
     }
     dc.rank = (uint16_t*)dev_alloc(groups * 2);
     if (!dc.all_dense && groups) launch_rank_index(dc.view(), dc.rank, ctx->stream);
+    if (hc.type == HJ_INT32) {
+        dc.flat = (int32_t*)dev_alloc(rows * 4);
+        if (!dc.all_dense) {
+            const size_t vb = ((rows + 63) / 64 + 1) * 8;
+            dc.vbits = (uint64_t*)dev_alloc(vb);
+            HJ_HIP(hipMemsetAsync(dc.vbits, 0, vb, ctx->stream));
+        }
+        // decode through the page path (flat unset in this view)
+        DCol v = dc.view();
+        v.flat = nullptr;
+        v.vbits = nullptr;
+        launch_decode_int32(v, dc.flat, dc.vbits, ctx->stream);
+    }
     HJ_HIP(hipStreamSynchronize(ctx->stream));
 }
 

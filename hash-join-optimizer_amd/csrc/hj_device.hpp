@@ -8,6 +8,12 @@
 //                                    non-NULL rows of the page before the group
 //   group_start u32[num_pages]       index of the page's first group in rank
 //   chunk_page  u32[rows/2048 + 2]   page holding the first row of each 2048-row chunk
+// and, for INT32 columns, a row-space mirror decoded from the pages at upload:
+//   flat        i32[rows]            the value of every row (0 for NULL)
+//   vbits       u64[rows/64 + 1]     validity, bit r & 63 of word r >> 6 (absent: no NULLs)
+// so that row r of an INT32 column is one load (plus one broadcast load of its validity word),
+// and 64 consecutive rows are one coalesced load: the same bytes as the pages, without the
+// page search.
 // A row r is found by page search (or r / uniform_rows when every page but the last holds the
 // same number of rows, the dense 1984-row case of include/column_t.h:14), then its value index
 // among the page's packed non-NULL values is rank + popcount of the bitmap bits below it.
@@ -26,6 +32,8 @@ struct DCol {
     const uint16_t* rank;
     const uint32_t* group_start;
     const uint32_t* chunk_page;  // page holding row c * 2048, for c = 0 .. ceil(rows/2048)
+    const int32_t* flat;         // INT32: value per row (0 for NULL)
+    const uint64_t* vbits;       // INT32: validity per row; nullptr when the column has no NULL
     uint32_t num_pages;
     int32_t type;
     uint32_t uniform_rows;
@@ -109,8 +117,17 @@ __device__ inline bool value_index(const DCol& c, uint32_t p, const uint8_t* pg,
     return true;
 }
 
+// validity of row r of an INT32 column's row-space mirror
+__device__ inline bool flat_valid(const DCol& c, uint32_t r) {
+    return !c.vbits || ((c.vbits[r >> 6] >> (r & 63u)) & 1ull);
+}
+
 // INT32 value of global row r; false for NULL
 __device__ inline bool get_int32(const DCol& c, uint32_t r, int32_t* v) {
+    if (c.flat) {
+        *v = c.flat[r];
+        return flat_valid(c, r);
+    }
     uint32_t p = find_page(c, r);
     const uint8_t* pg = page_ptr(c, p);
     uint32_t hdr = *(const uint32_t*)pg;

@@ -85,6 +85,8 @@ struct DevColumn {
     uint16_t* rank = nullptr;      // device
     uint32_t* group_start = nullptr;
     uint32_t* chunk_page = nullptr;
+    int32_t* flat = nullptr;       // device, INT32 only: value per row
+    uint64_t* vbits = nullptr;     // device, INT32 with NULLs only: validity per row
     uint32_t uniform_rows = 0;
     uint32_t all_dense = 1;
     std::vector<uint32_t> h_row_start;
@@ -95,6 +97,8 @@ struct DevColumn {
         d.rank = rank;
         d.group_start = group_start;
         d.chunk_page = chunk_page;
+        d.flat = flat;
+        d.vbits = vbits;
         d.num_pages = num_pages;
         d.type = type;
         d.uniform_rows = uniform_rows;

@@ -122,6 +122,39 @@ void launch_rank_index(const DCol& c, uint16_t* rank_out, hipStream_t s) {
     hipLaunchKernelGGL(k_rank_index, dim3(c.num_pages), dim3(64), 0, s, c, rank_out);
 }
 
+// INT32 pages -> row-space mirror: one block per page, one row per thread; each wave's 64
+// validity bits are or-ed into the (zeroed) bitmap at the page's first row, which need not be
+// 64-aligned, so a wave touches up to two words
+__global__ __launch_bounds__(kBlock) void k_decode_int32(DCol c, int32_t* flat, uint64_t* vbits) {
+    const uint32_t p = blockIdx.x;
+    const uint8_t* pg = page_ptr(c, p);
+    const uint32_t hdr = *(const uint32_t*)pg;
+    const uint32_t nr = hdr & 0xffffu;
+    const uint32_t r0 = c.row_start[p];
+    for (uint32_t l0 = 0; l0 < nr; l0 += kBlock) {
+        const uint32_t l = l0 + threadIdx.x;
+        int32_t v = 0;
+        const bool ok = l < nr && get_int32_in_page(c, p, pg, hdr, l, &v);
+        if (l < nr) flat[r0 + l] = ok ? v : 0;
+        if (vbits) {
+            const uint64_t bits = __ballot(ok);
+            if (lane_id() == 0 && bits) {
+                const uint32_t rb = r0 + l0 + (threadIdx.x & ~(uint32_t)(HJ_WAVE - 1));
+                const uint32_t w0 = rb >> 6, sh = rb & 63u;
+                atomicOr((unsigned long long*)&vbits[w0], (unsigned long long)(bits << sh));
+                if (sh && (bits >> (64u - sh)))
+                    atomicOr((unsigned long long*)&vbits[w0 + 1],
+                             (unsigned long long)(bits >> (64u - sh)));
+            }
+        }
+    }
+}
+
+void launch_decode_int32(const DCol& c, int32_t* flat, uint64_t* vbits, hipStream_t s) {
+    if (c.num_pages == 0) return;
+    hipLaunchKernelGGL(k_decode_int32, dim3(c.num_pages), dim3(kBlock), 0, s, c, flat, vbits);
+}
+
 // =====================================================================================
 // Exclusive scans: reduce tiles -> scan tile partials in one block -> rescan tiles.
 // A tile is 4096 elements (256 threads x 16), staged through LDS for coalesced access.

@@ -64,6 +64,9 @@ void zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
 
 // ---- resident-table page index
 void launch_rank_index(const DCol& c, uint16_t* rank_out, hipStream_t s);
+// INT32 pages -> flat values + row-space validity bitmap (vbits zeroed beforehand, or nullptr
+// for a column without NULLs)
+void launch_decode_int32(const DCol& c, int32_t* flat, uint64_t* vbits, hipStream_t s);
 
 // ---- generic exclusive scans (tile = 4096 elements per block)
 size_t scan_partials_needed(uint64_t n);

@@ -1,6 +1,8 @@
-// pipeline.hpp - the probe pipeline for left-deep join chains whose build sides are base scans:
-// all build sides are built in one batch of launches (sort-free unchained directories); the probe
-// source is streamed once for the first join, and the survivors run the remaining joins.
+// pipeline.hpp - the probe pipeline for left-deep join chains whose build sides are base scans
+// with unique join keys (star and snowflake joins): every build side is built in the same three
+// launches into an open-addressing table, and every join keyed by a column of the probe source
+// runs in one fused streaming pass over the source; joins keyed by a column of an earlier
+// build table run afterwards over the survivors.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -11,86 +13,92 @@
 namespace hj {
 
 constexpr int kMaxStages = 8;
-constexpr uint32_t kMaxFusedSlot = 64;  // longest directory slot the pipeline accepts
-constexpr uint32_t kDead = 0xffffffffu; // a survivor that failed a later join
+constexpr uint32_t kDead = 0xffffffffu;  // no match / a survivor that failed a later join
+constexpr uint64_t kEmpty = ~0ull;       // a free table entry
+constexpr uint32_t kBloomWords = 4096;   // 16 KiB: the LDS-resident filter of a small build side
+constexpr uint32_t kMinTableBits = 6;
 
-// One build side: a (filtered) base scan turned into an unchained directory whose adjacency
-// array is grouped by slot (order inside a slot is arbitrary; keys are expected unique).
+// table bits for n build tuples: 2^bits >= 2n entries (load factor <= 1/2)
+__host__ __device__ inline uint32_t table_bits(uint32_t n) {
+    uint32_t k = 0;
+    while ((1ull << k) < (uint64_t)n) ++k;
+    return k + 1 < kMinTableBits ? kMinTableBits : k + 1;
+}
+
+// One build side: the rows of a (filtered) base scan with a non-NULL key, inserted into an
+// open-addressing table (linear probing) of entries row << 32 | key.
 struct BuildJob {
     DCol key;
     DCol fcol[kMaxFilters];
     int32_t flo[kMaxFilters];
     int32_t fhi[kMaxFilters];
     int32_t nf;
-    uint32_t page_block0;     // first block of this job in page-parallel grids
-    uint32_t kmax;            // slot arrays hold 2^kmax (+1) entries
-    uint32_t* meta;           // [0] tuples, [1] unused, [2] longest slot
-    uint32_t* blk;            // per key page: passing rows
-    uint32_t* blk_off;        // per key page: exclusive prefix of blk
-    uint32_t* cnt;            // per slot
-    uint32_t* bloom;          // per slot
-    uint64_t* dir;            // 2^kmax + 1 words: start << 32 | min(count, 0xffff) << 16 | Bloom
-    uint32_t* part;           // slot-scan partials (one per 4096 slots)
-    uint32_t* st_h;           // staged tuples at their page's first row (capacity: table rows)
-    uint32_t* st_row;
-    uint32_t* st_pos;         // position of the tuple inside its slot
-    uint64_t* adj;            // adjacency array grouped by slot: row << 32 | hash
-    uint32_t* bf;             // blocked Bloom filter, kBloomWords words (used for small builds)
-    uint64_t rows;            // table rows (upper bound of the tuple count)
+    uint32_t rows;      // table rows (bound of the tuple count)
+    uint32_t kmax;      // the table holds 2^kmax entries
+    uint32_t* meta;     // [0] tuples, [1] a key occurs twice, [2..3] unused
+    uint32_t* st_key;   // staged tuples, each block of the filter pass at the start of its
+    uint32_t* st_row;   // own row range (capacity: rows rounded up to whole ranges)
+    uint32_t* blk;      // tuples staged per block of the filter pass
+    uint64_t* table;    // 2^table_bits(tuples) entries used
+    uint32_t* bf;       // blocked Bloom filter, kBloomWords words (small build sides only)
 };
-
-constexpr uint32_t kBloomWords = 4096;  // 16 KiB: the LDS-resident filter of a build side
 
 struct BuildBatch {
     BuildJob job[kMaxStages];
     int32_t njobs;
-    uint32_t page_blocks;
-    uint32_t grid;            // blocks of the grid-stride kernels
+    uint32_t grid;      // blocks per job of the grid-stride kernels
 };
 
 struct PipeStage {
-    int32_t job;       // build job of this join
-    int32_t key_slot;  // 0: key is a column of the probe source; j+1: of stage j's build table
+    int32_t job;        // build job of this join
+    int32_t key_slot;   // 0: key is a column of the probe source; j+1: of stage j's build table
     DCol keycol;
 };
 
 struct PipeDesc {
-    DCol driver;                  // source column holding stage 0's key; one block per page
     DCol fcol[kMaxFilters];       // filters of the source scan
     int32_t flo[kMaxFilters];
     int32_t fhi[kMaxFilters];
     int32_t nf;
     PipeStage st[kMaxStages];
     int32_t nst;
+    int32_t nstream;              // stages keyed by the source, fused into one pass ...
+    int32_t order[kMaxStages];    // ... in this order (smallest build first)
+    int32_t qpos[kMaxStages];     // stage -> its position in order (-1: not streamed)
+    int32_t ndep;                 // stages keyed by a build table (run over the survivors)
+    uint32_t src_rows;
     int32_t nout;
     int32_t out_slot[kMaxSlots];  // 0: source row, j+1: build row of stage j
     uint32_t* final_out[kMaxSlots];
-    uint32_t* s0_row;             // stage-0 survivors at their page's first row
-    uint32_t* s0_b;
-    uint32_t* page_cnt;           // per driver page
-    uint32_t* page_off;
-    uint32_t* l_row;              // dense survivor list after stage 0
+    uint32_t* stage[kMaxStages + 1];  // survivors of the fused pass, slot values (source row,
+                                      // then build row per stage) at the start of each block's
+                                      // range; compacted by a scan of the block counts
+    uint32_t* blk_cnt;            // survivors per block of the fused pass
+    uint32_t* blk_off;
+    uint32_t probe_blocks;        // blocks of the fused pass ...
+    uint32_t probe_gpb;           // ... and 64-row groups per block
+    uint32_t* l_row;              // ndep > 0: survivors of the fused pass
     uint32_t* l_b[kMaxStages];    // build row per stage, kDead once a join failed
     uint32_t* tile_cnt;           // final compaction, per 4096 survivors
     uint32_t* tile_off;
-    uint32_t* meta;               // [0] a key matched more than one build tuple,
-                                  // [1] a directory slot exceeded kMaxFusedSlot,
-                                  // [2] stage-0 survivors, [3] output rows
+    uint32_t* meta;               // [0] fall back (a build key occurs twice), [1] unused,
+                                  // [2] survivors of the fused pass, [3] output rows
     const BuildJob* jobs;
     uint32_t grid;                // blocks of the grid-stride kernels
-    // multi-join chains: joins keyed by the source run as streaming passes over their key
-    // column in `order` (smallest build first), exchanging survivor bitmaps; the remaining
-    // joins (keys from a build table) run over the compacted survivors afterwards
-    int32_t nstream;
-    int32_t order[kMaxStages];
-    uint32_t* bm[2];              // survivor bitmaps, one bit per source row
-    uint32_t bm_words;
 };
 
-// batch build: extract + stage, per-job page scan, clear, slot counts + Bloom bits,
-// slot scan -> directory, scatter
+// 64-row groups per wave and iteration of the fused pass over NS streamed joins
+constexpr int probe_unroll(int ns) { return ns == 1 ? 8 : (ns == 2 ? 6 : (ns <= 4 ? 4 : 2)); }
+// rows per block-range of the build filter pass
+constexpr uint32_t kBuildTile = 1024;
+__host__ __device__ inline uint32_t build_range(uint32_t rows, uint32_t blocks) {
+    const uint32_t tiles = (rows + kBuildTile - 1) / kBuildTile;
+    return ((tiles + blocks - 1) / blocks) * kBuildTile;
+}
+
+// build: filter + stage, clear tables, insert
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s);
-// stage 0 over the driver pages, survivor compaction, later stages, final compaction
+// fused pass over the source, then the joins keyed by build tables and the final compaction
 void pipe_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s);
 
 }  // namespace hj
