@@ -26,6 +26,16 @@
 
 namespace hj {
 
+// Pointers read from descriptors in device memory are generic ("flat") to the compiler, and
+// flat loads count against both the vector-memory and the LDS/scalar wait counters, so a wait
+// for an LDS read also waits for every HBM load in flight. gp() marks such a pointer as global
+// so that its loads and stores are global_* instructions.
+#define HJ_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline HJ_GLOBAL T* gp(T* p) {
+    return (HJ_GLOBAL T*)p;
+}
+
 struct DCol {
     const uint8_t* pages;
     const uint32_t* row_start;
@@ -61,8 +71,8 @@ __host__ __device__ inline uint32_t slot_of(uint32_t h, uint32_t k) {
 }
 
 // ---------------------------------------------------------------- page access
-__device__ inline const uint8_t* page_ptr(const DCol& c, uint32_t p) {
-    return c.pages + (size_t)p * HJ_PAGE_BYTES;
+__device__ inline const HJ_GLOBAL uint8_t* page_ptr(const DCol& c, uint32_t p) {
+    return gp(c.pages) + (size_t)p * HJ_PAGE_BYTES;
 }
 
 #define HJ_CHUNK_SHIFT 11
@@ -74,11 +84,11 @@ __device__ inline uint32_t find_page(const DCol& c, uint32_t r) {
         return p < c.num_pages ? p : c.num_pages - 1;
     }
     const uint32_t ch = r >> HJ_CHUNK_SHIFT;
-    uint32_t lo = c.chunk_page[ch], hi = c.chunk_page[ch + 1] + 1;
+    uint32_t lo = gp(c.chunk_page)[ch], hi = gp(c.chunk_page)[ch + 1] + 1;
     if (hi > c.num_pages) hi = c.num_pages;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
-        if (c.row_start[mid] <= r)
+        if (gp(c.row_start)[mid] <= r)
             lo = mid;
         else
             hi = mid;
@@ -90,20 +100,25 @@ __device__ inline uint32_t find_page(const DCol& c, uint32_t r) {
 // two aligned 8-byte loads and a funnel shift (bits past the page's last row are garbage and are
 // never used: callers test bit (local & 63) and count bits below it). The second word is only
 // loaded while it lies inside the page.
-__device__ inline uint64_t group_bits(const uint8_t* pg, uint32_t nb, uint32_t g) {
-    const uintptr_t a = (uintptr_t)(pg + HJ_PAGE_BYTES - nb + 8u * g);
+__device__ inline uint64_t group_bits(const HJ_GLOBAL uint8_t* pg, uint32_t nb, uint32_t g) {
+    // 64-bit address arithmetic on a generic pointer (flat loads). Forms that loaded the word
+    // through a global pointer at a 32-bit page offset returned wrong words for some lanes now
+    // and then on gfx950 when all lanes of a wave read the same word (reproduced outside the
+    // engine, with and without forced memory waits); this form was exact in every run.
+    const uintptr_t a = (uintptr_t)((const uint8_t*)pg + HJ_PAGE_BYTES - nb + 8u * g);
     const uint64_t* w = (const uint64_t*)(a & ~(uintptr_t)7);
     const uint32_t sh = (uint32_t)(a & 7) * 8u;
     const uint64_t w0 = w[0];
     if (sh == 0) return w0;
-    const uint64_t w1 = ((uintptr_t)(w + 1) < (uintptr_t)(pg + HJ_PAGE_BYTES)) ? w[1] : 0ull;
+    const uint64_t w1 =
+        ((uintptr_t)(w + 1) < (uintptr_t)((const uint8_t*)pg + HJ_PAGE_BYTES)) ? w[1] : 0ull;
     return (w0 >> sh) | (w1 << (64u - sh));
 }
 
 // value index of local row `local` in page p (header word hdr = nrows | nnonnull << 16);
 // returns false for a NULL row. In a sequential scan the 64 lanes of a wave share one group,
 // so the group word and its rank are one broadcast load each.
-__device__ inline bool value_index(const DCol& c, uint32_t p, const uint8_t* pg, uint32_t hdr,
+__device__ inline bool value_index(const DCol& c, uint32_t p, const HJ_GLOBAL uint8_t* pg, uint32_t hdr,
                                    uint32_t local, uint32_t* vi) {
     const uint32_t nr = hdr & 0xffffu, nn = hdr >> 16;
     if (nn == nr) {
@@ -113,36 +128,37 @@ __device__ inline bool value_index(const DCol& c, uint32_t p, const uint8_t* pg,
     const uint32_t g = local >> 6, b = local & 63u;
     const uint64_t bits = group_bits(pg, (nr + 7) >> 3, g);
     if (!((bits >> b) & 1ull)) return false;
-    *vi = (uint32_t)c.rank[c.group_start[p] + g] + (uint32_t)__popcll(bits & ((1ull << b) - 1ull));
+    *vi = (uint32_t)gp(c.rank)[gp(c.group_start)[p] + g] +
+          (uint32_t)__popcll(bits & ((1ull << b) - 1ull));
     return true;
 }
 
 // validity of row r of an INT32 column's row-space mirror
 __device__ inline bool flat_valid(const DCol& c, uint32_t r) {
-    return !c.vbits || ((c.vbits[r >> 6] >> (r & 63u)) & 1ull);
+    return !c.vbits || ((gp(c.vbits)[r >> 6] >> (r & 63u)) & 1ull);
 }
 
 // INT32 value of global row r; false for NULL
 __device__ inline bool get_int32(const DCol& c, uint32_t r, int32_t* v) {
     if (c.flat) {
-        *v = c.flat[r];
+        *v = gp(c.flat)[r];
         return flat_valid(c, r);
     }
     uint32_t p = find_page(c, r);
-    const uint8_t* pg = page_ptr(c, p);
-    uint32_t hdr = *(const uint32_t*)pg;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(c, p);
+    uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
     uint32_t vi;
-    if (!value_index(c, p, pg, hdr, r - c.row_start[p], &vi)) return false;
-    *v = *(const int32_t*)(pg + 4 + 4 * vi);
+    if (!value_index(c, p, pg, hdr, r - gp(c.row_start)[p], &vi)) return false;
+    *v = *(const HJ_GLOBAL int32_t*)(pg + 4 + 4 * vi);
     return true;
 }
 
 // INT32 value of local row `local` of a known page
-__device__ inline bool get_int32_in_page(const DCol& c, uint32_t p, const uint8_t* pg,
+__device__ inline bool get_int32_in_page(const DCol& c, uint32_t p, const HJ_GLOBAL uint8_t* pg,
                                          uint32_t hdr, uint32_t local, int32_t* v) {
     uint32_t vi;
     if (!value_index(c, p, pg, hdr, local, &vi)) return false;
-    *v = *(const int32_t*)(pg + 4 + 4 * vi);
+    *v = *(const HJ_GLOBAL int32_t*)(pg + 4 + 4 * vi);
     return true;
 }
 
@@ -152,27 +168,27 @@ __device__ inline bool get_int32_in_page(const DCol& c, uint32_t p, const uint8_
 __device__ inline bool get_string(const DCol& c, uint32_t r, const uint8_t** src, uint32_t* len,
                                   uint32_t* long_page) {
     uint32_t p = find_page(c, r);
-    const uint8_t* pg = page_ptr(c, p);
-    uint32_t hdr = *(const uint32_t*)pg;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(c, p);
+    uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
     uint32_t nr = hdr & 0xffffu;
     if (nr == 0xffffu) {
         uint32_t total = hdr >> 16;
         for (uint32_t q = p + 1; q < c.num_pages; ++q) {
-            uint32_t h2 = *(const uint32_t*)page_ptr(c, q);
+            uint32_t h2 = *(const HJ_GLOBAL uint32_t*)page_ptr(c, q);
             if ((h2 & 0xffffu) != 0xfffeu) break;
             total += h2 >> 16;
         }
-        *src = pg + 4;
+        *src = (const uint8_t*)(pg + 4);
         *len = total;
         *long_page = p;
         return true;
     }
     uint32_t vi;
-    if (!value_index(c, p, pg, hdr, r - c.row_start[p], &vi)) return false;
+    if (!value_index(c, p, pg, hdr, r - gp(c.row_start)[p], &vi)) return false;
     uint32_t nn = hdr >> 16;
-    const uint16_t* offs = (const uint16_t*)(pg + 4);
+    const HJ_GLOBAL uint16_t* offs = (const HJ_GLOBAL uint16_t*)(pg + 4);
     uint32_t start = vi ? offs[vi - 1] : 0u;
-    *src = pg + 4 + 2 * nn + start;
+    *src = (const uint8_t*)(pg + 4 + 2 * nn + start);
     *len = (uint32_t)offs[vi] - start;
     *long_page = 0xffffffffu;
     return true;

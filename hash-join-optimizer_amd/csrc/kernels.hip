@@ -99,8 +99,8 @@ void zero_u32(uint32_t* p, uint64_t n, hipStream_t s) { fill_u32(p, n, 0u, s); }
 // =====================================================================================
 __global__ __launch_bounds__(64) void k_rank_index(DCol c, uint16_t* rank_out) {
     const uint32_t p = blockIdx.x;
-    const uint8_t* pg = page_ptr(c, p);
-    const uint32_t hdr = *(const uint32_t*)pg;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(c, p);
+    const uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
     const uint32_t nr = hdr & 0xffffu, nn = hdr >> 16;
     if (c.type == 3 /*VARCHAR*/ && (nr == 0xffffu || nr == 0xfffeu)) return;
     if (nr == nn) return;
@@ -127,8 +127,8 @@ void launch_rank_index(const DCol& c, uint16_t* rank_out, hipStream_t s) {
 // 64-aligned, so a wave touches up to two words
 __global__ __launch_bounds__(kBlock) void k_decode_int32(DCol c, int32_t* flat, uint64_t* vbits) {
     const uint32_t p = blockIdx.x;
-    const uint8_t* pg = page_ptr(c, p);
-    const uint32_t hdr = *(const uint32_t*)pg;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(c, p);
+    const uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
     const uint32_t nr = hdr & 0xffffu;
     const uint32_t r0 = c.row_start[p];
     for (uint32_t l0 = 0; l0 < nr; l0 += kBlock) {
@@ -502,7 +502,7 @@ __device__ inline uint32_t dir_lookup(const DirView& d, uint32_t key, uint32_t* 
 }
 
 // key of local row `local` of key page p, after the scan's filters; false = row dropped
-__device__ inline bool scan_row_key(const ScanSrc& s, uint32_t p, const uint8_t* pg, uint32_t hdr,
+__device__ inline bool scan_row_key(const ScanSrc& s, uint32_t p, const HJ_GLOBAL uint8_t* pg, uint32_t hdr,
                                     uint32_t local, uint32_t r, uint32_t* key) {
     int32_t k;
     if (!get_int32_in_page(s.key, p, pg, hdr, local, &k)) return false;
@@ -560,8 +560,8 @@ __global__ __launch_bounds__(kBlock) void k_extract_scan(ScanSrc src, uint32_t* 
                                                          uint32_t* out_row) {
     __shared__ uint32_t lds[kBlock / HJ_WAVE];
     const uint32_t p = blockIdx.x;
-    const uint8_t* pg = page_ptr(src.key, p);
-    const uint32_t hdr = *(const uint32_t*)pg;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(src.key, p);
+    const uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
     const uint32_t nr = hdr & 0xffffu;
     const uint32_t r0 = src.key.row_start[p];
     uint32_t run = WRITE ? offsets[p] : 0u;
@@ -608,8 +608,8 @@ __global__ __launch_bounds__(kBlock) void k_select_scan(ScanSrc src, uint32_t* c
                                                         uint32_t* out_row) {
     __shared__ uint32_t lds[kBlock / HJ_WAVE];
     const uint32_t p = blockIdx.x;
-    const uint8_t* pg = page_ptr(src.key, p);
-    const uint32_t hdr = *(const uint32_t*)pg;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(src.key, p);
+    const uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
     uint32_t nr = hdr & 0xffffu;
     if (src.key.type == 3 && nr == 0xffffu) nr = 1;
     if (src.key.type == 3 && nr == 0xfffeu) nr = 0;
@@ -680,8 +680,8 @@ __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, u
                                                        HeavyEntry* heavy, uint32_t* heavy_count) {
     __shared__ uint64_t lds[kBlock / HJ_WAVE];
     const uint32_t p = blockIdx.x;
-    const uint8_t* pg = page_ptr(src.key, p);
-    const uint32_t hdr = *(const uint32_t*)pg;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(src.key, p);
+    const uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
     const uint32_t nr = hdr & 0xffffu;
     const uint32_t r0 = src.key.row_start[p];
     if (!WRITE) {
@@ -824,8 +824,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_strcopy(DCol c, const uint32_
     // long string: the 0xffff page and its 0xfffe continuation pages, in order
     uint32_t q = lp, done = 0;
     while (done < len && q < c.num_pages) {
-        const uint8_t* pg = page_ptr(c, q);
-        const uint32_t h = *(const uint32_t*)pg;
+        const HJ_GLOBAL uint8_t* pg = page_ptr(c, q);
+        const uint32_t h = *(const HJ_GLOBAL uint32_t*)pg;
         if (q != lp && (h & 0xffffu) != 0xfffeu) break;
         const uint32_t l = h >> 16;
         for (uint32_t j = 0; j < l && done + j < len; ++j) dst[done + j] = (char)pg[4 + j];

@@ -58,7 +58,7 @@ void encode_int32(const int32_t* values, const ValidFn& valid, uint64_t n, hj_ou
     PageSink sink;
     uint32_t rows = 0, nvals = 0;
     int32_t vals[2048];
-    uint8_t bits[P / 8 + 8];
+    uint8_t bits[P];  // a page of NULLs holds up to 8 * (P - 5) rows
     std::memset(bits, 0, sizeof bits);
     auto flush = [&]() {
         uint8_t* pg = sink.fresh();
@@ -90,7 +90,7 @@ void encode_varchar(const uint64_t* offsets, const char* chars, const ValidFn& v
     uint32_t rows = 0, nstr = 0, nchars = 0;
     uint16_t ends[P / 2];
     uint8_t data[P];
-    uint8_t bits[P / 8 + 8];
+    uint8_t bits[P];  // a page of NULLs holds up to 8 * (P - 5) rows
     std::memset(bits, 0, sizeof bits);
     auto flush = [&]() {
         uint8_t* pg = sink.fresh();

@@ -23,6 +23,7 @@ namespace {
 
 constexpr int kB = 256;
 constexpr uint32_t kNW = kB / HJ_WAVE;
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));  // one 16-byte load or store
 
 __device__ inline uint32_t log2ceil(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
@@ -84,7 +85,7 @@ __device__ inline uint32_t home_slot(uint32_t key, uint32_t k) { return fib32(ke
 
 // build row of `key`, kDead when absent, given the two entries e0, e1 at the even slot at or
 // below its home slot s; a run past them (rare at load factor <= 1/2) continues entry by entry
-__device__ inline uint32_t tab_resolve(const uint64_t* T, uint32_t mask, uint32_t s, uint32_t key,
+__device__ inline uint32_t tab_resolve(const HJ_GLOBAL uint64_t* T, uint32_t mask, uint32_t s, uint32_t key,
                                        uint64_t e0, uint64_t e1) {
     if (!(s & 1u)) {
         if (e0 == kEmpty) return kDead;
@@ -99,8 +100,8 @@ __device__ inline uint32_t tab_resolve(const uint64_t* T, uint32_t mask, uint32_
     }
 }
 
-__device__ inline void tab_pair(const uint64_t* T, uint32_t s, uint64_t* e0, uint64_t* e1) {
-    const ulonglong2 v = *(const ulonglong2*)(T + (s & ~1u));
+__device__ inline void tab_pair(const HJ_GLOBAL uint64_t* T, uint32_t s, uint64_t* e0, uint64_t* e1) {
+    const u64x2 v = *(const HJ_GLOBAL u64x2*)(T + (s & ~1u));
     *e0 = v.x;
     *e1 = v.y;
 }
@@ -161,19 +162,19 @@ __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__
 #pragma unroll
         for (int u = 0; u < kBU; ++u) {
             if (ok[u]) {
-                jb.st_key[r0 + pos[u]] = (uint32_t)key[u];
-                jb.st_row[r0 + pos[u]] = r[u];
+                gp(jb.st_key)[r0 + pos[u]] = (uint32_t)key[u];
+                gp(jb.st_row)[r0 + pos[u]] = r[u];
             }
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) jb.blk[blockIdx.x] = s_cnt;
+    if (threadIdx.x == 0) gp(jb.blk)[blockIdx.x] = s_cnt;
 }
 
 // tuple count of a job = sum of the filter pass's block counts
 __device__ inline uint32_t job_tuples(const BuildJob& jb, uint32_t blocks, uint32_t* lds) {
     uint32_t c = 0;
-    for (uint32_t i = threadIdx.x; i < blocks; i += kB) c += jb.blk[i];
+    for (uint32_t i = threadIdx.x; i < blocks; i += kB) c += gp(jb.blk)[i];
     return bsum_u32<kNW>(c, lds);
 }
 
@@ -181,41 +182,42 @@ __global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ 
     __shared__ uint32_t lds[kNW];
     const BuildJob& jb = bb->job[blockIdx.y];
     const uint32_t n = job_tuples(jb, gridDim.x, lds);
-    if (blockIdx.x == 0 && threadIdx.x == 0) jb.meta[0] = n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) gp(jb.meta)[0] = n;
     const uint32_t half = (1u << table_bits(n)) / 2u;
     const uint32_t g = blockIdx.x * kB + threadIdx.x, stride = gridDim.x * kB;
-    ulonglong2* T = (ulonglong2*)jb.table;
-    for (uint32_t i = g; i < half; i += stride) T[i] = make_ulonglong2(kEmpty, kEmpty);
+    HJ_GLOBAL u64x2* T = (HJ_GLOBAL u64x2*)gp(jb.table);
+    const u64x2 empty = {kEmpty, kEmpty};
+    for (uint32_t i = g; i < half; i += stride) T[i] = empty;
     if (bf_enabled(n)) {
         const uint32_t W = 1u << bf_log2w(n);
-        for (uint32_t i = g; i < W; i += stride) jb.bf[i] = 0;
+        for (uint32_t i = g; i < W; i += stride) gp(jb.bf)[i] = 0;
     }
 }
 
 // insert: block x of job y inserts the tuples staged by block x of the filter pass
 __global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__ bb) {
     const BuildJob& jb = bb->job[blockIdx.y];
-    const uint32_t n = jb.meta[0];
+    const uint32_t n = gp(jb.meta)[0];
     const uint32_t k = table_bits(n), mask = (1u << k) - 1u;
     const bool use_bf = bf_enabled(n);
     const uint32_t lw = bf_log2w(n);
     const uint32_t r0 = blockIdx.x * build_range(jb.rows, gridDim.x);
-    const uint32_t cnt = jb.blk[blockIdx.x];
+    const uint32_t cnt = gp(jb.blk)[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < cnt; i += kB) {
-        const uint32_t key = jb.st_key[r0 + i], row = jb.st_row[r0 + i];
+        const uint32_t key = gp(jb.st_key)[r0 + i], row = gp(jb.st_row)[r0 + i];
         const unsigned long long e = ((unsigned long long)row << 32) | key;
         for (uint32_t s = home_slot(key, k);; s = (s + 1u) & mask) {
             const unsigned long long old =
-                atomicCAS((unsigned long long*)&jb.table[s], (unsigned long long)kEmpty, e);
+                atomicCAS((unsigned long long*)&gp(jb.table)[s], (unsigned long long)kEmpty, e);
             if (old == kEmpty) break;
             if ((uint32_t)old == key) {
-                jb.meta[1] = 1;
+                gp(jb.meta)[1] = 1;
                 break;
             }
         }
         if (use_bf) {
             const uint32_t h = fib32(key);
-            atomicOr(&jb.bf[bf_word(h, lw)], bf_mask(h));
+            atomicOr((uint32_t*)&gp(jb.bf)[bf_word(h, lw)], bf_mask(h));
         }
     }
 }
@@ -232,35 +234,35 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     if (t == 0) {
         s_cnt = 0;
         uint32_t bad = 0;
-        for (int j = 0; j < pd.nst; ++j) bad |= pd.jobs[pd.st[j].job].meta[1];
+        for (int j = 0; j < pd.nst; ++j) bad |= gp((gp(pd.jobs) + pd.st[j].job)->meta)[1];
         s_bad = bad;
     }
-    const BuildJob& j0 = pd.jobs[pd.st[pd.order[0]].job];
-    const uint32_t n0 = j0.meta[0];
+    const HJ_GLOBAL BuildJob* j0p = gp(pd.jobs) + pd.st[pd.order[0]].job;
+    const uint32_t n0 = gp(j0p->meta)[0];
     const bool use_bf = bf_enabled(n0);
     const uint32_t lw = bf_log2w(n0);
     if (use_bf)
-        for (uint32_t i = t; i < (1u << lw); i += kB) bfl[i] = j0.bf[i];
+        for (uint32_t i = t; i < (1u << lw); i += kB) bfl[i] = gp(j0p->bf)[i];
     __syncthreads();
     if (s_bad) {
         if (t == 0) {
-            if (blockIdx.x == 0) pd.meta[0] = 1;
-            pd.blk_cnt[blockIdx.x] = 0;
+            if (blockIdx.x == 0) gp(pd.meta)[0] = 1;
+            gp(pd.blk_cnt)[blockIdx.x] = 0;
         }
         return;
     }
-    const int32_t* kf[NS];
-    const uint64_t* kv[NS];
-    const uint64_t* tab[NS];
+    const HJ_GLOBAL int32_t* kf[NS];
+    const HJ_GLOBAL uint64_t* kv[NS];
+    const HJ_GLOBAL uint64_t* tab[NS];
     uint32_t tk[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
         const PipeStage& ps = pd.st[pd.order[q]];
-        kf[q] = ps.keycol.flat;
-        kv[q] = ps.keycol.vbits;
-        const BuildJob& jb = pd.jobs[ps.job];
-        tab[q] = jb.table;
-        tk[q] = table_bits(jb.meta[0]);
+        kf[q] = gp(ps.keycol.flat);
+        kv[q] = gp(ps.keycol.vbits);
+        const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
+        tab[q] = gp(jp->table);
+        tk[q] = table_bits(gp(jp->meta)[0]);
     }
     const uint32_t nsrc = pd.src_rows;
     const uint32_t G = (nsrc + 63u) >> 6;
@@ -335,32 +337,33 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (!alive[u]) continue;
-            pd.stage[0][st0 + pos[u]] = r[u];
+            gp(pd.stage[0])[st0 + pos[u]] = r[u];
 #pragma unroll
-            for (int q = 0; q < NS; ++q) pd.stage[1 + q][st0 + pos[u]] = brow[q][u];
+            for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[u]] = brow[q][u];
         }
     }
     __syncthreads();
-    if (t == 0) pd.blk_cnt[blockIdx.x] = s_cnt;
+    if (t == 0) gp(pd.blk_cnt)[blockIdx.x] = s_cnt;
 }
 
 // the fused pass's staged survivors -> the outputs (no joins keyed by a build table) or the
 // dense survivor list, at the block's offset from the scan of the block counts
 __global__ __launch_bounds__(kB) void k_sp_compact(const PipeDesc* __restrict__ pdp) {
     const PipeDesc& pd = *pdp;
-    if (pd.meta[0]) return;
+    if (gp(pd.meta)[0]) return;
     const uint32_t b = blockIdx.x;
-    const uint32_t cnt = pd.blk_cnt[b], off = pd.blk_off[b];
+    const uint32_t cnt = gp(pd.blk_cnt)[b], off = gp(pd.blk_off)[b];
     const uint32_t st0 = b * pd.probe_gpb * 64u;
     for (uint32_t i = threadIdx.x; i < cnt; i += kB) {
         if (!pd.ndep) {
             for (int o = 0; o < pd.nout; ++o) {
                 const int sl = pd.out_slot[o];
-                pd.final_out[o][off + i] = pd.stage[sl ? 1 + pd.qpos[sl - 1] : 0][st0 + i];
+                gp(pd.final_out[o])[off + i] = gp(pd.stage[sl ? 1 + pd.qpos[sl - 1] : 0])[st0 + i];
             }
         } else {
-            pd.l_row[off + i] = pd.stage[0][st0 + i];
-            for (int q = 0; q < pd.nstream; ++q) pd.l_b[pd.order[q]][off + i] = pd.stage[1 + q][st0 + i];
+            gp(pd.l_row)[off + i] = gp(pd.stage[0])[st0 + i];
+            for (int q = 0; q < pd.nstream; ++q)
+                gp(pd.l_b[pd.order[q]])[off + i] = gp(pd.stage[1 + q])[st0 + i];
         }
     }
 }
@@ -370,14 +373,16 @@ __global__ __launch_bounds__(kB) void k_sp_compact(const PipeDesc* __restrict__ 
 __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pdp, int s,
                                                  int previous) {
     const PipeDesc& pd = *pdp;
-    if (pd.meta[0]) return;
-    const uint32_t n = pd.meta[2];
+    if (gp(pd.meta)[0]) return;
+    const uint32_t n = gp(pd.meta)[2];
     const PipeStage& ps = pd.st[s];
-    const BuildJob& jb = pd.jobs[ps.job];
-    const uint32_t k = table_bits(jb.meta[0]), mask = (1u << k) - 1u;
-    const uint32_t* prev = previous < 0 ? nullptr : pd.l_b[previous];
-    const uint32_t* src = ps.key_slot == 0 ? pd.l_row : pd.l_b[ps.key_slot - 1];
-    uint32_t* outb = pd.l_b[s];
+    const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
+    const HJ_GLOBAL uint64_t* table = gp(jp->table);
+    const uint32_t k = table_bits(gp(jp->meta)[0]), mask = (1u << k) - 1u;
+    const HJ_GLOBAL uint32_t* prev = previous < 0 ? nullptr : gp(pd.l_b[previous]);
+    const HJ_GLOBAL uint32_t* src =
+        ps.key_slot == 0 ? gp(pd.l_row) : gp(pd.l_b[ps.key_slot - 1]);
+    HJ_GLOBAL uint32_t* outb = gp(pd.l_b[s]);
     constexpr int U = 4;
     const uint32_t stride = gridDim.x * kB * U;
     for (uint32_t base = blockIdx.x * kB * U; base < n; base += stride) {
@@ -401,12 +406,12 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
         for (int u = 0; u < U; ++u) {
             hs[u] = home_slot((uint32_t)key[u], k);
             e0[u] = e1[u] = kEmpty;
-            if (ok[u]) tab_pair(jb.table, hs[u], &e0[u], &e1[u]);
+            if (ok[u]) tab_pair(table, hs[u], &e0[u], &e1[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (jx[u] >= n) continue;
-            outb[jx[u]] = ok[u] ? tab_resolve(jb.table, mask, hs[u], (uint32_t)key[u], e0[u], e1[u])
+            outb[jx[u]] = ok[u] ? tab_resolve(table, mask, hs[u], (uint32_t)key[u], e0[u], e1[u])
                                 : kDead;
         }
     }
@@ -443,15 +448,15 @@ __global__ __launch_bounds__(1024) void k_scan1_u32(const uint32_t* in, uint32_t
 __global__ __launch_bounds__(kB) void k_pp_fcount(const PipeDesc* __restrict__ pdp, int lastj) {
     __shared__ uint32_t lds[kB / HJ_WAVE];
     const PipeDesc& pd = *pdp;
-    const uint32_t n = pd.meta[2];
+    const uint32_t n = gp(pd.meta)[2];
     const uint32_t nt = (n + 4095u) / 4096u;
-    const uint32_t* last = lastj < 0 ? nullptr : pd.l_b[lastj];
+    const HJ_GLOBAL uint32_t* last = lastj < 0 ? nullptr : gp(pd.l_b[lastj]);
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         uint32_t c = 0;
         for (uint32_t i = t * 4096u + threadIdx.x; i < (t + 1) * 4096u && i < n; i += kB)
             c += !last || last[i] != kDead;
         const uint32_t tot = bsum_u32<kB / HJ_WAVE>(c, lds);
-        if (threadIdx.x == 0) pd.tile_cnt[t] = tot;
+        if (threadIdx.x == 0) gp(pd.tile_cnt)[t] = tot;
     }
 }
 
@@ -459,11 +464,11 @@ __global__ __launch_bounds__(kB) void k_pp_fcount(const PipeDesc* __restrict__ p
 __global__ __launch_bounds__(kB) void k_pp_fwrite(const PipeDesc* __restrict__ pdp, int lastj) {
     __shared__ uint32_t lds[kB / HJ_WAVE];
     const PipeDesc& pd = *pdp;
-    const uint32_t n = pd.meta[2];
+    const uint32_t n = gp(pd.meta)[2];
     const uint32_t nt = (n + 4095u) / 4096u;
-    const uint32_t* last = lastj < 0 ? nullptr : pd.l_b[lastj];
+    const HJ_GLOBAL uint32_t* last = lastj < 0 ? nullptr : gp(pd.l_b[lastj]);
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        uint32_t run = pd.tile_off[t];
+        uint32_t run = gp(pd.tile_off)[t];
         for (uint32_t c0 = 0; c0 < 4096u; c0 += kB) {
             const uint32_t i = t * 4096u + c0 + threadIdx.x;
             const bool alive = i < n && (!last || last[i] != kDead);
@@ -472,7 +477,7 @@ __global__ __launch_bounds__(kB) void k_pp_fwrite(const PipeDesc* __restrict__ p
             if (alive) {
                 for (int q = 0; q < pd.nout; ++q) {
                     const int sl = pd.out_slot[q];
-                    pd.final_out[q][o] = sl == 0 ? pd.l_row[i] : pd.l_b[sl - 1][i];
+                    gp(pd.final_out[q])[o] = sl == 0 ? gp(pd.l_row)[i] : gp(pd.l_b[sl - 1])[i];
                 }
             }
             run += tot;

@@ -116,6 +116,33 @@ def expected_pairs(bk, bp, bv, pk, pp, pv):
     return p[np.lexsort((p[:, 1], p[:, 0]))]
 
 
+def test_resident_int32_mirror(gpu_ctx, oracle):
+    """the row-space INT32 mirror decoded at upload (values + validity) over 1M rows with 5 %
+    NULLs, runs of NULLs and an all-NULL stretch, read back through a root scan's gathers"""
+    rng = np.random.default_rng(17)
+    n = 1_000_000
+    v = rng.integers(-2**31, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+    ok = (rng.random(n) >= 0.05).astype(np.uint8)
+    ok[300_000:420_000] = 0                       # all-NULL pages
+    ok[700_000:700_100] = 0
+    I = hjtest.DataType.INT32
+    plan = hjtest.Plan()
+    plan.inputs.append(hjtest.ColumnarTable(n, [oracle.codec.encode_int32(v, ok)]))
+    plan.new_scan_node(0, [(0, I)])
+    plan.root = 0
+    dt = gpu_ctx.upload(plan.inputs[0])
+    try:
+        for _ in range(2):
+            r = gpu_ctx.execute_device(plan, [dt])
+            res = r.to_host()
+            r.free()
+            got, gok = oracle.codec.decode(res.columns[0])
+            assert np.array_equal(gok.astype(np.uint8), ok)
+            assert np.array_equal(got[ok == 1], v[ok == 1])
+    finally:
+        dt.free()
+
+
 def test_two_way_unique_build_1m(gpu_ctx, oracle, mode):
     """the shape of the 2-way config at 1/10 size: unique shuffled build keys, 5 % NULL FKs"""
     rng = np.random.default_rng(42)
