@@ -161,15 +161,20 @@ def main():
     dts = [ctx.upload(tb) for tb in tables]
     log(f"uploaded {len(dts)} tables in {time.time() - t:.1f}s")
 
-    # warmup with every kernel timed, to find the dominant one
-    ctx.profile("")
+    # untimed warmup, then a few steps with every engine kernel timed to find the dominant one
     for _ in range(args.warmup):
         r = ctx.execute_device(plan, dts)
         got = r.num_rows
         r.free()
     assert got == info["out_rows"], f"output rows {got} != expected {info['out_rows']}"
+    ctx.synchronize()
+    ctx.profile("")
+    nprof = 3
+    for _ in range(nprof):
+        r = ctx.execute_device(plan, dts)
+        r.free()
     prof = ctx.profile_report()
-    per_step = {k: (n / args.warmup, ms / args.warmup) for k, (n, ms) in prof.items()}
+    per_step = {k: (n / nprof, ms / nprof) for k, (n, ms) in prof.items()}
     for k, (n, ms) in sorted(per_step.items(), key=lambda x: -x[1][1]):
         log(f"  {k:22s} {n:5.1f} launches/step {ms:8.3f} ms/step")
     modeled = [k for k in per_step if kernel_bytes(k, info, 1) is not None]

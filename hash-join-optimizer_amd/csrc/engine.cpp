@@ -29,8 +29,13 @@
 #include "kernels.hpp"
 #include "pagecodec.hpp"
 #include "pipeline.hpp"
+#include "materialize.hpp"
 
 namespace hj {
+
+// string bytes of an output column are allocated from rows x longest string (no host round trip)
+// up to this size; beyond it they are counted first
+constexpr uint64_t kCharsBound = 1ull << 30;
 
 namespace {
 
@@ -422,7 +427,7 @@ class Executor {
             if (bcol.type != HJ_INT32 || pcol.type != HJ_INT32)
                 throw Error(HJ_E_TYPE, "Join key isn't of type int32_t");
             if (i == 0 && pk.slot != 0) return false;
-            if (brel.n >= (1ull << 30)) return false;  // table index bits
+            if (brel.n >= (1ull << 28)) return false;  // key slots of the table
 
             BuildJob& jb = hb.job[i];
             jb.key = bcol.view();
@@ -441,7 +446,8 @@ class Executor {
             jb.st_key = scratch<uint32_t>(scap);
             jb.st_row = scratch<uint32_t>(scap);
             jb.blk = scratch<uint32_t>(bgrid);
-            jb.table = scratch<uint64_t>(1ull << jb.kmax);
+            jb.tkey = scratch<uint32_t>(1ull << jb.kmax);
+            jb.trow = scratch<uint32_t>((1ull << jb.kmax) + 1);
             jb.bf = scratch<uint32_t>(kBloomWords);
 
             PipeStage& st = hp.st[i];
@@ -511,7 +517,7 @@ class Executor {
             // fused pass: contiguous ranges of 64-row groups per block
             const int U = probe_unroll(hp.nstream);
             const uint64_t groups = (nsrc + 63) / 64;
-            const uint64_t maxb = (uint64_t)ctx_->num_cus * 8;
+            const uint64_t maxb = probe_resident_blocks(hp.nstream, (uint32_t)ctx_->num_cus);
             const uint64_t per = (uint64_t)U * 4;  // groups per block and iteration
             uint64_t gpb = std::max<uint64_t>(per, (groups + maxb - 1) / maxb);
             gpb = (gpb + per - 1) / per * per;
@@ -600,39 +606,66 @@ class Executor {
         auto* res = new hj_dresult();
         res->num_rows = r.n;
         const uint64_t n = r.n;
+        const uint64_t vbytes = ((n + 63) / 64) * 8;
+        const uint32_t tiles = (uint32_t)((n + kMatTile - 1) / kMatTile);
+        std::vector<MatCol> mcols;
         for (size_t i = 0; i < root.num_output_attrs; ++i) {
             const AttrRef& a = r.attrs[i];
             const Slot& sl = r.slots[a.slot];
             const DevColumn& dc = column(sl.table, a.col);
-            hj_dresult::Col c;
-            c.type = dc.type;
             if (root.output_attrs[i].type != dc.type)
                 throw Error(HJ_E_TYPE, "output attribute type does not match its column");
-            const uint64_t vbytes = ((n + 63) / 64) * 8;
+            if (dc.type != HJ_INT32 && dc.type != HJ_VARCHAR)
+                throw Error(HJ_E_TYPE, "unsupported output column type (only INT32 and VARCHAR)");
+            hj_dresult::Col c;
+            c.type = dc.type;
+            c.validity = result<uint8_t>(vbytes ? vbytes : 8);
+            MatCol m;
+            std::memset(&m, 0, sizeof m);
+            m.col = dc.view();
+            m.rid = sl.rid;
+            m.type = dc.type;
+            m.validity = c.validity;
             if (dc.type == HJ_INT32) {
                 c.values = result<int32_t>(n ? n : 1);
-                c.validity = result<uint8_t>(vbytes ? vbytes : 8);
-                { Prof p_(ctx_, "gather_int32"); gather_int32(dc.view(), sl.rid, n, c.values, c.validity, s_); }
-            } else if (dc.type == HJ_VARCHAR) {
-                c.offsets = result<uint64_t>(n + 1);
-                c.validity = result<uint8_t>(vbytes ? vbytes : 8);
-                if (n) {
-                    const size_t mark = A_.mark();
-                    uint64_t* lens = scratch<uint64_t>(n);
-                    uint64_t* part = scratch<uint64_t>(scan_partials_needed(n));
-                    { Prof p_(ctx_, "gather_strlen"); gather_strlen(dc.view(), sl.rid, n, lens, c.validity, s_); }
-                    { Prof p_(ctx_, "scan"); scan_excl_sum_u64(lens, c.offsets, n, part, c.offsets + n, s_); }
-                    c.num_chars = fetch(ctx_, c.offsets + n);
-                    A_.reset(mark);
-                } else {
-                    HJ_HIP(hipMemsetAsync(c.offsets, 0, sizeof(uint64_t), s_));
-                }
-                c.chars = result<char>(c.num_chars ? c.num_chars : 1);
-                { Prof p_(ctx_, "gather_strcopy"); gather_strcopy(dc.view(), sl.rid, n, c.offsets, c.chars, s_); }
+                m.values = c.values;
             } else {
-                throw Error(HJ_E_TYPE, "unsupported output column type (only INT32 and VARCHAR)");
+                c.offsets = result<uint64_t>(n + 1);
+                m.offsets = c.offsets;
+                m.tile_sum = scratch<uint64_t>((uint64_t)tiles + 1);
+                m.src = scratch<uint64_t>(n ? n : 1);
+                if (!n) HJ_HIP(hipMemsetAsync(c.offsets, 0, sizeof(uint64_t), s_));
             }
             res->cols.push_back(c);
+            mcols.push_back(m);
+        }
+        if (!n) {
+            res->live = true;
+            return res;
+        }
+        // string bytes: bounded by rows x longest string when that is small, else counted first
+        for (size_t b0 = 0; b0 < mcols.size(); b0 += kMatCols) {
+            MatBatch mb;
+            std::memset(&mb, 0, sizeof mb);
+            mb.n = n;
+            mb.tiles = tiles;
+            mb.ncol = (int32_t)std::min<size_t>(kMatCols, mcols.size() - b0);
+            for (int j = 0; j < mb.ncol; ++j) mb.c[j] = mcols[b0 + j];
+            { Prof p_(ctx_, "mat_gather"); mat_gather(mb, s_); }
+            bool any_str = false;
+            for (int j = 0; j < mb.ncol; ++j) {
+                if (mb.c[j].type != HJ_VARCHAR) continue;
+                any_str = true;
+                const DevColumn& dc = column(r.slots[r.attrs[b0 + j].slot].table,
+                                             r.attrs[b0 + j].col);
+                const uint64_t bound = n * (uint64_t)dc.max_len;
+                uint64_t cap = bound;
+                if (bound > kCharsBound) cap = fetch(ctx_, mb.c[j].tile_sum + tiles);
+                mb.c[j].chars = result<char>(cap ? cap : 1);
+                res->cols[b0 + j].chars = mb.c[j].chars;
+                res->cols[b0 + j].num_chars = cap;
+            }
+            if (any_str) { Prof p_(ctx_, "mat_strings"); mat_strings(mb, s_); }
         }
         res->live = true;
         return res;
@@ -653,7 +686,7 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
     if (hc.num_pages && !hc.pages) throw Error(HJ_E_INVALID, "column pages missing");
     dc.h_row_start.assign((size_t)hc.num_pages + 1, 0);
     std::vector<uint32_t> group_start(hc.num_pages ? hc.num_pages : 1, 0);
-    uint64_t rows = 0, groups = 0;
+    uint64_t rows = 0, groups = 0, long_len = 0;
     uint32_t first_rows = 0;
     bool uniform = true;
     dc.all_dense = 1;
@@ -669,6 +702,8 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
             if (nn > HJ_PAGE_SIZE - 4) throw Error(HJ_E_INVALID, "corrupt long-string page");
             prow = nr == 0xffffu ? 1 : 0;
             dc.num_chars += nn;
+            long_len = (nr == 0xffffu ? 0 : long_len) + nn;
+            dc.max_len = (uint32_t)std::max<uint64_t>(dc.max_len, long_len);
             dc.all_dense = 0;  // long-string pages have no bitmap but break uniformity
             uniform = false;
         } else if (hc.type == HJ_INT32) {
@@ -821,9 +856,9 @@ void to_host(hj_context* ctx, const hj_dresult* r, hj_result* out) {
         } else {
             std::vector<uint64_t> offs(n + 1);
             HJ_HIP(hipMemcpy(offs.data(), col.offsets, (n + 1) * 8, hipMemcpyDeviceToHost));
-            std::vector<char> chars(col.num_chars ? col.num_chars : 1);
-            if (col.num_chars)
-                HJ_HIP(hipMemcpy(chars.data(), col.chars, col.num_chars, hipMemcpyDeviceToHost));
+            const uint64_t nc = offs[n];
+            std::vector<char> chars(nc ? nc : 1);
+            if (nc) HJ_HIP(hipMemcpy(chars.data(), col.chars, nc, hipMemcpyDeviceToHost));
             encode_varchar(offs.data(), chars.data(), vf, n, &out->columns[c]);
         }
     }

@@ -23,7 +23,6 @@ namespace {
 
 constexpr int kB = 256;
 constexpr uint32_t kNW = kB / HJ_WAVE;
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));  // one 16-byte load or store
 
 __device__ inline uint32_t log2ceil(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
@@ -80,30 +79,30 @@ __device__ inline uint32_t ballot_prefix(bool pred, uint32_t* lds, uint32_t* tot
 }
 
 // ---------------------------------------------------------------- tables
-// home slot of a key in a table of 2^k entries (k >= kMinTableBits)
-__device__ inline uint32_t home_slot(uint32_t key, uint32_t k) { return fib32(key) >> (32u - k); }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte bucket
 
-// build row of `key`, kDead when absent, given the two entries e0, e1 at the even slot at or
-// below its home slot s; a run past them (rare at load factor <= 1/2) continues entry by entry
-__device__ inline uint32_t tab_resolve(const HJ_GLOBAL uint64_t* T, uint32_t mask, uint32_t s, uint32_t key,
-                                       uint64_t e0, uint64_t e1) {
-    if (!(s & 1u)) {
-        if (e0 == kEmpty) return kDead;
-        if ((uint32_t)e0 == key) return (uint32_t)(e0 >> 32);
-    }
-    if (e1 == kEmpty) return kDead;
-    if ((uint32_t)e1 == key) return (uint32_t)(e1 >> 32);
-    for (uint32_t q = ((s | 1u) + 1u) & mask;; q = (q + 1u) & mask) {
-        const uint64_t e = T[q];
-        if (e == kEmpty) return kDead;
-        if ((uint32_t)e == key) return (uint32_t)(e >> 32);
-    }
+// home bucket of a key in a table of 2^k key slots (k >= kMinTableBits): 2^(k-2) buckets
+__device__ inline uint32_t home_bucket(uint32_t key, uint32_t k) { return fib32(key) >> (34u - k); }
+
+__device__ inline u32x4 load_bucket(const HJ_GLOBAL uint32_t* K, uint32_t b) {
+    return *(const HJ_GLOBAL u32x4*)(K + 4u * b);
 }
 
-__device__ inline void tab_pair(const HJ_GLOBAL uint64_t* T, uint32_t s, uint64_t* e0, uint64_t* e1) {
-    const u64x2 v = *(const HJ_GLOBAL u64x2*)(T + (s & ~1u));
-    *e0 = v.x;
-    *e1 = v.y;
+// slot of `key` (!= kEmptyKey), kDead when absent, given the keys v of its home bucket b;
+// occupied slots form a prefix of the probe sequence, so an empty slot ends the search, and at
+// load factor <= 1/8 the home bucket nearly always does
+__device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t bmask, uint32_t b,
+                                          uint32_t key, u32x4 v) {
+    for (;;) {
+        if (v.x == key) return 4u * b;
+        if (v.y == key) return 4u * b + 1u;
+        if (v.z == key) return 4u * b + 2u;
+        if (v.w == key) return 4u * b + 3u;
+        if (v.x == kEmptyKey || v.y == kEmptyKey || v.z == kEmptyKey || v.w == kEmptyKey)
+            return kDead;
+        b = (b + 1u) & bmask;
+        v = load_bucket(K, b);
+    }
 }
 
 // ---------------------------------------------------------------- build
@@ -120,6 +119,26 @@ __device__ inline void block_append(uint32_t* lds_cnt, const bool (&pred)[U], ui
         tot += (uint32_t)__popcll(b[u]);
     }
     if (!tot) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(lds_cnt, tot);
+    base = __shfl(base, 0, HJ_WAVE);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        pos[u] = base + (uint32_t)__popcll(b[u] & lanemask_lt());
+        base += (uint32_t)__popcll(b[u]);
+    }
+}
+
+// block_append over a bit mask of U predicates (bit u = item u)
+template <int U>
+__device__ inline void block_append_mask(uint32_t* lds_cnt, uint32_t mask, uint32_t (&pos)[U]) {
+    uint64_t b[U];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        b[u] = __ballot((mask >> u) & 1u);
+        tot += (uint32_t)__popcll(b[u]);
+    }
     uint32_t base = 0;
     if (lane_id() == 0) base = atomicAdd(lds_cnt, tot);
     base = __shfl(base, 0, HJ_WAVE);
@@ -183,11 +202,11 @@ __global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ 
     const BuildJob& jb = bb->job[blockIdx.y];
     const uint32_t n = job_tuples(jb, gridDim.x, lds);
     if (blockIdx.x == 0 && threadIdx.x == 0) gp(jb.meta)[0] = n;
-    const uint32_t half = (1u << table_bits(n)) / 2u;
+    const uint32_t nbk = (1u << table_bits(n)) / kBucketKeys;
     const uint32_t g = blockIdx.x * kB + threadIdx.x, stride = gridDim.x * kB;
-    HJ_GLOBAL u64x2* T = (HJ_GLOBAL u64x2*)gp(jb.table);
-    const u64x2 empty = {kEmpty, kEmpty};
-    for (uint32_t i = g; i < half; i += stride) T[i] = empty;
+    HJ_GLOBAL u32x4* T = (HJ_GLOBAL u32x4*)gp(jb.tkey);
+    const u32x4 empty = {kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
+    for (uint32_t i = g; i < nbk; i += stride) T[i] = empty;
     if (bf_enabled(n)) {
         const uint32_t W = 1u << bf_log2w(n);
         for (uint32_t i = g; i < W; i += stride) gp(jb.bf)[i] = 0;
@@ -198,21 +217,34 @@ __global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ 
 __global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__ bb) {
     const BuildJob& jb = bb->job[blockIdx.y];
     const uint32_t n = gp(jb.meta)[0];
-    const uint32_t k = table_bits(n), mask = (1u << k) - 1u;
+    const uint32_t k = table_bits(n), bmask = (1u << (k - 2)) - 1u;
     const bool use_bf = bf_enabled(n);
     const uint32_t lw = bf_log2w(n);
     const uint32_t r0 = blockIdx.x * build_range(jb.rows, gridDim.x);
     const uint32_t cnt = gp(jb.blk)[blockIdx.x];
+    HJ_GLOBAL uint32_t* K = gp(jb.tkey);
+    HJ_GLOBAL uint32_t* R = gp(jb.trow);
     for (uint32_t i = threadIdx.x; i < cnt; i += kB) {
         const uint32_t key = gp(jb.st_key)[r0 + i], row = gp(jb.st_row)[r0 + i];
-        const unsigned long long e = ((unsigned long long)row << 32) | key;
-        for (uint32_t s = home_slot(key, k);; s = (s + 1u) & mask) {
-            const unsigned long long old =
-                atomicCAS((unsigned long long*)&gp(jb.table)[s], (unsigned long long)kEmpty, e);
-            if (old == kEmpty) break;
-            if ((uint32_t)old == key) {
+        if (key == kEmptyKey) {  // kept aside in slot 2^k
+            if (atomicCAS((uint32_t*)&gp(jb.meta)[2], 0u, 1u) != 0u)
                 gp(jb.meta)[1] = 1;
-                break;
+            else
+                R[1u << k] = row;
+        } else {
+            bool done = false;
+            for (uint32_t b = home_bucket(key, k); !done; b = (b + 1u) & bmask) {
+                for (uint32_t j = 0; j < kBucketKeys && !done; ++j) {
+                    const uint32_t s = 4u * b + j;
+                    const uint32_t old = atomicCAS((uint32_t*)&K[s], kEmptyKey, key);
+                    if (old == kEmptyKey) {
+                        R[s] = row;
+                        done = true;
+                    } else if (old == key) {
+                        gp(jb.meta)[1] = 1;
+                        done = true;
+                    }
+                }
             }
         }
         if (use_bf) {
@@ -253,93 +285,123 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     }
     const HJ_GLOBAL int32_t* kf[NS];
     const HJ_GLOBAL uint64_t* kv[NS];
-    const HJ_GLOBAL uint64_t* tab[NS];
-    uint32_t tk[NS];
+    const HJ_GLOBAL uint32_t* tkey[NS];
+    const HJ_GLOBAL uint32_t* trow[NS];
+    uint32_t tk[NS], tsp[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
         const PipeStage& ps = pd.st[pd.order[q]];
         kf[q] = gp(ps.keycol.flat);
         kv[q] = gp(ps.keycol.vbits);
         const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
-        tab[q] = gp(jp->table);
+        tkey[q] = gp(jp->tkey);
+        trow[q] = gp(jp->trow);
         tk[q] = table_bits(gp(jp->meta)[0]);
+        tsp[q] = gp(jp->meta)[2];
     }
     const uint32_t nsrc = pd.src_rows;
     const uint32_t G = (nsrc + 63u) >> 6;
     const int nf = pd.nf;
     // this block's groups [g_lo, g_hi); survivors are staged from row g_lo * 64 of the stage
-    // arrays on
+    // arrays on. The loop body avoids per-lane branches (each costs scalar exec-mask work on the
+    // CU's one scalar unit): liveness is a bit mask, key loads are clamped in bounds, and dead
+    // lanes read bucket 0, which the wave's load coalesces into one request.
     const uint32_t g_lo = blockIdx.x * pd.probe_gpb;
     const uint32_t g_hi = g_lo + pd.probe_gpb < G ? g_lo + pd.probe_gpb : G;
     const uint32_t st0 = g_lo * 64u;
+    const uint32_t rmax = nsrc - 1u;
     for (uint32_t gb = g_lo / U + w; gb * U < g_hi; gb += kNW) {
-        uint32_t r[U];
-        bool alive[U];
+        uint32_t r[U], am = 0;
         int32_t key[NS][U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             r[u] = (gb * U + u) * 64u + lane;
-            alive[u] = r[u] < nsrc && gb * U + u < g_hi;
+            am |= (r[u] < nsrc && gb * U + u < g_hi) ? 1u << u : 0u;
         }
         // keys of every streamed join first: NS * U coalesced loads in flight
 #pragma unroll
         for (int q = 0; q < NS; ++q)
 #pragma unroll
-            for (int u = 0; u < U; ++u) key[q][u] = alive[u] ? kf[q][r[u]] : 0;
+            for (int u = 0; u < U; ++u) key[q][u] = kf[q][r[u] < rmax ? r[u] : rmax];
         for (int f = 0; f < nf; ++f) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 int32_t v = 0;
-                alive[u] = alive[u] && get_int32(pd.fcol[f], r[u], &v) && v >= pd.flo[f] &&
-                           v <= pd.fhi[f];
+                const bool ok = ((am >> u) & 1u) && get_int32(pd.fcol[f], r[u], &v) &&
+                                v >= pd.flo[f] && v <= pd.fhi[f];
+                am &= ok ? ~0u : ~(1u << u);
             }
         }
-        uint32_t brow[NS][U];
+        uint32_t brow[NS][U];  // key slot of the match, until the output
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int u = 0; u < U; ++u) brow[q][u] = kDead;
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
+            if (kv[q]) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                brow[q][u] = kDead;
-                const uint32_t g = gb * U + u;
-                if (kv[q] && g < g_hi) alive[u] = alive[u] && ((kv[q][g] >> lane) & 1ull);
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t g = gb * U + u < g_hi ? gb * U + u : g_hi - 1u;
+                    am &= ((kv[q][g] >> lane) & 1ull) ? ~0u : ~(1u << u);
+                }
             }
             if (q == 0 && use_bf) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t h = fib32((uint32_t)key[0][u]);
                     const uint32_t m = bf_mask(h);
-                    alive[u] = alive[u] && (bfl[bf_word(h, lw)] & m) == m;
+                    am &= (bfl[bf_word(h, lw)] & m) == m ? ~0u : ~(1u << u);
                 }
             }
-            bool any = false;
-#pragma unroll
-            for (int u = 0; u < U; ++u) any = any || alive[u];
-            if (!__ballot(any)) break;
-            uint64_t e0[U], e1[U];
-            uint32_t hs[U];
+            if (!__ballot(am != 0)) continue;  // uniform: every later join is skipped too
+            const uint32_t kq = tk[q];
+            u32x4 v[U];
+            uint32_t hb[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                hs[u] = home_slot((uint32_t)key[q][u], tk[q]);
-                e0[u] = e1[u] = kEmpty;
-                if (alive[u]) tab_pair(tab[q], hs[u], &e0[u], &e1[u]);
+                hb[u] = ((am >> u) & 1u) ? home_bucket((uint32_t)key[q][u], kq) : 0u;
+                v[u] = load_bucket(tkey[q], hb[u]);
             }
-            const uint32_t mask = (1u << tk[q]) - 1u;
+            // home bucket: match, or an empty slot (absent), or full (continue: kDead - 1)
+            uint32_t cont = 0;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                if (alive[u]) {
-                    brow[q][u] = tab_resolve(tab[q], mask, hs[u], (uint32_t)key[q][u], e0[u], e1[u]);
-                    alive[u] = brow[q][u] != kDead;
-                }
+                const uint32_t kk = (uint32_t)key[q][u];
+                const uint32_t b4 = 4u * hb[u];
+                uint32_t sl = v[u].w == kk ? b4 + 3u : kDead;
+                sl = v[u].z == kk ? b4 + 2u : sl;
+                sl = v[u].y == kk ? b4 + 1u : sl;
+                sl = v[u].x == kk ? b4 : sl;
+                const bool full = v[u].x != kEmptyKey && v[u].y != kEmptyKey &&
+                                  v[u].z != kEmptyKey && v[u].w != kEmptyKey;
+                sl = (sl == kDead && full) ? kDead - 1u : sl;
+                sl = kk == kEmptyKey ? (tsp[q] ? (1u << kq) : kDead) : sl;
+                sl = ((am >> u) & 1u) ? sl : kDead;
+                cont |= sl == kDead - 1u ? 1u << u : 0u;
+                brow[q][u] = sl;
             }
+            if (__ballot(cont != 0)) {  // a full home bucket: rare at load factor <= 1/8
+                const uint32_t bmask = (1u << (kq - 2)) - 1u;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if ((cont >> u) & 1u)
+                        brow[q][u] = bucket_resolve(tkey[q], bmask, (hb[u] + 1u) & bmask,
+                                                    (uint32_t)key[q][u],
+                                                    load_bucket(tkey[q], (hb[u] + 1u) & bmask));
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) am &= brow[q][u] != kDead ? ~0u : ~(1u << u);
         }
+        if (!__ballot(am != 0)) continue;
         uint32_t pos[U];
-        block_append<U>(&s_cnt, alive, pos);
+        block_append_mask<U>(&s_cnt, am, pos);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (!alive[u]) continue;
+            if (!((am >> u) & 1u)) continue;
             gp(pd.stage[0])[st0 + pos[u]] = r[u];
 #pragma unroll
-            for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[u]] = brow[q][u];
+            for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[u]] = trow[q][brow[q][u]];
         }
     }
     __syncthreads();
@@ -377,8 +439,10 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
     const uint32_t n = gp(pd.meta)[2];
     const PipeStage& ps = pd.st[s];
     const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
-    const HJ_GLOBAL uint64_t* table = gp(jp->table);
-    const uint32_t k = table_bits(gp(jp->meta)[0]), mask = (1u << k) - 1u;
+    const HJ_GLOBAL uint32_t* tkey = gp(jp->tkey);
+    const HJ_GLOBAL uint32_t* trow = gp(jp->trow);
+    const uint32_t k = table_bits(gp(jp->meta)[0]), bmask = (1u << (k - 2)) - 1u;
+    const uint32_t special = gp(jp->meta)[2] ? (1u << k) : kDead;
     const HJ_GLOBAL uint32_t* prev = previous < 0 ? nullptr : gp(pd.l_b[previous]);
     const HJ_GLOBAL uint32_t* src =
         ps.key_slot == 0 ? gp(pd.l_row) : gp(pd.l_b[ps.key_slot - 1]);
@@ -386,10 +450,10 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
     constexpr int U = 4;
     const uint32_t stride = gridDim.x * kB * U;
     for (uint32_t base = blockIdx.x * kB * U; base < n; base += stride) {
-        uint32_t jx[U], r[U], hs[U];
+        uint32_t jx[U], r[U], hb[U];
         int32_t key[U];
         bool ok[U];
-        uint64_t e0[U], e1[U];
+        u32x4 v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             jx[u] = base + u * kB + threadIdx.x;
@@ -404,15 +468,19 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            hs[u] = home_slot((uint32_t)key[u], k);
-            e0[u] = e1[u] = kEmpty;
-            if (ok[u]) tab_pair(table, hs[u], &e0[u], &e1[u]);
+            hb[u] = home_bucket((uint32_t)key[u], k);
+            v[u] = u32x4{kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
+            if (ok[u]) v[u] = load_bucket(tkey, hb[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (jx[u] >= n) continue;
-            outb[jx[u]] = ok[u] ? tab_resolve(table, mask, hs[u], (uint32_t)key[u], e0[u], e1[u])
-                                : kDead;
+            uint32_t slot = kDead;
+            if (ok[u]) {
+                const uint32_t kk = (uint32_t)key[u];
+                slot = kk == kEmptyKey ? special : bucket_resolve(tkey, bmask, hb[u], kk, v[u]);
+            }
+            outb[jx[u]] = slot == kDead ? kDead : trow[slot];
         }
     }
 }
@@ -491,7 +559,33 @@ void launch_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
     hipLaunchKernelGGL((k_sp_probe<NS, U>), dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
 }
 
+template <int NS>
+uint32_t resident_blocks(uint32_t num_cus) {
+    static uint32_t per_cu = 0;
+    if (!per_cu) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sp_probe<NS, probe_unroll(NS)>, kB,
+                                                         0) != hipSuccess || n < 1)
+            n = 1;
+        per_cu = (uint32_t)n;
+    }
+    return per_cu * num_cus;
+}
+
 }  // namespace
+
+uint32_t probe_resident_blocks(int ns, uint32_t num_cus) {
+    switch (ns) {
+        case 1: return resident_blocks<1>(num_cus);
+        case 2: return resident_blocks<2>(num_cus);
+        case 3: return resident_blocks<3>(num_cus);
+        case 4: return resident_blocks<4>(num_cus);
+        case 5: return resident_blocks<5>(num_cus);
+        case 6: return resident_blocks<6>(num_cus);
+        case 7: return resident_blocks<7>(num_cus);
+        default: return resident_blocks<8>(num_cus);
+    }
+}
 
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s) {
     if (hb.njobs == 0) return;

@@ -13,20 +13,24 @@
 namespace hj {
 
 constexpr int kMaxStages = 8;
-constexpr uint32_t kDead = 0xffffffffu;  // no match / a survivor that failed a later join
-constexpr uint64_t kEmpty = ~0ull;       // a free table entry
-constexpr uint32_t kBloomWords = 4096;   // 16 KiB: the LDS-resident filter of a small build side
+constexpr uint32_t kDead = 0xffffffffu;      // no match / a survivor that failed a later join
+constexpr uint32_t kEmptyKey = 0x80000000u;  // a free key slot (INT32_MIN itself is kept aside)
+constexpr uint32_t kBloomWords = 4096;       // 16 KiB: the LDS-resident filter of a small build side
+constexpr uint32_t kBucketKeys = 4;          // one 16-byte load reads a whole bucket
 constexpr uint32_t kMinTableBits = 6;
 
-// table bits for n build tuples: 2^bits >= 2n entries (load factor <= 1/2)
+// key slots for n build tuples: 2^bits >= 8n (load factor <= 1/8, about half a key per
+// 4-key bucket, so a lookup nearly always ends in its home bucket)
 __host__ __device__ inline uint32_t table_bits(uint32_t n) {
     uint32_t k = 0;
     while ((1ull << k) < (uint64_t)n) ++k;
-    return k + 1 < kMinTableBits ? kMinTableBits : k + 1;
+    return k + 3 < kMinTableBits ? kMinTableBits : k + 3;
 }
 
-// One build side: the rows of a (filtered) base scan with a non-NULL key, inserted into an
-// open-addressing table (linear probing) of entries row << 32 | key.
+// One build side: the rows of a (filtered) base scan with a non-NULL key, in a bucketized
+// open-addressing table: tkey holds the keys (buckets of kBucketKeys consecutive slots, linear
+// probing over buckets), trow the build row of each occupied slot. A key equal to kEmptyKey is
+// kept in slot 2^bits of trow, flagged by meta[2].
 struct BuildJob {
     DCol key;
     DCol fcol[kMaxFilters];
@@ -34,12 +38,13 @@ struct BuildJob {
     int32_t fhi[kMaxFilters];
     int32_t nf;
     uint32_t rows;      // table rows (bound of the tuple count)
-    uint32_t kmax;      // the table holds 2^kmax entries
-    uint32_t* meta;     // [0] tuples, [1] a key occurs twice, [2..3] unused
+    uint32_t kmax;      // the table holds 2^kmax key slots
+    uint32_t* meta;     // [0] tuples, [1] a key occurs twice, [2] kEmptyKey present, [3] unused
     uint32_t* st_key;   // staged tuples, each block of the filter pass at the start of its
     uint32_t* st_row;   // own row range (capacity: rows rounded up to whole ranges)
     uint32_t* blk;      // tuples staged per block of the filter pass
-    uint64_t* table;    // 2^table_bits(tuples) entries used
+    uint32_t* tkey;     // 2^table_bits(tuples) key slots used
+    uint32_t* trow;     // 2^kmax + 1 build rows
     uint32_t* bf;       // blocked Bloom filter, kBloomWords words (small build sides only)
 };
 
@@ -95,6 +100,11 @@ __host__ __device__ inline uint32_t build_range(uint32_t rows, uint32_t blocks) 
     const uint32_t tiles = (rows + kBuildTile - 1) / kBuildTile;
     return ((tiles + blocks - 1) / blocks) * kBuildTile;
 }
+
+// blocks of the fused pass over ns streamed joins that are resident at once on the device (the
+// pass gives each a contiguous share of the source, so a second round of blocks would double
+// its time)
+uint32_t probe_resident_blocks(int ns, uint32_t num_cus);
 
 // build: filter + stage, clear tables, insert
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s);

@@ -143,6 +143,29 @@ def test_resident_int32_mirror(gpu_ctx, oracle):
         dt.free()
 
 
+def test_extreme_keys(gpu_ctx, oracle, mode):
+    """INT32_MIN (the pipeline table's free-slot marker, kept aside), INT32_MAX, 0 and -1 as
+    unique build keys and as probe keys; then INT32_MIN twice in the build (duplicate key)"""
+    rng = np.random.default_rng(5)
+    ext = np.array([-2**31, 2**31 - 1, 0, -1], np.int32)
+    bk = np.concatenate([ext, rng.choice(np.arange(1, 10**6), 5000, replace=False)]).astype(np.int32)
+    bk = bk[rng.permutation(len(bk))]
+    bp = rng.integers(0, 2**31, len(bk)).astype(np.int32)
+    bv = np.ones(len(bk), np.uint8)
+    pk = np.concatenate([np.repeat(ext, 3), rng.choice(bk, 20000)]).astype(np.int32)
+    pp = rng.integers(0, 2**31, len(pk)).astype(np.int32)
+    pv = (rng.random(len(pk)) > 0.1).astype(np.uint8)
+    plan = two_way(oracle.codec, bk, bp, bv, pk, pp, pv)
+    assert np.array_equal(pairs_of(oracle.codec, gpu_ctx.execute(plan)),
+                          expected_pairs(bk, bp, bv, pk, pp, pv))
+    bk2 = np.concatenate([bk, [-2**31]]).astype(np.int32)
+    bp2 = np.concatenate([bp, [12345]]).astype(np.int32)
+    bv2 = np.ones(len(bk2), np.uint8)
+    plan = two_way(oracle.codec, bk2, bp2, bv2, pk, pp, pv)
+    assert np.array_equal(pairs_of(oracle.codec, gpu_ctx.execute(plan)),
+                          expected_pairs(bk2, bp2, bv2, pk, pp, pv))
+
+
 def test_two_way_unique_build_1m(gpu_ctx, oracle, mode):
     """the shape of the 2-way config at 1/10 size: unique shuffled build keys, 5 % NULL FKs"""
     rng = np.random.default_rng(42)
