@@ -96,9 +96,9 @@ def kernel_bytes(name, info, launches):
     """algorithmic HBM bytes of all launches of one step of kernel `name` (None: no model)"""
     fk1 = info["fk_bytes"][0]
     outs, probes = info["outs"], info["probes"]
-    if name == "pipe_probe":
-        # every probe-side key column read once in full (packed values + validity bits), plus
-        # the surviving rows' ids written once per output slot
+    if name == "probe_fused":
+        # the fused probe pass (one launch): every probe-side key column read once in full
+        # (packed values + validity bits), plus the surviving rows' ids written once per slot
         return sum(info["fk_bytes"]) + info["out_rows"] * info["out_slots"] * 4
     if name == "probe_scan_count":
         return fk1

@@ -550,7 +550,8 @@ class Executor {
         const BuildBatch* dbb = (const BuildBatch*)ddesc;
         const PipeDesc* dpd = (const PipeDesc*)(ddesc + pd_off);
         { Prof p_(ctx_, "pipe_build"); pipe_build(hb, dbb, s_); }
-        { Prof p_(ctx_, "pipe_probe"); pipe_probe(hp, dpd, s_); }
+        { Prof p_(ctx_, "probe_fused"); pipe_probe_fused(hp, dpd, s_); }
+        { Prof p_(ctx_, "probe_tail"); pipe_probe_tail(hp, dpd, s_); }
         struct F4 {
             uint32_t v[4];
         };

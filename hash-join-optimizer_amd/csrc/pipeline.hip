@@ -595,7 +595,7 @@ void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s) {
     hipLaunchKernelGGL(k_sb_insert, grid, dim3(kB), 0, s, db);
 }
 
-void pipe_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
+void pipe_probe_fused(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
     switch (hp.nstream) {
         case 1: launch_probe<1>(hp, dp, s); break;
         case 2: launch_probe<2>(hp, dp, s); break;
@@ -607,6 +607,10 @@ void pipe_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
         case 8: launch_probe<8>(hp, dp, s); break;
         default: return;
     }
+}
+
+void pipe_probe_tail(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
+    if (hp.nstream < 1 || hp.nstream > kMaxStages) return;
     hipLaunchKernelGGL(k_scan1_u32, dim3(1), dim3(1024), 0, s, hp.blk_cnt, hp.blk_off,
                        hp.probe_blocks, (const uint32_t*)nullptr, hp.meta + (hp.ndep ? 2 : 3));
     hipLaunchKernelGGL(k_sp_compact, dim3(hp.probe_blocks), dim3(kB), 0, s, dp);

@@ -108,7 +108,9 @@ uint32_t probe_resident_blocks(int ns, uint32_t num_cus);
 
 // build: filter + stage, clear tables, insert
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s);
-// fused pass over the source, then the joins keyed by build tables and the final compaction
-void pipe_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s);
+// the fused pass over the source (one launch) ...
+void pipe_probe_fused(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s);
+// ... then the survivor compaction, the joins keyed by build tables and the final compaction
+void pipe_probe_tail(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s);
 
 }  // namespace hj
