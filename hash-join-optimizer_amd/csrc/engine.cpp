@@ -36,6 +36,9 @@ namespace hj {
 // string bytes of an output column are allocated from rows x longest string (no host round trip)
 // up to this size; beyond it they are counted first
 constexpr uint64_t kCharsBound = 1ull << 30;
+// blocks per CU and build job of the pipeline's build kernels (block x of a job owns one range
+// of its rows, so the filter pass is a few iterations per block)
+constexpr uint32_t kBuildBlocksPerCU = 8;
 
 namespace {
 
@@ -441,7 +444,7 @@ class Executor {
             jb.rows = rows;
             jb.kmax = table_bits(rows);
             jb.meta = metas + 4 * (i + 1);
-            const uint64_t bgrid = (uint64_t)ctx_->num_cus * 2;
+            const uint64_t bgrid = (uint64_t)ctx_->num_cus * kBuildBlocksPerCU;
             const uint64_t scap = std::max<uint64_t>(1, build_range(rows, (uint32_t)bgrid) * bgrid);
             jb.st_key = scratch<uint32_t>(scap);
             jb.st_row = scratch<uint32_t>(scap);
@@ -469,7 +472,7 @@ class Executor {
             }
             cur.swap(nxt);
         }
-        hb.grid = (uint32_t)ctx_->num_cus * 2;
+        hb.grid = (uint32_t)ctx_->num_cus * kBuildBlocksPerCU;
 
         hp.nf = (int32_t)srel.filters.size();
         for (size_t f = 0; f < srel.filters.size(); ++f) {

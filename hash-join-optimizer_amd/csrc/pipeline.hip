@@ -190,18 +190,19 @@ __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__
     if (threadIdx.x == 0) gp(jb.blk)[blockIdx.x] = s_cnt;
 }
 
-// tuple count of a job = sum of the filter pass's block counts
-__device__ inline uint32_t job_tuples(const BuildJob& jb, uint32_t blocks, uint32_t* lds) {
+// tuple count of each job = sum of its filter blocks' counts (one block per job)
+__global__ __launch_bounds__(1024) void k_sb_count(const BuildBatch* __restrict__ bb) {
+    __shared__ uint32_t lds[1024 / HJ_WAVE];
+    const BuildJob& jb = bb->job[blockIdx.x];
     uint32_t c = 0;
-    for (uint32_t i = threadIdx.x; i < blocks; i += kB) c += gp(jb.blk)[i];
-    return bsum_u32<kNW>(c, lds);
+    for (uint32_t i = threadIdx.x; i < bb->grid; i += 1024) c += gp(jb.blk)[i];
+    const uint32_t n = block_sum<1024>(c, lds);
+    if (threadIdx.x == 0) gp(jb.meta)[0] = n;
 }
 
 __global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ bb) {
-    __shared__ uint32_t lds[kNW];
     const BuildJob& jb = bb->job[blockIdx.y];
-    const uint32_t n = job_tuples(jb, gridDim.x, lds);
-    if (blockIdx.x == 0 && threadIdx.x == 0) gp(jb.meta)[0] = n;
+    const uint32_t n = gp(jb.meta)[0];
     const uint32_t nbk = (1u << table_bits(n)) / kBucketKeys;
     const uint32_t g = blockIdx.x * kB + threadIdx.x, stride = gridDim.x * kB;
     HJ_GLOBAL u32x4* T = (HJ_GLOBAL u32x4*)gp(jb.tkey);
@@ -255,10 +256,110 @@ __global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__
 }
 
 // ---------------------------------------------------------------- fused probe
+// drop the lanes whose key in this join is NULL (vb: the key column's validity words, uniform
+// per 64-row group; nullptr when the column has no NULL)
+template <int U>
+__device__ inline uint32_t valid_mask(uint32_t am, const HJ_GLOBAL uint64_t* vb, uint32_t gb,
+                                      uint32_t g_hi, uint32_t lane) {
+    if (!vb) return am;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t g = gb * U + u < g_hi ? gb * U + u : g_hi - 1u;
+        am &= ((vb[g] >> lane) & 1ull) ? ~0u : ~(1u << u);
+    }
+    return am;
+}
+
+// key slot of kk from its home bucket's keys v: the slot, kDead (absent, or a dead lane), or
+// kDead - 1 (the home bucket is full: continue in the next buckets). The key kEmptyKey lives
+// in slot 2^k when the build holds it (sp).
+__device__ inline uint32_t resolve_home(u32x4 v, uint32_t kk, uint32_t hb, uint32_t k,
+                                        uint32_t sp, uint32_t alive) {
+    const uint32_t b4 = 4u * hb;
+    uint32_t sl = v.w == kk ? b4 + 3u : kDead;
+    sl = v.z == kk ? b4 + 2u : sl;
+    sl = v.y == kk ? b4 + 1u : sl;
+    sl = v.x == kk ? b4 : sl;
+    const bool full = v.x != kEmptyKey && v.y != kEmptyKey && v.z != kEmptyKey && v.w != kEmptyKey;
+    sl = (sl == kDead && full) ? kDead - 1u : sl;
+    sl = kk == kEmptyKey ? (sp ? (1u << k) : kDead) : sl;
+    return alive ? sl : kDead;
+}
+
+// lanes whose home bucket was full continue through the following buckets (rare at load
+// factor <= 1/8; one wave-uniform test when none does)
+template <int U>
+__device__ inline void resolve_rest(uint32_t (&slot)[U], uint32_t cont,
+                                    const HJ_GLOBAL uint32_t* tkey, uint32_t k,
+                                    const uint32_t (&hb)[U], const int32_t (&key)[U]) {
+    if (!__ballot(cont != 0)) return;
+    const uint32_t bmask = (1u << (k - 2)) - 1u;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if ((cont >> u) & 1u)
+            slot[u] = bucket_resolve(tkey, bmask, (hb[u] + 1u) & bmask, (uint32_t)key[u],
+                                     load_bucket(tkey, (hb[u] + 1u) & bmask));
+}
+
 // NS streamed joins, U 64-row groups per wave and iteration
+constexpr uint32_t kQueue = 128;  // per-wave queue of rows that passed the LDS filter
+
+// look up every join for the first n (<= 64) entries of a wave's queue, one entry per lane, and
+// stage the rows that match all of them (row, then the build row of each join in probe order)
+template <int NS>
+__device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, uint32_t lane,
+                                   const HJ_GLOBAL uint32_t* const (&tkey)[NS],
+                                   const HJ_GLOBAL uint32_t* const (&trow)[NS],
+                                   const uint32_t (&tk)[NS], const uint32_t (&tsp)[NS],
+                                   uint32_t* s_cnt, const PipeDesc& pd, uint32_t st0) {
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t live = lane < n ? 1u : 0u;
+    const uint32_t row = qb[0][lane];
+    int32_t key[NS][1];
+    uint32_t hb[NS][1], slot[NS][1];
+    u32x4 v[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        key[q][0] = (int32_t)qb[1 + q][lane];
+        hb[q][0] = live ? home_bucket((uint32_t)key[q][0], tk[q]) : 0u;
+        v[q] = load_bucket(tkey[q], hb[q][0]);
+    }
+    uint32_t alive = live;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        slot[q][0] = resolve_home(v[q], (uint32_t)key[q][0], hb[q][0], tk[q], tsp[q], live);
+        resolve_rest<1>(slot[q], slot[q][0] == kDead - 1u ? 1u : 0u, tkey[q], tk[q], hb[q],
+                        key[q]);
+        alive &= slot[q][0] != kDead ? 1u : 0u;
+    }
+    if (!__ballot(alive)) return;
+    uint32_t pos[1];
+    block_append_mask<1>(s_cnt, alive, pos);
+    if (alive) {
+        gp(pd.stage[0])[st0 + pos[0]] = row;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[0]] = trow[q][slot[q][0]];
+    }
+}
+
+// move the queue's entries [64, 64 + rem) to the front
+template <int NS>
+__device__ inline void queue_shift(uint32_t (&qb)[NS + 1][kQueue], uint32_t rem, uint32_t lane) {
+    __builtin_amdgcn_wave_barrier();
+    uint32_t e[NS + 1];
+#pragma unroll
+    for (int q = 0; q <= NS; ++q) e[q] = qb[q][HJ_WAVE + lane];
+    __builtin_amdgcn_wave_barrier();
+    if (lane < rem)
+#pragma unroll
+        for (int q = 0; q <= NS; ++q) qb[q][lane] = e[q];
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int NS, int U>
 __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pdp) {
     __shared__ uint32_t bfl[kBloomWords];
+    __shared__ uint32_t qbuf[kNW][NS + 1][kQueue];  // [wave][row, key of each join][entry]
     __shared__ uint32_t s_bad, s_cnt;
     const PipeDesc& pd = *pdp;
     const uint32_t t = threadIdx.x, lane = lane_id();
@@ -310,6 +411,7 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     const uint32_t g_hi = g_lo + pd.probe_gpb < G ? g_lo + pd.probe_gpb : G;
     const uint32_t st0 = g_lo * 64u;
     const uint32_t rmax = nsrc - 1u;
+    uint32_t qc = 0;  // entries in this wave's queue (uniform)
     for (uint32_t gb = g_lo / U + w; gb * U < g_hi; gb += kNW) {
         uint32_t r[U], am = 0;
         int32_t key[NS][U];
@@ -337,61 +439,60 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
         for (int q = 0; q < NS; ++q)
 #pragma unroll
             for (int u = 0; u < U; ++u) brow[q][u] = kDead;
+        if (use_bf) {
+            // a strong LDS filter on the first join: rows that pass it and every validity test
+            // go to this wave's queue; full 64-lane batches of the queue then look up all joins
+            // at once (one 16-byte bucket load per join and live lane)
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
-            if (kv[q]) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t g = gb * U + u < g_hi ? gb * U + u : g_hi - 1u;
-                    am &= ((kv[q][g] >> lane) & 1ull) ? ~0u : ~(1u << u);
-                }
-            }
-            if (q == 0 && use_bf) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t h = fib32((uint32_t)key[0][u]);
-                    const uint32_t m = bf_mask(h);
-                    am &= (bfl[bf_word(h, lw)] & m) == m ? ~0u : ~(1u << u);
-                }
-            }
-            if (!__ballot(am != 0)) continue;  // uniform: every later join is skipped too
-            const uint32_t kq = tk[q];
-            u32x4 v[U];
-            uint32_t hb[U];
+            for (int q = 0; q < NS; ++q) am = valid_mask<U>(am, kv[q], gb, g_hi, lane);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                hb[u] = ((am >> u) & 1u) ? home_bucket((uint32_t)key[q][u], kq) : 0u;
-                v[u] = load_bucket(tkey[q], hb[u]);
+                const uint32_t h = fib32((uint32_t)key[0][u]);
+                const uint32_t m = bf_mask(h);
+                am &= (bfl[bf_word(h, lw)] & m) == m ? ~0u : ~(1u << u);
             }
-            // home bucket: match, or an empty slot (absent), or full (continue: kDead - 1)
-            uint32_t cont = 0;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t kk = (uint32_t)key[q][u];
-                const uint32_t b4 = 4u * hb[u];
-                uint32_t sl = v[u].w == kk ? b4 + 3u : kDead;
-                sl = v[u].z == kk ? b4 + 2u : sl;
-                sl = v[u].y == kk ? b4 + 1u : sl;
-                sl = v[u].x == kk ? b4 : sl;
-                const bool full = v[u].x != kEmptyKey && v[u].y != kEmptyKey &&
-                                  v[u].z != kEmptyKey && v[u].w != kEmptyKey;
-                sl = (sl == kDead && full) ? kDead - 1u : sl;
-                sl = kk == kEmptyKey ? (tsp[q] ? (1u << kq) : kDead) : sl;
-                sl = ((am >> u) & 1u) ? sl : kDead;
-                cont |= sl == kDead - 1u ? 1u << u : 0u;
-                brow[q][u] = sl;
-            }
-            if (__ballot(cont != 0)) {  // a full home bucket: rare at load factor <= 1/8
-                const uint32_t bmask = (1u << (kq - 2)) - 1u;
+                const uint64_t bits = __ballot((am >> u) & 1u);
+                if (!bits) continue;
+                const uint32_t slot = qc + (uint32_t)__popcll(bits & lanemask_lt());
+                if ((am >> u) & 1u) {
+                    qbuf[w][0][slot] = r[u];
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if ((cont >> u) & 1u)
-                        brow[q][u] = bucket_resolve(tkey[q], bmask, (hb[u] + 1u) & bmask,
-                                                    (uint32_t)key[q][u],
-                                                    load_bucket(tkey[q], (hb[u] + 1u) & bmask));
+                    for (int q = 0; q < NS; ++q) qbuf[w][1 + q][slot] = (uint32_t)key[q][u];
+                }
+                qc += (uint32_t)__popcll(bits);
+                if (qc >= HJ_WAVE) {
+                    queue_flush<NS>(qbuf[w], HJ_WAVE, lane, tkey, trow, tk, tsp, &s_cnt, pd, st0);
+                    queue_shift<NS>(qbuf[w], qc - HJ_WAVE, lane);
+                    qc -= HJ_WAVE;
+                }
             }
+            continue;
+        } else {
+            // joins in order, each on the lanes the previous ones kept
 #pragma unroll
-            for (int u = 0; u < U; ++u) am &= brow[q][u] != kDead ? ~0u : ~(1u << u);
+            for (int q = 0; q < NS; ++q) {
+                am = valid_mask<U>(am, kv[q], gb, g_hi, lane);
+                if (!__ballot(am != 0)) continue;  // uniform: every later join is skipped too
+                u32x4 v[U];
+                uint32_t hb[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    hb[u] = ((am >> u) & 1u) ? home_bucket((uint32_t)key[q][u], tk[q]) : 0u;
+                    v[u] = load_bucket(tkey[q], hb[u]);
+                }
+                uint32_t cont = 0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    brow[q][u] = resolve_home(v[u], (uint32_t)key[q][u], hb[u], tk[q], tsp[q],
+                                              (am >> u) & 1u);
+                    cont |= brow[q][u] == kDead - 1u ? 1u << u : 0u;
+                }
+                resolve_rest<U>(brow[q], cont, tkey[q], tk[q], hb, key[q]);
+#pragma unroll
+                for (int u = 0; u < U; ++u) am &= brow[q][u] != kDead ? ~0u : ~(1u << u);
+            }
         }
         if (!__ballot(am != 0)) continue;
         uint32_t pos[U];
@@ -404,6 +505,7 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
             for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[u]] = trow[q][brow[q][u]];
         }
     }
+    if (qc) queue_flush<NS>(qbuf[w], qc, lane, tkey, trow, tk, tsp, &s_cnt, pd, st0);
     __syncthreads();
     if (t == 0) gp(pd.blk_cnt)[blockIdx.x] = s_cnt;
 }
@@ -591,6 +693,7 @@ void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s) {
     if (hb.njobs == 0) return;
     const dim3 grid(hb.grid, hb.njobs);
     hipLaunchKernelGGL(k_sb_filter, grid, dim3(kB), 0, s, db);
+    hipLaunchKernelGGL(k_sb_count, dim3(hb.njobs), dim3(1024), 0, s, db);
     hipLaunchKernelGGL(k_sb_clear, grid, dim3(kB), 0, s, db);
     hipLaunchKernelGGL(k_sb_insert, grid, dim3(kB), 0, s, db);
 }
