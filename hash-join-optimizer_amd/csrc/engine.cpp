@@ -760,7 +760,7 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
         t->allocations.push_back(p);
         return p;
     };
-    dc.pages = (uint8_t*)dev_alloc(bytes);
+    dc.pages = (uint8_t*)dev_alloc(bytes + 256);  // slack for wide loads near the end
     if (bytes) {
         std::vector<uint8_t> staging(bytes);
         for (uint32_t p = 0; p < hc.num_pages; ++p)
@@ -804,6 +804,9 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
         v.flat = nullptr;
         v.vbits = nullptr;
         launch_decode_int32(v, dc.flat, dc.vbits, ctx->stream);
+    } else if (hc.type == HJ_VARCHAR) {
+        dc.sref = (uint64_t*)dev_alloc(rows * 8);
+        launch_decode_varchar(dc.view(), dc.sref, ctx->stream);
     }
     HJ_HIP(hipStreamSynchronize(ctx->stream));
 }
@@ -1077,7 +1080,8 @@ int hj_dresult_to_host(hj_context* ctx, const hj_dresult* r, hj_result* result) 
 
 int hj_dresult_free(hj_context* ctx, hj_dresult* r) {
     if (!r) return HJ_E_INVALID;
-    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    // no wait: the result lives in the context's arena, which the next query reuses in stream
+    // order, after this query's kernels
     free_dresult(ctx, r);
     return HJ_OK;
 }

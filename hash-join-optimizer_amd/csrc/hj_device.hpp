@@ -13,7 +13,7 @@
 //   vbits       u64[rows/64 + 1]     validity, bit r & 63 of word r >> 6 (absent: no NULLs)
 // so that row r of an INT32 column is one load (plus one broadcast load of its validity word),
 // and 64 consecutive rows are one coalesced load: the same bytes as the pages, without the
-// page search.
+// page search. VARCHAR columns get sref[rows], one 8-byte reference per row (see DCol).
 // A row r is found by page search (or r / uniform_rows when every page but the last holds the
 // same number of rows, the dense 1984-row case of include/column_t.h:14), then its value index
 // among the page's packed non-NULL values is rank + popcount of the bitmap bits below it.
@@ -44,6 +44,8 @@ struct DCol {
     const uint32_t* chunk_page;  // page holding row c * 2048, for c = 0 .. ceil(rows/2048)
     const int32_t* flat;         // INT32: value per row (0 for NULL)
     const uint64_t* vbits;       // INT32: validity per row; nullptr when the column has no NULL
+    const uint64_t* sref;        // VARCHAR: per row, kNullRef, kLongRef | first page, or
+                                 // (byte offset into pages) << 16 | length
     uint32_t num_pages;
     int32_t type;
     uint32_t uniform_rows;
@@ -69,6 +71,9 @@ __host__ __device__ inline uint32_t bloom_mask(uint16_t s) {
 __host__ __device__ inline uint32_t slot_of(uint32_t h, uint32_t k) {
     return k ? (h >> (32u - k)) : 0u;
 }
+
+constexpr uint64_t kNullRef = ~0ull;        // sref of a NULL string
+constexpr uint64_t kLongRef = 1ull << 63;    // sref of a long string: | its 0xffff page
 
 // ---------------------------------------------------------------- page access
 __device__ inline const HJ_GLOBAL uint8_t* page_ptr(const DCol& c, uint32_t p) {

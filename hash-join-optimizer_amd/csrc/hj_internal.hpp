@@ -87,6 +87,7 @@ struct DevColumn {
     uint32_t* chunk_page = nullptr;
     int32_t* flat = nullptr;       // device, INT32 only: value per row
     uint64_t* vbits = nullptr;     // device, INT32 with NULLs only: validity per row
+    uint64_t* sref = nullptr;      // device, VARCHAR only: string reference per row
     uint32_t uniform_rows = 0;
     uint32_t all_dense = 1;
     std::vector<uint32_t> h_row_start;
@@ -99,6 +100,7 @@ struct DevColumn {
         d.chunk_page = chunk_page;
         d.flat = flat;
         d.vbits = vbits;
+        d.sref = sref;
         d.num_pages = num_pages;
         d.type = type;
         d.uniform_rows = uniform_rows;

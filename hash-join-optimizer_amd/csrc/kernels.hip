@@ -150,6 +150,36 @@ __global__ __launch_bounds__(kBlock) void k_decode_int32(DCol c, int32_t* flat, 
     }
 }
 
+// VARCHAR pages -> one reference per row: one block per page, one row per thread
+__global__ __launch_bounds__(kBlock) void k_decode_varchar(DCol c, uint64_t* sref) {
+    const uint32_t p = blockIdx.x;
+    const HJ_GLOBAL uint8_t* pg = page_ptr(c, p);
+    const uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;
+    const uint32_t nr = hdr & 0xffffu, nn = hdr >> 16;
+    const uint32_t r0 = c.row_start[p];
+    if (nr == 0xfffeu) return;  // continuation page: no row of its own
+    if (nr == 0xffffu) {
+        if (threadIdx.x == 0) sref[r0] = kLongRef | p;
+        return;
+    }
+    const HJ_GLOBAL uint16_t* offs = (const HJ_GLOBAL uint16_t*)(pg + 4);
+    const uint64_t base = (uint64_t)p * HJ_PAGE_BYTES + 4u + 2u * nn;
+    for (uint32_t l = threadIdx.x; l < nr; l += kBlock) {
+        uint32_t vi;
+        uint64_t ref = kNullRef;
+        if (value_index(c, p, pg, hdr, l, &vi)) {
+            const uint32_t start = vi ? offs[vi - 1] : 0u;
+            ref = ((base + start) << 16) | (uint64_t)(offs[vi] - start);
+        }
+        sref[r0 + l] = ref;
+    }
+}
+
+void launch_decode_varchar(const DCol& c, uint64_t* sref, hipStream_t s) {
+    if (c.num_pages == 0) return;
+    hipLaunchKernelGGL(k_decode_varchar, dim3(c.num_pages), dim3(kBlock), 0, s, c, sref);
+}
+
 void launch_decode_int32(const DCol& c, int32_t* flat, uint64_t* vbits, hipStream_t s) {
     if (c.num_pages == 0) return;
     hipLaunchKernelGGL(k_decode_int32, dim3(c.num_pages), dim3(kBlock), 0, s, c, flat, vbits);
@@ -781,78 +811,7 @@ void expand_heavy(const DirView& d, const OutSlots& o, const HeavyEntry* heavy,
 // output row; validity is written as a bitmap, one 64-bit ballot per wave.
 // =====================================================================================
 namespace {
-__global__ __launch_bounds__(kBlock) void k_gather_int32(DCol c, const uint32_t* rid, uint64_t n,
-                                                         int32_t* vals, uint8_t* valid) {
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    int32_t v = 0;
-    bool ok = false;
-    if (i < n) {
-        ok = get_int32(c, rid ? rid[i] : (uint32_t)i, &v);
-        vals[i] = ok ? v : 0;
-    }
-    const uint64_t b = __ballot(ok);
-    if (lane_id() == 0 && i < n) *(uint64_t*)(valid + (i >> 3)) = b;
-}
-
-__global__ __launch_bounds__(kBlock) void k_gather_strlen(DCol c, const uint32_t* rid, uint64_t n,
-                                                          uint64_t* lens, uint8_t* valid) {
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool ok = false;
-    if (i < n) {
-        const uint8_t* src;
-        uint32_t len = 0, lp;
-        ok = get_string(c, rid ? rid[i] : (uint32_t)i, &src, &len, &lp);
-        lens[i] = ok ? len : 0;
-    }
-    const uint64_t b = __ballot(ok);
-    if (lane_id() == 0 && i < n) *(uint64_t*)(valid + (i >> 3)) = b;
-}
-
-__global__ __launch_bounds__(kBlock) void k_gather_strcopy(DCol c, const uint32_t* rid,
-                                                           uint64_t n, const uint64_t* offsets,
-                                                           char* chars) {
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint8_t* src;
-    uint32_t len, lp;
-    if (!get_string(c, rid ? rid[i] : (uint32_t)i, &src, &len, &lp)) return;
-    char* dst = chars + offsets[i];
-    if (lp == 0xffffffffu) {
-        for (uint32_t j = 0; j < len; ++j) dst[j] = (char)src[j];
-        return;
-    }
-    // long string: the 0xffff page and its 0xfffe continuation pages, in order
-    uint32_t q = lp, done = 0;
-    while (done < len && q < c.num_pages) {
-        const HJ_GLOBAL uint8_t* pg = page_ptr(c, q);
-        const uint32_t h = *(const HJ_GLOBAL uint32_t*)pg;
-        if (q != lp && (h & 0xffffu) != 0xfffeu) break;
-        const uint32_t l = h >> 16;
-        for (uint32_t j = 0; j < l && done + j < len; ++j) dst[done + j] = (char)pg[4 + j];
-        done += l;
-        ++q;
-    }
-}
 }  // namespace
-
-void gather_int32(const DCol& c, const uint32_t* rid, uint64_t n, int32_t* vals, uint8_t* valid,
-                  hipStream_t s) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_gather_int32, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, c, rid, n, vals, valid);
-}
-void gather_strlen(const DCol& c, const uint32_t* rid, uint64_t n, uint64_t* lens,
-                   uint8_t* valid, hipStream_t s) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_gather_strlen, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, c, rid, n, lens, valid);
-}
-void gather_strcopy(const DCol& c, const uint32_t* rid, uint64_t n, const uint64_t* offsets,
-                    char* chars, hipStream_t s) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_gather_strcopy, dim3((uint32_t)((n + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, s, c, rid, n, offsets, chars);
-}
 
 // =====================================================================================
 // Directory test surface (raw u32 keys)

@@ -67,6 +67,8 @@ void launch_rank_index(const DCol& c, uint16_t* rank_out, hipStream_t s);
 // INT32 pages -> flat values + row-space validity bitmap (vbits zeroed beforehand, or nullptr
 // for a column without NULLs)
 void launch_decode_int32(const DCol& c, int32_t* flat, uint64_t* vbits, hipStream_t s);
+// VARCHAR pages -> sref[rows] (kNullRef / kLongRef | page / offset << 16 | length)
+void launch_decode_varchar(const DCol& c, uint64_t* sref, hipStream_t s);
 
 // ---- generic exclusive scans (tile = 4096 elements per block)
 size_t scan_partials_needed(uint64_t n);
@@ -114,13 +116,6 @@ void select_scan_count(const ScanSrc& src, uint32_t* counts, hipStream_t s);
 void select_scan_write(const ScanSrc& src, const uint32_t* offsets, uint32_t* out_row,
                        hipStream_t s);
 
-// ---- late materialization
-void gather_int32(const DCol& c, const uint32_t* rid, uint64_t n, int32_t* vals, uint8_t* valid,
-                  hipStream_t s);
-void gather_strlen(const DCol& c, const uint32_t* rid, uint64_t n, uint64_t* lens,
-                   uint8_t* valid, hipStream_t s);
-void gather_strcopy(const DCol& c, const uint32_t* rid, uint64_t n, const uint64_t* offsets,
-                    char* chars, hipStream_t s);
 
 // ---- directory test surface: raw u32 keys
 void hash_keys(const uint32_t* keys, uint64_t n, uint32_t* h, uint32_t* row, hipStream_t s);

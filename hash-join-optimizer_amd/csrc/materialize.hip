@@ -7,16 +7,35 @@ namespace {
 constexpr int kB = 256;
 constexpr int kRowsPerThread = kMatTile / kB;
 
-// copy n bytes: 16 independent loads in flight per round, then their stores
-__device__ inline void copy_bytes(HJ_GLOBAL char* dst, const HJ_GLOBAL uint8_t* src, uint32_t n) {
-    for (uint32_t j0 = 0; j0 < n; j0 += 16) {
-        uint8_t t[16];
+// copy n bytes in 64-byte rounds: 17 independent aligned dword loads (the source pages carry
+// slack past their end), realigned in registers, then byte stores
+__device__ inline void copy_span(HJ_GLOBAL char* dst, const HJ_GLOBAL uint8_t* src, uint32_t n) {
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uintptr_t a = (uintptr_t)(src + j0);
+        const HJ_GLOBAL uint32_t* w = (const HJ_GLOBAL uint32_t*)(a & ~(uintptr_t)3);
+        const uint32_t sh = (uint32_t)(a & 3u);
+        uint32_t d[17];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) t[k] = j0 + k < n ? src[j0 + k] : (uint8_t)0;
+        for (int k = 0; k < 17; ++k) d[k] = w[k];
+        uint32_t o[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (j0 + k < n) dst[j0 + k] = (char)t[k];
+        for (int k = 0; k < 16; ++k) o[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], sh);
+        const uint32_t m = n - j0 < 64u ? n - j0 : 64u;
+#pragma unroll
+        for (int k = 0; k < 64; ++k)
+            if ((uint32_t)k < m) dst[j0 + k] = (char)(o[k >> 2] >> (8 * (k & 3)));
     }
+}
+
+// total bytes of the long string starting at its 0xffff page p
+__device__ inline uint64_t long_length(const DCol& c, uint32_t p) {
+    uint64_t total = *(const HJ_GLOBAL uint32_t*)page_ptr(c, p) >> 16;
+    for (uint32_t q = p + 1; q < c.num_pages; ++q) {
+        const uint32_t h = *(const HJ_GLOBAL uint32_t*)page_ptr(c, q);
+        if ((h & 0xffffu) != 0xfffeu) break;
+        total += h >> 16;
+    }
+    return total;
 }
 
 // one 1024-row tile of column blockIdx.y; rows i = tile * 1024 + k * 256 + threadIdx.x, so each
@@ -38,12 +57,12 @@ __global__ __launch_bounds__(kB) void k_mat_a(MatBatch mb) {
                 ok = get_int32(mc.col, r, &v);
                 gp(mc.values)[i] = ok ? v : 0;
             } else {
-                const uint8_t* src = nullptr;
-                uint32_t len = 0, lp = 0xffffffffu;
-                ok = get_string(mc.col, r, &src, &len, &lp);
-                const uint64_t l = ok ? len : 0u;
+                const uint64_t ref = gp(mc.col.sref)[r];
+                ok = ref != kNullRef;
+                uint64_t l = 0;
+                if (ok) l = (ref & kLongRef) ? long_length(mc.col, (uint32_t)ref) : (ref & 0xffffu);
                 gp(mc.offsets)[i + 1] = l;
-                gp(mc.src)[i] = lp == 0xffffffffu ? (uint64_t)(uintptr_t)src : (kLongSrc | lp);
+                gp(mc.src)[i] = ref;
                 bytes += l;
             }
         }
@@ -95,21 +114,22 @@ __global__ __launch_bounds__(kB) void k_mat_c(MatBatch mb) {
         if (i >= n) continue;
         off[i + 1] = start + len;
         if (!len) continue;
-        const uint64_t sw = gp(mc.src)[i];
+        const uint64_t ref = gp(mc.src)[i];
         HJ_GLOBAL char* dst = gp(mc.chars) + start;
-        if (!(sw & kLongSrc)) {
-            copy_bytes(dst, (const HJ_GLOBAL uint8_t*)(uintptr_t)sw, (uint32_t)len);
+        if (!(ref & kLongRef)) {
+            copy_span(dst, page_ptr(mc.col, 0) + (ref >> 16), (uint32_t)len);
             continue;
         }
         // long string: the 0xffff page and its 0xfffe continuation pages, in order
-        const uint32_t lp = (uint32_t)(sw & 0xffffffffu);
-        uint32_t q = lp, done = 0;
+        const uint32_t lp = (uint32_t)ref;
+        uint32_t q = lp;
+        uint64_t done = 0;
         while (done < len && q < mc.col.num_pages) {
             const HJ_GLOBAL uint8_t* pg = page_ptr(mc.col, q);
             const uint32_t h = *(const HJ_GLOBAL uint32_t*)pg;
             if (q != lp && (h & 0xffffu) != 0xfffeu) break;
             const uint32_t pl = (h >> 16) < len - done ? (h >> 16) : (uint32_t)(len - done);
-            copy_bytes(dst + done, pg + 4, pl);
+            copy_span(dst + done, pg + 4, pl);
             done += pl;
             ++q;
         }

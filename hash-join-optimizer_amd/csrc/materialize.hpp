@@ -29,10 +29,8 @@ struct MatCol {
     uint64_t* offsets;     // VARCHAR, n + 1
     char* chars;           // VARCHAR
     uint64_t* tile_sum;    // VARCHAR, tiles + 1 (the last entry receives the total)
-    uint64_t* src;         // VARCHAR, n: source of row i's bytes (address, or kLongSrc | page)
+    uint64_t* src;         // VARCHAR, n: the string reference (sref) of row i
 };
-
-constexpr uint64_t kLongSrc = 1ull << 63;  // a long string: its first (0xffff) page follows
 
 struct MatBatch {
     MatCol c[kMatCols];

@@ -90,7 +90,13 @@ class Context:
         return DeviceTable(self, h, table.num_rows)
 
     def execute_device(self, plan, dtables):
-        p, keep = plan.to_c(with_inputs=False)
+        # the C plan is rebuilt only when the plan changed (node count, root)
+        sig = (len(plan.nodes), plan.root)
+        cached = getattr(plan, "_c_device_plan", None)
+        if cached is None or cached[0] != sig:
+            cached = (sig,) + plan.to_c(with_inputs=False)
+            plan._c_device_plan = cached
+        _, p, keep = cached
         arr = (C.c_void_p * max(1, len(dtables)))(*[t.handle for t in dtables])
         h = C.c_void_p()
         self._check(self.lib.hj_execute_device(self.handle, C.byref(p), arr, C.byref(h)))

@@ -153,6 +153,9 @@ int hj_dresult_to_host(hj_context* ctx, const hj_dresult* r, hj_result* result);
  * byte i/8); VARCHAR -> values = uint64 offsets[n+1] into chars, chars, validity. */
 int hj_dresult_column(const hj_dresult* r, uint32_t c, int32_t* type, const void** values,
                       const void** chars, const uint8_t** validity);
+/* The device columns are complete for work ordered after the query on the context's stream
+ * (hj_context_stream), or after hj_synchronize. Freeing does not wait: the memory is reused by
+ * the context's next query, in stream order. */
 int hj_dresult_free(hj_context* ctx, hj_dresult* r);
 /* stream the context launches on (hipStream_t), for event timing by the caller */
 void* hj_context_stream(hj_context* ctx);
