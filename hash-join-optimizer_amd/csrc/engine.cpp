@@ -449,8 +449,7 @@ class Executor {
             jb.st_key = scratch<uint32_t>(scap);
             jb.st_row = scratch<uint32_t>(scap);
             jb.blk = scratch<uint32_t>(bgrid);
-            jb.tkey = scratch<uint32_t>(1ull << jb.kmax);
-            jb.trow = scratch<uint32_t>((1ull << jb.kmax) + 1);
+            jb.table = scratch<uint32_t>(2ull * (1ull << jb.kmax) + 8);
             jb.bf = scratch<uint32_t>(kBloomWords);
 
             PipeStage& st = hp.st[i];

@@ -85,7 +85,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte buck
 __device__ inline uint32_t home_bucket(uint32_t key, uint32_t k) { return fib32(key) >> (34u - k); }
 
 __device__ inline u32x4 load_bucket(const HJ_GLOBAL uint32_t* K, uint32_t b) {
-    return *(const HJ_GLOBAL u32x4*)(K + 4u * b);
+    return *(const HJ_GLOBAL u32x4*)(K + 8u * b);
 }
 
 // slot of `key` (!= kEmptyKey), kDead when absent, given the keys v of its home bucket b;
@@ -205,9 +205,9 @@ __global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ 
     const uint32_t n = gp(jb.meta)[0];
     const uint32_t nbk = (1u << table_bits(n)) / kBucketKeys;
     const uint32_t g = blockIdx.x * kB + threadIdx.x, stride = gridDim.x * kB;
-    HJ_GLOBAL u32x4* T = (HJ_GLOBAL u32x4*)gp(jb.tkey);
+    HJ_GLOBAL u32x4* T = (HJ_GLOBAL u32x4*)gp(jb.table);
     const u32x4 empty = {kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
-    for (uint32_t i = g; i < nbk; i += stride) T[i] = empty;
+    for (uint32_t i = g; i < nbk; i += stride) T[2u * i] = empty;  // the keys half of bucket i
     if (bf_enabled(n)) {
         const uint32_t W = 1u << bf_log2w(n);
         for (uint32_t i = g; i < W; i += stride) gp(jb.bf)[i] = 0;
@@ -223,23 +223,22 @@ __global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__
     const uint32_t lw = bf_log2w(n);
     const uint32_t r0 = blockIdx.x * build_range(jb.rows, gridDim.x);
     const uint32_t cnt = gp(jb.blk)[blockIdx.x];
-    HJ_GLOBAL uint32_t* K = gp(jb.tkey);
-    HJ_GLOBAL uint32_t* R = gp(jb.trow);
+    HJ_GLOBAL uint32_t* K = gp(jb.table);
     for (uint32_t i = threadIdx.x; i < cnt; i += kB) {
         const uint32_t key = gp(jb.st_key)[r0 + i], row = gp(jb.st_row)[r0 + i];
         if (key == kEmptyKey) {  // kept aside in slot 2^k
             if (atomicCAS((uint32_t*)&gp(jb.meta)[2], 0u, 1u) != 0u)
                 gp(jb.meta)[1] = 1;
             else
-                R[1u << k] = row;
+                K[row_word(1u << k)] = row;
         } else {
             bool done = false;
             for (uint32_t b = home_bucket(key, k); !done; b = (b + 1u) & bmask) {
                 for (uint32_t j = 0; j < kBucketKeys && !done; ++j) {
                     const uint32_t s = 4u * b + j;
-                    const uint32_t old = atomicCAS((uint32_t*)&K[s], kEmptyKey, key);
+                    const uint32_t old = atomicCAS((uint32_t*)&K[8u * b + j], kEmptyKey, key);
                     if (old == kEmptyKey) {
-                        R[s] = row;
+                        K[row_word(s)] = row;
                         done = true;
                     } else if (old == key) {
                         gp(jb.meta)[1] = 1;
@@ -309,7 +308,6 @@ constexpr uint32_t kQueue = 128;  // per-wave queue of rows that passed the LDS 
 template <int NS>
 __device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, uint32_t lane,
                                    const HJ_GLOBAL uint32_t* const (&tkey)[NS],
-                                   const HJ_GLOBAL uint32_t* const (&trow)[NS],
                                    const uint32_t (&tk)[NS], const uint32_t (&tsp)[NS],
                                    uint32_t* s_cnt, const PipeDesc& pd, uint32_t st0) {
     __builtin_amdgcn_wave_barrier();
@@ -338,7 +336,8 @@ __device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, u
     if (alive) {
         gp(pd.stage[0])[st0 + pos[0]] = row;
 #pragma unroll
-        for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[0]] = trow[q][slot[q][0]];
+        for (int q = 0; q < NS; ++q)
+            gp(pd.stage[1 + q])[st0 + pos[0]] = tkey[q][row_word(slot[q][0])];
     }
 }
 
@@ -387,7 +386,6 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     const HJ_GLOBAL int32_t* kf[NS];
     const HJ_GLOBAL uint64_t* kv[NS];
     const HJ_GLOBAL uint32_t* tkey[NS];
-    const HJ_GLOBAL uint32_t* trow[NS];
     uint32_t tk[NS], tsp[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -395,8 +393,7 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
         kf[q] = gp(ps.keycol.flat);
         kv[q] = gp(ps.keycol.vbits);
         const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
-        tkey[q] = gp(jp->tkey);
-        trow[q] = gp(jp->trow);
+        tkey[q] = gp(jp->table);
         tk[q] = table_bits(gp(jp->meta)[0]);
         tsp[q] = gp(jp->meta)[2];
     }
@@ -463,7 +460,7 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
                 }
                 qc += (uint32_t)__popcll(bits);
                 if (qc >= HJ_WAVE) {
-                    queue_flush<NS>(qbuf[w], HJ_WAVE, lane, tkey, trow, tk, tsp, &s_cnt, pd, st0);
+                    queue_flush<NS>(qbuf[w], HJ_WAVE, lane, tkey, tk, tsp, &s_cnt, pd, st0);
                     queue_shift<NS>(qbuf[w], qc - HJ_WAVE, lane);
                     qc -= HJ_WAVE;
                 }
@@ -502,10 +499,11 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
             if (!((am >> u) & 1u)) continue;
             gp(pd.stage[0])[st0 + pos[u]] = r[u];
 #pragma unroll
-            for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[u]] = trow[q][brow[q][u]];
+            for (int q = 0; q < NS; ++q)
+                gp(pd.stage[1 + q])[st0 + pos[u]] = tkey[q][row_word(brow[q][u])];
         }
     }
-    if (qc) queue_flush<NS>(qbuf[w], qc, lane, tkey, trow, tk, tsp, &s_cnt, pd, st0);
+    if (qc) queue_flush<NS>(qbuf[w], qc, lane, tkey, tk, tsp, &s_cnt, pd, st0);
     __syncthreads();
     if (t == 0) gp(pd.blk_cnt)[blockIdx.x] = s_cnt;
 }
@@ -541,8 +539,7 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
     const uint32_t n = gp(pd.meta)[2];
     const PipeStage& ps = pd.st[s];
     const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
-    const HJ_GLOBAL uint32_t* tkey = gp(jp->tkey);
-    const HJ_GLOBAL uint32_t* trow = gp(jp->trow);
+    const HJ_GLOBAL uint32_t* tkey = gp(jp->table);
     const uint32_t k = table_bits(gp(jp->meta)[0]), bmask = (1u << (k - 2)) - 1u;
     const uint32_t special = gp(jp->meta)[2] ? (1u << k) : kDead;
     const HJ_GLOBAL uint32_t* prev = previous < 0 ? nullptr : gp(pd.l_b[previous]);
@@ -582,7 +579,7 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
                 const uint32_t kk = (uint32_t)key[u];
                 slot = kk == kEmptyKey ? special : bucket_resolve(tkey, bmask, hb[u], kk, v[u]);
             }
-            outb[jx[u]] = slot == kDead ? kDead : trow[slot];
+            outb[jx[u]] = slot == kDead ? kDead : tkey[row_word(slot)];
         }
     }
 }

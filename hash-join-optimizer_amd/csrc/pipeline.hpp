@@ -19,6 +19,9 @@ constexpr uint32_t kBloomWords = 4096;       // 16 KiB: the LDS-resident filter 
 constexpr uint32_t kBucketKeys = 4;          // one 16-byte load reads a whole bucket
 constexpr uint32_t kMinTableBits = 6;
 
+// word index of slot s's row in a table
+__host__ __device__ inline uint32_t row_word(uint32_t s) { return 8u * (s >> 2) + 4u + (s & 3u); }
+
 // key slots for n build tuples: 2^bits >= 8n (load factor <= 1/8, about half a key per
 // 4-key bucket, so a lookup nearly always ends in its home bucket)
 __host__ __device__ inline uint32_t table_bits(uint32_t n) {
@@ -28,9 +31,10 @@ __host__ __device__ inline uint32_t table_bits(uint32_t n) {
 }
 
 // One build side: the rows of a (filtered) base scan with a non-NULL key, in a bucketized
-// open-addressing table: tkey holds the keys (buckets of kBucketKeys consecutive slots, linear
-// probing over buckets), trow the build row of each occupied slot. A key equal to kEmptyKey is
-// kept in slot 2^bits of trow, flagged by meta[2].
+// open-addressing table (linear probing over buckets). A bucket is 32 bytes: kBucketKeys keys,
+// then the build rows of those key slots, so an insert touches one cache line and a lookup
+// reads the keys with one 16-byte load. Slot s is key (s & 3) of bucket s >> 2. A key equal to
+// kEmptyKey is kept aside in slot 2^bits (its row at table_row(2^bits)), flagged by meta[2].
 struct BuildJob {
     DCol key;
     DCol fcol[kMaxFilters];
@@ -43,8 +47,7 @@ struct BuildJob {
     uint32_t* st_key;   // staged tuples, each block of the filter pass at the start of its
     uint32_t* st_row;   // own row range (capacity: rows rounded up to whole ranges)
     uint32_t* blk;      // tuples staged per block of the filter pass
-    uint32_t* tkey;     // 2^table_bits(tuples) key slots used
-    uint32_t* trow;     // 2^kmax + 1 build rows
+    uint32_t* table;    // 2 * 2^kmax + 8 words; 2^table_bits(tuples) key slots used
     uint32_t* bf;       // blocked Bloom filter, kBloomWords words (small build sides only)
 };
 
