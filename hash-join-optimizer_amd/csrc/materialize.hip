@@ -38,7 +38,7 @@ __device__ inline uint64_t long_length(const DCol& c, uint32_t p) {
     return total;
 }
 
-// one 1024-row tile of column blockIdx.y; rows i = tile * 1024 + k * 256 + threadIdx.x, so each
+// one tile of column blockIdx.y; rows i = tile * kMatTile + k * 256 + threadIdx.x, so each
 // wave's 64 rows of a k-slice are consecutive and its validity ballot is one aligned word
 __global__ __launch_bounds__(kB) void k_mat_a(MatBatch mb) {
     __shared__ uint64_t lds[kB / HJ_WAVE];

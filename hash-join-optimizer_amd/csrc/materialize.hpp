@@ -1,7 +1,7 @@
 // materialize.hpp - output columns of a join result gathered in one batch of launches.
 //
 // For up to kMatCols output columns per launch (descriptors passed by value):
-//   pass a   per 1024-row tile and column: INT32 values + validity; VARCHAR lengths (kept in
+//   pass a   per 256-row tile and column: INT32 values + validity; VARCHAR lengths (kept in
 //            offsets[i+1]), validity and the tile's byte count
 //   pass b   per VARCHAR column, one block: exclusive scan of the tile byte counts
 //   pass c   per tile and VARCHAR column: offsets (tile base + in-tile scan) and the bytes
@@ -17,7 +17,7 @@
 namespace hj {
 
 constexpr int kMatCols = 8;
-constexpr uint32_t kMatTile = 1024;
+constexpr uint32_t kMatTile = 256;  // rows per block: one per thread
 
 struct MatCol {
     DCol col;

@@ -28,26 +28,20 @@ __device__ inline uint32_t log2ceil(uint32_t n) { return n <= 1 ? 0u : 32u - __c
 
 // Blocked Bloom filter of a build side, held in LDS by the probe: one 32-bit word per key with
 // 4 bits set, about 16 bits per key; only used when it gets >= 8 bits per key within
-// kBloomWords words. The word and bit hashes apply the "lowbias32" integer mixer (C. Wellons,
-// hash-prospector) to the key's Fibonacci hash.
-__device__ inline uint32_t mix32(uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x7feb352du;
-    x ^= x >> 15;
-    x *= 0x846ca68bu;
-    x ^= x >> 16;
-    return x;
-}
+// kBloomWords words. Word and bits come from the high bits of two more multiplicative hashes of
+// the key's Fibonacci hash (high bits of a product depend on every bit of the operand).
 __device__ inline bool bf_enabled(uint32_t n) { return n && (uint64_t)n * 8u <= kBloomWords * 32u; }
 __device__ inline uint32_t bf_log2w(uint32_t n) {
     uint32_t b = log2ceil((n + 1u) / 2u);  // n/2 words = 16 bits per key
     return b < 5u ? 5u : (b > 12u ? 12u : b);
 }
-__device__ inline uint32_t bf_word(uint32_t h, uint32_t lw) { return mix32(h) >> (32u - lw); }
+__device__ inline uint32_t bf_word(uint32_t h, uint32_t lw) {
+    return ((h ^ (h >> 15)) * 0x2C1B3C6Du) >> (32u - lw);
+}
 __device__ inline uint32_t bf_mask(uint32_t h) {
-    const uint32_t g = mix32(h ^ 0x9E3779B9u);
-    return (1u << (g & 31u)) | (1u << ((g >> 5) & 31u)) | (1u << ((g >> 10) & 31u)) |
-           (1u << ((g >> 15) & 31u));
+    const uint32_t g = (h ^ (h >> 13)) * 0x297A2D39u;
+    return (1u << (g >> 27)) | (1u << ((g >> 22) & 31u)) | (1u << ((g >> 17) & 31u)) |
+           (1u << ((g >> 12) & 31u));
 }
 
 template <int NW>
@@ -207,7 +201,7 @@ __global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ 
     const uint32_t g = blockIdx.x * kB + threadIdx.x, stride = gridDim.x * kB;
     HJ_GLOBAL u32x4* T = (HJ_GLOBAL u32x4*)gp(jb.table);
     const u32x4 empty = {kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
-    for (uint32_t i = g; i < nbk; i += stride) T[2u * i] = empty;  // the keys half of bucket i
+    for (uint32_t i = g; i < 2u * nbk; i += stride) T[i] = empty;  // whole buckets: full lines
     if (bf_enabled(n)) {
         const uint32_t W = 1u << bf_log2w(n);
         for (uint32_t i = g; i < W; i += stride) gp(jb.bf)[i] = 0;
@@ -305,30 +299,33 @@ constexpr uint32_t kQueue = 128;  // per-wave queue of rows that passed the LDS 
 
 // look up every join for the first n (<= 64) entries of a wave's queue, one entry per lane, and
 // stage the rows that match all of them (row, then the build row of each join in probe order)
+// look up the joins for the first n (<= 64) entries of a wave's queue, one entry per lane, and
+// stage the rows that match all of them (row, then the build row of each join in probe order).
+// The joins run as a cascade, each on the lanes the previous ones kept: first those without an
+// LDS filter (probe order), last the LDS-filtered first join, whose filter already dropped most
+// non-matching rows, so its table is read only for the final candidates.
 template <int NS>
 __device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, uint32_t lane,
                                    const HJ_GLOBAL uint32_t* const (&tkey)[NS],
                                    const uint32_t (&tk)[NS], const uint32_t (&tsp)[NS],
                                    uint32_t* s_cnt, const PipeDesc& pd, uint32_t st0) {
     __builtin_amdgcn_wave_barrier();
-    const uint32_t live = lane < n ? 1u : 0u;
+    uint32_t alive = lane < n ? 1u : 0u;
     const uint32_t row = qb[0][lane];
-    int32_t key[NS][1];
-    uint32_t hb[NS][1], slot[NS][1];
-    u32x4 v[NS];
+    uint32_t slot[NS];
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-        key[q][0] = (int32_t)qb[1 + q][lane];
-        hb[q][0] = live ? home_bucket((uint32_t)key[q][0], tk[q]) : 0u;
-        v[q] = load_bucket(tkey[q], hb[q][0]);
-    }
-    uint32_t alive = live;
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-        slot[q][0] = resolve_home(v[q], (uint32_t)key[q][0], hb[q][0], tk[q], tsp[q], live);
-        resolve_rest<1>(slot[q], slot[q][0] == kDead - 1u ? 1u : 0u, tkey[q], tk[q], hb[q],
-                        key[q]);
-        alive &= slot[q][0] != kDead ? 1u : 0u;
+    for (int i = 0; i < NS; ++i) {
+        const int q = i + 1 < NS ? i + 1 : 0;  // joins 1 .. NS-1, then join 0
+        if (!__ballot(alive)) return;
+        const uint32_t kk = qb[1 + q][lane];
+        uint32_t hb[1] = {alive ? home_bucket(kk, tk[q]) : 0u};
+        u32x4 v = u32x4{kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
+        if (i == 0 || alive) v = load_bucket(tkey[q], hb[0]);  // the first join: all lanes
+        uint32_t sl[1] = {resolve_home(v, kk, hb[0], tk[q], tsp[q], alive)};
+        const int32_t key1[1] = {(int32_t)kk};
+        resolve_rest<1>(sl, sl[0] == kDead - 1u ? 1u : 0u, tkey[q], tk[q], hb, key1);
+        slot[q] = sl[0];
+        alive &= sl[0] != kDead ? 1u : 0u;
     }
     if (!__ballot(alive)) return;
     uint32_t pos[1];
@@ -337,7 +334,7 @@ __device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, u
         gp(pd.stage[0])[st0 + pos[0]] = row;
 #pragma unroll
         for (int q = 0; q < NS; ++q)
-            gp(pd.stage[1 + q])[st0 + pos[0]] = tkey[q][row_word(slot[q][0])];
+            gp(pd.stage[1 + q])[st0 + pos[0]] = tkey[q][row_word(slot[q])];
     }
 }
 
@@ -409,7 +406,18 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     const uint32_t st0 = g_lo * 64u;
     const uint32_t rmax = nsrc - 1u;
     uint32_t qc = 0;  // entries in this wave's queue (uniform)
-    for (uint32_t gb = g_lo / U + w; gb * U < g_hi; gb += kNW) {
+    // the keys of every streamed join, NS * U coalesced loads, are fetched one iteration ahead
+    // so that they are in flight while the current groups are filtered and looked up
+    int32_t knext[NS][U];
+    const uint32_t gb0 = g_lo / U + w;
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t rr = (gb0 * U + u) * 64u + lane;
+            knext[q][u] = kf[q][rr < rmax ? rr : rmax];
+        }
+    for (uint32_t gb = gb0; gb * U < g_hi; gb += kNW) {
         uint32_t r[U], am = 0;
         int32_t key[NS][U];
 #pragma unroll
@@ -417,11 +425,14 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
             r[u] = (gb * U + u) * 64u + lane;
             am |= (r[u] < nsrc && gb * U + u < g_hi) ? 1u << u : 0u;
         }
-        // keys of every streamed join first: NS * U coalesced loads in flight
 #pragma unroll
         for (int q = 0; q < NS; ++q)
 #pragma unroll
-            for (int u = 0; u < U; ++u) key[q][u] = kf[q][r[u] < rmax ? r[u] : rmax];
+            for (int u = 0; u < U; ++u) {
+                key[q][u] = knext[q][u];
+                const uint32_t rn = ((gb + kNW) * U + u) * 64u + lane;
+                knext[q][u] = kf[q][rn < rmax ? rn : rmax];
+            }
         for (int f = 0; f < nf; ++f) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
