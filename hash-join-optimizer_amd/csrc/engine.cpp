@@ -494,6 +494,8 @@ class Executor {
             res.attrs.push_back({q, a.col, column(slot_table[a.slot], a.col).type});
         }
         const uint64_t nsrc = srel.n;
+        // the fused pass addresses the source's key columns with 32-bit byte offsets
+        if (nsrc >= (1ull << 29)) return false;
         hp.src_rows = (uint32_t)nsrc;
         hp.nout = (int32_t)oslots.size();
         const uint64_t ncap = nsrc ? nsrc : 1;

@@ -123,26 +123,6 @@ __device__ inline void block_append(uint32_t* lds_cnt, const bool (&pred)[U], ui
     }
 }
 
-// block_append over a bit mask of U predicates (bit u = item u)
-template <int U>
-__device__ inline void block_append_mask(uint32_t* lds_cnt, uint32_t mask, uint32_t (&pos)[U]) {
-    uint64_t b[U];
-    uint32_t tot = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        b[u] = __ballot((mask >> u) & 1u);
-        tot += (uint32_t)__popcll(b[u]);
-    }
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(lds_cnt, tot);
-    base = __shfl(base, 0, HJ_WAVE);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        pos[u] = base + (uint32_t)__popcll(b[u] & lanemask_lt());
-        base += (uint32_t)__popcll(b[u]);
-    }
-}
-
 // filter pass: block x of job y takes rows [x * range, (x + 1) * range) and stages the tuples
 // that pass at the start of that range
 __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__ bb) {
@@ -249,25 +229,37 @@ __global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__
 }
 
 // ---------------------------------------------------------------- fused probe
-// drop the lanes whose key in this join is NULL (vb: the key column's validity words, uniform
-// per 64-row group; nullptr when the column has no NULL)
-template <int U>
-__device__ inline uint32_t valid_mask(uint32_t am, const HJ_GLOBAL uint64_t* vb, uint32_t gb,
-                                      uint32_t g_hi, uint32_t lane) {
-    if (!vb) return am;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const uint32_t g = gb * U + u < g_hi ? gb * U + u : g_hi - 1u;
-        am &= ((vb[g] >> lane) & 1ull) ? ~0u : ~(1u << u);
-    }
-    return am;
+// The streaming loop keeps liveness as 64-bit wave masks (one per 64-row group, in scalar
+// registers): validity words are lane masks already, the LDS filter's result is a ballot, and
+// the lanes of a mask run a step with exactly that execution mask (inverse ballot). Source
+// columns are read with buffer loads: one per-lane offset serves every column and group of an
+// iteration (the group is the instruction's immediate offset), and rows past the end of a
+// column read 0 instead of faulting (the loads of the last groups need no clamp).
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, 32-bit data format (gfx9 family)
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
 }
 
-// key slot of kk from its home bucket's keys v: the slot, kDead (absent, or a dead lane), or
-// kDead - 1 (the home bucket is full: continue in the next buckets). The key kEmptyKey lives
-// in slot 2^k when the build holds it (sp).
+// lanes set in mask m below this lane
+__device__ inline uint32_t mask_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// the lanes of m run what the condition guards (m is wave-uniform)
+__device__ inline bool in_mask(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// 64-bit word of lane u (wave-uniform)
+__device__ inline uint64_t lane_word(uint32_t lo, uint32_t hi, int u) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, u) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)lo, u);
+}
+
+// key slot of kk from its home bucket's keys v: the slot, kDead (absent), or kDead - 1 (the
+// home bucket is full: continue in the next buckets). The key kEmptyKey lives in slot 2^k when
+// the build holds it (sp).
 __device__ inline uint32_t resolve_home(u32x4 v, uint32_t kk, uint32_t hb, uint32_t k,
-                                        uint32_t sp, uint32_t alive) {
+                                        uint32_t sp) {
     const uint32_t b4 = 4u * hb;
     uint32_t sl = v.w == kk ? b4 + 3u : kDead;
     sl = v.z == kk ? b4 + 2u : sl;
@@ -275,30 +267,44 @@ __device__ inline uint32_t resolve_home(u32x4 v, uint32_t kk, uint32_t hb, uint3
     sl = v.x == kk ? b4 : sl;
     const bool full = v.x != kEmptyKey && v.y != kEmptyKey && v.z != kEmptyKey && v.w != kEmptyKey;
     sl = (sl == kDead && full) ? kDead - 1u : sl;
-    sl = kk == kEmptyKey ? (sp ? (1u << k) : kDead) : sl;
-    return alive ? sl : kDead;
+    return kk == kEmptyKey ? (sp ? (1u << k) : kDead) : sl;
 }
 
-// lanes whose home bucket was full continue through the following buckets (rare at load
-// factor <= 1/8; one wave-uniform test when none does)
+// slot of kk in table K (2^k key slots), kDead when absent; one 16-byte load settles nearly
+// every lookup (the lanes whose home bucket is full continue through the following buckets)
+__device__ inline uint32_t table_lookup(const HJ_GLOBAL uint32_t* K, uint32_t k, uint32_t sp,
+                                        uint32_t kk) {
+    const uint32_t hb = home_bucket(kk, k);
+    uint32_t sl = resolve_home(load_bucket(K, hb), kk, hb, k, sp);
+    if (sl == kDead - 1u) {
+        const uint32_t bmask = (1u << (k - 2)) - 1u;
+        const uint32_t b = (hb + 1u) & bmask;
+        sl = bucket_resolve(K, bmask, b, kk, load_bucket(K, b));
+    }
+    return sl;
+}
+
+// block-local append of the lanes of n (uniform) masks through an LDS counter: pos[u] is lane's
+// position among this block's appends for mask u (valid on the lanes of m[u])
 template <int U>
-__device__ inline void resolve_rest(uint32_t (&slot)[U], uint32_t cont,
-                                    const HJ_GLOBAL uint32_t* tkey, uint32_t k,
-                                    const uint32_t (&hb)[U], const int32_t (&key)[U]) {
-    if (!__ballot(cont != 0)) return;
-    const uint32_t bmask = (1u << (k - 2)) - 1u;
+__device__ inline void block_append_masks(uint32_t* lds_cnt, const uint64_t (&m)[U],
+                                          uint32_t (&pos)[U]) {
+    uint32_t tot = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-        if ((cont >> u) & 1u)
-            slot[u] = bucket_resolve(tkey, bmask, (hb[u] + 1u) & bmask, (uint32_t)key[u],
-                                     load_bucket(tkey, (hb[u] + 1u) & bmask));
+    for (int u = 0; u < U; ++u) tot += (uint32_t)__popcll(m[u]);
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(lds_cnt, tot);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        pos[u] = base + mask_rank(m[u]);
+        base += (uint32_t)__popcll(m[u]);
+    }
 }
 
 // NS streamed joins, U 64-row groups per wave and iteration
 constexpr uint32_t kQueue = 128;  // per-wave queue of rows that passed the LDS filter
 
-// look up every join for the first n (<= 64) entries of a wave's queue, one entry per lane, and
-// stage the rows that match all of them (row, then the build row of each join in probe order)
 // look up the joins for the first n (<= 64) entries of a wave's queue, one entry per lane, and
 // stage the rows that match all of them (row, then the build row of each join in probe order).
 // The joins run as a cascade, each on the lanes the previous ones kept: first those without an
@@ -310,27 +316,23 @@ __device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, u
                                    const uint32_t (&tk)[NS], const uint32_t (&tsp)[NS],
                                    uint32_t* s_cnt, const PipeDesc& pd, uint32_t st0) {
     __builtin_amdgcn_wave_barrier();
-    uint32_t alive = lane < n ? 1u : 0u;
+    uint64_t alive = n >= HJ_WAVE ? ~0ull : (1ull << n) - 1ull;
     const uint32_t row = qb[0][lane];
     uint32_t slot[NS];
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
         const int q = i + 1 < NS ? i + 1 : 0;  // joins 1 .. NS-1, then join 0
-        if (!__ballot(alive)) return;
-        const uint32_t kk = qb[1 + q][lane];
-        uint32_t hb[1] = {alive ? home_bucket(kk, tk[q]) : 0u};
-        u32x4 v = u32x4{kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
-        if (i == 0 || alive) v = load_bucket(tkey[q], hb[0]);  // the first join: all lanes
-        uint32_t sl[1] = {resolve_home(v, kk, hb[0], tk[q], tsp[q], alive)};
-        const int32_t key1[1] = {(int32_t)kk};
-        resolve_rest<1>(sl, sl[0] == kDead - 1u ? 1u : 0u, tkey[q], tk[q], hb, key1);
-        slot[q] = sl[0];
-        alive &= sl[0] != kDead ? 1u : 0u;
+        if (!alive) return;
+        uint32_t sl = kDead;
+        if (in_mask(alive)) sl = table_lookup(tkey[q], tk[q], tsp[q], qb[1 + q][lane]);
+        slot[q] = sl;
+        alive &= __ballot(sl != kDead);
     }
-    if (!__ballot(alive)) return;
+    if (!alive) return;
+    uint64_t am[1] = {alive};
     uint32_t pos[1];
-    block_append_mask<1>(s_cnt, alive, pos);
-    if (alive) {
+    block_append_masks<1>(s_cnt, am, pos);
+    if (in_mask(alive)) {
         gp(pd.stage[0])[st0 + pos[0]] = row;
 #pragma unroll
         for (int q = 0; q < NS; ++q)
@@ -352,7 +354,7 @@ __device__ inline void queue_shift(uint32_t (&qb)[NS + 1][kQueue], uint32_t rem,
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int NS, int U>
+template <int NS, int U, bool FILT>
 __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pdp) {
     __shared__ uint32_t bfl[kBloomWords];
     __shared__ uint32_t qbuf[kNW][NS + 1][kQueue];  // [wave][row, key of each join][entry]
@@ -380,139 +382,155 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
         }
         return;
     }
-    const HJ_GLOBAL int32_t* kf[NS];
-    const HJ_GLOBAL uint64_t* kv[NS];
+    const uint32_t nsrc = pd.src_rows;
+    const uint32_t G = (nsrc + 63u) >> 6;
+    __amdgpu_buffer_rsrc_t rk[NS], rv[NS];  // key columns, their validity words (0 bytes: none)
+    uint32_t nonull[NS];                    // ~0: the key column has no NULL
     const HJ_GLOBAL uint32_t* tkey[NS];
     uint32_t tk[NS], tsp[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
         const PipeStage& ps = pd.st[pd.order[q]];
-        kf[q] = gp(ps.keycol.flat);
-        kv[q] = gp(ps.keycol.vbits);
+        const int32_t* kf = ps.keycol.flat;
+        const uint64_t* kv = ps.keycol.vbits;
+        rk[q] = make_rsrc(kf, nsrc * 4u);
+        rv[q] = make_rsrc(kv ? (const void*)kv : (const void*)kf, kv ? G * 8u : 0u);
+        nonull[q] = kv ? 0u : ~0u;
         const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
         tkey[q] = gp(jp->table);
         tk[q] = table_bits(gp(jp->meta)[0]);
         tsp[q] = gp(jp->meta)[2];
     }
-    const uint32_t nsrc = pd.src_rows;
-    const uint32_t G = (nsrc + 63u) >> 6;
-    const int nf = pd.nf;
-    // this block's groups [g_lo, g_hi); survivors are staged from row g_lo * 64 of the stage
-    // arrays on. The loop body avoids per-lane branches (each costs scalar exec-mask work on the
-    // CU's one scalar unit): liveness is a bit mask, key loads are clamped in bounds, and dead
-    // lanes read bucket 0, which the wave's load coalesces into one request.
+    const int nf = FILT ? pd.nf : 0;  // filters of the source scan (their loads are not
+                                       // prefetched: a separate instantiation)
+    // this block's groups [g_lo, g_hi) (g_lo a multiple of U); survivors are staged from row
+    // g_lo * 64 of the stage arrays on
     const uint32_t g_lo = blockIdx.x * pd.probe_gpb;
     const uint32_t g_hi = g_lo + pd.probe_gpb < G ? g_lo + pd.probe_gpb : G;
     const uint32_t st0 = g_lo * 64u;
-    const uint32_t rmax = nsrc - 1u;
     uint32_t qc = 0;  // entries in this wave's queue (uniform)
-    // the keys of every streamed join, NS * U coalesced loads, are fetched one iteration ahead
-    // so that they are in flight while the current groups are filtered and looked up
-    int32_t knext[NS][U];
-    const uint32_t gb0 = g_lo / U + w;
+    // keys and validity words of every streamed join are fetched one iteration ahead into one
+    // of two register sets (alternating), so that they are in flight while the current groups
+    // are filtered and looked up. Lane u < U holds the validity word of group u.
+    const uint32_t lane4 = lane * 4u;
+    const uint32_t lanev = (lane < (uint32_t)U ? lane : (uint32_t)U - 1u) * 8u;
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    int32_t ka[NS][U], kb[NS][U];
+    u32x2 va[NS], vb[NS];
+    auto fetch = [&](int32_t (&kx)[NS][U], u32x2 (&vx)[NS], uint32_t gx) {
+        const uint32_t vo = gx * (U * 256u) + lane4, vv = gx * (U * 8u) + lanev;
 #pragma unroll
-    for (int q = 0; q < NS; ++q)
+        for (int q = 0; q < NS; ++q) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                kx[q][u] = __builtin_amdgcn_raw_buffer_load_b32(rk[q], vo + u * 256u, 0, 0);
+            vx[q] = __builtin_amdgcn_raw_buffer_load_b64(rv[q], vv, 0, 0);
+        }
+    };
+    // one iteration: the U groups of wave-iteration gx, whose keys and validity words are in
+    // key / vw
+    auto step = [&](const int32_t (&key)[NS][U], const u32x2 (&vw)[NS], uint32_t gx) {
+        uint32_t vl = ~0u, vh = ~0u;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            vl &= vw[q].x | nonull[q];
+            vh &= vw[q].y | nonull[q];
+        }
+        // live rows of each group: in this block's range, non-NULL in every streamed join
+        uint64_t live[U];
+        uint32_t r[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t rr = (gb0 * U + u) * 64u + lane;
-            knext[q][u] = kf[q][rr < rmax ? rr : rmax];
+            const uint32_t g = gx * U + u;
+            const uint32_t rem = g < g_hi ? nsrc - g * 64u : 0u;
+            live[u] = (rem >= 64u ? ~0ull : (1ull << rem) - 1ull) & lane_word(vl, vh, u);
+            r[u] = g * 64u + lane;
         }
-    for (uint32_t gb = gb0; gb * U < g_hi; gb += kNW) {
-        uint32_t r[U], am = 0;
-        int32_t key[NS][U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            r[u] = (gb * U + u) * 64u + lane;
-            am |= (r[u] < nsrc && gb * U + u < g_hi) ? 1u << u : 0u;
-        }
-#pragma unroll
-        for (int q = 0; q < NS; ++q)
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                key[q][u] = knext[q][u];
-                const uint32_t rn = ((gb + kNW) * U + u) * 64u + lane;
-                knext[q][u] = kf[q][rn < rmax ? rn : rmax];
-            }
         for (int f = 0; f < nf; ++f) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                int32_t v = 0;
-                const bool ok = ((am >> u) & 1u) && get_int32(pd.fcol[f], r[u], &v) &&
-                                v >= pd.flo[f] && v <= pd.fhi[f];
-                am &= ok ? ~0u : ~(1u << u);
+                bool ok = false;
+                if (in_mask(live[u])) {
+                    int32_t v = 0;
+                    ok = get_int32(pd.fcol[f], r[u], &v) && v >= pd.flo[f] && v <= pd.fhi[f];
+                }
+                live[u] &= __ballot(ok);
             }
         }
-        uint32_t brow[NS][U];  // key slot of the match, until the output
-#pragma unroll
-        for (int q = 0; q < NS; ++q)
-#pragma unroll
-            for (int u = 0; u < U; ++u) brow[q][u] = kDead;
         if (use_bf) {
-            // a strong LDS filter on the first join: rows that pass it and every validity test
-            // go to this wave's queue; full 64-lane batches of the queue then look up all joins
-            // at once (one 16-byte bucket load per join and live lane)
-#pragma unroll
-            for (int q = 0; q < NS; ++q) am = valid_mask<U>(am, kv[q], gb, g_hi, lane);
+            // a strong LDS filter on the first join: rows that pass it go to this wave's queue;
+            // full 64-lane batches of the queue then look up every join
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t h = fib32((uint32_t)key[0][u]);
                 const uint32_t m = bf_mask(h);
-                am &= (bfl[bf_word(h, lw)] & m) == m ? ~0u : ~(1u << u);
+                live[u] &= __ballot((bfl[bf_word(h, lw)] & m) == m);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint64_t bits = __ballot((am >> u) & 1u);
-                if (!bits) continue;
-                const uint32_t slot = qc + (uint32_t)__popcll(bits & lanemask_lt());
-                if ((am >> u) & 1u) {
+                if (!live[u]) continue;
+                if (in_mask(live[u])) {
+                    const uint32_t slot = qc + mask_rank(live[u]);
                     qbuf[w][0][slot] = r[u];
 #pragma unroll
                     for (int q = 0; q < NS; ++q) qbuf[w][1 + q][slot] = (uint32_t)key[q][u];
                 }
-                qc += (uint32_t)__popcll(bits);
+                qc += (uint32_t)__popcll(live[u]);
                 if (qc >= HJ_WAVE) {
                     queue_flush<NS>(qbuf[w], HJ_WAVE, lane, tkey, tk, tsp, &s_cnt, pd, st0);
                     queue_shift<NS>(qbuf[w], qc - HJ_WAVE, lane);
                     qc -= HJ_WAVE;
                 }
             }
-            continue;
-        } else {
-            // joins in order, each on the lanes the previous ones kept
+            return;
+        }
+        // joins in order, each on the lanes the previous ones kept
+        uint32_t brow[NS][U];  // key slot of the match, until the output
 #pragma unroll
-            for (int q = 0; q < NS; ++q) {
-                am = valid_mask<U>(am, kv[q], gb, g_hi, lane);
-                if (!__ballot(am != 0)) continue;  // uniform: every later join is skipped too
-                u32x4 v[U];
-                uint32_t hb[U];
+        for (int q = 0; q < NS; ++q) {
+            uint64_t any = 0;
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    hb[u] = ((am >> u) & 1u) ? home_bucket((uint32_t)key[q][u], tk[q]) : 0u;
-                    v[u] = load_bucket(tkey[q], hb[u]);
-                }
-                uint32_t cont = 0;
+            for (int u = 0; u < U; ++u) {
+                any |= live[u];
+                brow[q][u] = kDead;
+            }
+            if (any) {  // uniform
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    brow[q][u] = resolve_home(v[u], (uint32_t)key[q][u], hb[u], tk[q], tsp[q],
-                                              (am >> u) & 1u);
-                    cont |= brow[q][u] == kDead - 1u ? 1u << u : 0u;
+                    uint32_t sl = kDead;
+                    if (in_mask(live[u]))
+                        sl = table_lookup(tkey[q], tk[q], tsp[q], (uint32_t)key[q][u]);
+                    brow[q][u] = sl;
+                    live[u] &= __ballot(sl != kDead);
                 }
-                resolve_rest<U>(brow[q], cont, tkey[q], tk[q], hb, key[q]);
-#pragma unroll
-                for (int u = 0; u < U; ++u) am &= brow[q][u] != kDead ? ~0u : ~(1u << u);
             }
         }
-        if (!__ballot(am != 0)) continue;
+        uint64_t any = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) any |= live[u];
+        if (!any) return;
         uint32_t pos[U];
-        block_append_mask<U>(&s_cnt, am, pos);
+        block_append_masks<U>(&s_cnt, live, pos);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (!((am >> u) & 1u)) continue;
-            gp(pd.stage[0])[st0 + pos[u]] = r[u];
+            if (in_mask(live[u])) {
+                gp(pd.stage[0])[st0 + pos[u]] = r[u];
 #pragma unroll
-            for (int q = 0; q < NS; ++q)
-                gp(pd.stage[1 + q])[st0 + pos[u]] = tkey[q][row_word(brow[q][u])];
+                for (int q = 0; q < NS; ++q)
+                    gp(pd.stage[1 + q])[st0 + pos[u]] = tkey[q][row_word(brow[q][u])];
+            }
         }
+    };
+    uint32_t gb = g_lo / U + w;
+    fetch(ka, va, gb);
+    while (gb * U < g_hi) {
+        fetch(kb, vb, gb + kNW);
+        step(ka, va, gb);
+        gb += kNW;
+        if (gb * U >= g_hi) break;
+        fetch(ka, va, gb + kNW);
+        step(kb, vb, gb);
+        gb += kNW;
     }
     if (qc) queue_flush<NS>(qbuf[w], qc, lane, tkey, tk, tsp, &s_cnt, pd, st0);
     __syncthreads();
@@ -666,18 +684,25 @@ __global__ __launch_bounds__(kB) void k_pp_fwrite(const PipeDesc* __restrict__ p
 template <int NS>
 void launch_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
     constexpr int U = probe_unroll(NS);
-    hipLaunchKernelGGL((k_sp_probe<NS, U>), dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
+    if (hp.nf)
+        hipLaunchKernelGGL((k_sp_probe<NS, U, true>), dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
+    else
+        hipLaunchKernelGGL((k_sp_probe<NS, U, false>), dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
 }
 
 template <int NS>
 uint32_t resident_blocks(uint32_t num_cus) {
     static uint32_t per_cu = 0;
     if (!per_cu) {
-        int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sp_probe<NS, probe_unroll(NS)>, kB,
-                                                         0) != hipSuccess || n < 1)
-            n = 1;
-        per_cu = (uint32_t)n;
+        // the smaller occupancy of the two instantiations (with and without source filters)
+        int n0 = 0, n1 = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, k_sp_probe<NS, probe_unroll(NS), false>,
+                                                         kB, 0) != hipSuccess)
+            n0 = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, k_sp_probe<NS, probe_unroll(NS), true>,
+                                                         kB, 0) != hipSuccess)
+            n1 = 1;
+        per_cu = (uint32_t)std::max(1, std::min(n0, n1));
     }
     return per_cu * num_cus;
 }
