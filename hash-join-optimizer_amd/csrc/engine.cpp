@@ -36,9 +36,13 @@ namespace hj {
 // string bytes of an output column are allocated from rows x longest string (no host round trip)
 // up to this size; beyond it they are counted first
 constexpr uint64_t kCharsBound = 1ull << 30;
-// blocks per CU and build job of the pipeline's build kernels (block x of a job owns one range
-// of its rows, so the filter pass is a few iterations per block)
-constexpr uint32_t kBuildBlocksPerCU = 8;
+// counters of the pipeline (4 words) and of each of kMaxStages build jobs (4 words each)
+constexpr uint32_t kMetaWords = 4 * (kMaxStages + 1);
+// totals of the direct output: rows, string bytes per column, overflow
+constexpr uint32_t kEmitOut = 2 + kEmitCols;
+// blocks of the fused pass per block of the direct output
+constexpr uint32_t kEmitGroup = 8;
+static_assert(kMaxOutStr >= kEmitCols, "every output column may be a string column");
 
 namespace {
 
@@ -94,9 +98,15 @@ class Executor {
             throw Error(HJ_E_INVALID, "plan root out of range");
         depth_ = 0;
         if (ctx_->use_pipeline) {
-            Rel pr;
-            if (try_pipeline(pr)) return materialize(pr, plan_->nodes[plan_->root]);
-            A_.clear();  // not a chain, or a build side with duplicate keys: join at a time
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                Rel pr;
+                hj_dresult* dr = nullptr;
+                const Pipe st = try_pipeline(pr, &dr, attempt == 0);
+                if (st == Pipe::kResult) return dr;
+                if (st == Pipe::kRel) return materialize(pr, plan_->nodes[plan_->root]);
+                A_.clear();
+                if (st != Pipe::kRetry) break;  // not a chain, or a duplicate build key
+            }
         }
         Rel r = node(plan_->root);
         return materialize(r, plan_->nodes[plan_->root]);
@@ -393,12 +403,17 @@ class Executor {
         return k;
     }
 
-    bool try_pipeline(Rel& out) {
+    // outcome of the fused pipeline: not applicable (or a build side with a duplicate key: join
+    // at a time instead), a relation of output row ids, a materialized result, or a string total
+    // beyond the capacity of the direct output (run again with the materializer)
+    enum class Pipe { kNo, kRel, kResult, kRetry };
+
+    Pipe try_pipeline(Rel& out, hj_dresult** dres, bool allow_emit) {
         uint64_t source = 0;
         std::vector<uint64_t> joins;
         if (!chain(plan_->root, source, joins, 0) || joins.empty() ||
             joins.size() > (size_t)kMaxStages)
-            return false;
+            return Pipe::kNo;
         const Rel srel = scan(plan_->nodes[source]);
         std::vector<AttrSlot> cur;
         for (auto& a : srel.attrs) cur.push_back({0, a.col});
@@ -410,8 +425,9 @@ class Executor {
         std::memset(&hp, 0, sizeof hp);
         hb.njobs = (int32_t)joins.size();
         hp.nst = (int32_t)joins.size();
-        // one zeroed block for the pipeline's and every job's counters
-        uint32_t* metas = scratch<uint32_t>(4 * (joins.size() + 1));
+        // the pipeline's and every job's counters (cleared by the build's filter pass)
+        // pipeline and job counters, then the direct output's totals (all cleared by the build)
+        uint32_t* metas = scratch<uint32_t>(kMetaWords + 2 * kEmitOut);
         hp.meta = metas;
         for (size_t i = 0; i < joins.size(); ++i) {
             const hj_node& jn = plan_->nodes[joins[i]];
@@ -429,8 +445,8 @@ class Executor {
             const DevColumn& pcol = column(slot_table[pk.slot], pk.col);
             if (bcol.type != HJ_INT32 || pcol.type != HJ_INT32)
                 throw Error(HJ_E_TYPE, "Join key isn't of type int32_t");
-            if (i == 0 && pk.slot != 0) return false;
-            if (brel.n >= (1ull << 28)) return false;  // key slots of the table
+            if (i == 0 && pk.slot != 0) return Pipe::kNo;
+            if (brel.n >= (1ull << 28)) return Pipe::kNo;  // key slots of the table
 
             BuildJob& jb = hb.job[i];
             jb.key = bcol.view();
@@ -444,11 +460,17 @@ class Executor {
             jb.rows = rows;
             jb.kmax = table_bits(rows);
             jb.meta = metas + 4 * (i + 1);
-            const uint64_t bgrid = (uint64_t)ctx_->num_cus * kBuildBlocksPerCU;
-            const uint64_t scap = std::max<uint64_t>(1, build_range(rows, (uint32_t)bgrid) * bgrid);
+            jb.nblk = std::max<uint32_t>(1, (rows + kBuildRows - 1) / kBuildRows);
+            jb.blk0 = hb.grid;
+            hb.grid += jb.nblk;
+            const uint64_t scap = (uint64_t)jb.nblk * kBuildRows;
             jb.st_key = scratch<uint32_t>(scap);
             jb.st_row = scratch<uint32_t>(scap);
-            jb.blk = scratch<uint32_t>(bgrid);
+            jb.blk = scratch<uint32_t>(jb.nblk);
+            // clear pass: 16-byte stores, 4 per thread and round, over the largest table
+            const uint64_t stores = (1ull << jb.kmax) / 2;
+            hb.cgrid = (uint32_t)std::max<uint64_t>(
+                hb.cgrid, std::min<uint64_t>(1024, (stores + 4 * 256 - 1) / (4 * 256)));
             jb.table = scratch<uint32_t>(2ull * (1ull << jb.kmax) + 8);
             jb.bf = scratch<uint32_t>(kBloomWords);
 
@@ -471,7 +493,9 @@ class Executor {
             }
             cur.swap(nxt);
         }
-        hb.grid = (uint32_t)ctx_->num_cus * kBuildBlocksPerCU;
+        for (int j = 0; j < kMaxStages; ++j) hb.blk0[j] = j < hb.njobs ? hb.job[j].blk0 : ~0u;
+        hb.zero = metas;  // cleared by the filter pass
+        hb.nzero = kMetaWords + 2 * kEmitOut;
 
         hp.nf = (int32_t)srel.filters.size();
         for (size_t f = 0; f < srel.filters.size(); ++f) {
@@ -495,15 +519,10 @@ class Executor {
         }
         const uint64_t nsrc = srel.n;
         // the fused pass addresses the source's key columns with 32-bit byte offsets
-        if (nsrc >= (1ull << 29)) return false;
+        if (nsrc >= (1ull << 29)) return Pipe::kNo;
         hp.src_rows = (uint32_t)nsrc;
         hp.nout = (int32_t)oslots.size();
         const uint64_t ncap = nsrc ? nsrc : 1;
-        for (size_t q = 0; q < oslots.size(); ++q) {
-            hp.out_slot[q] = oslots[q];
-            hp.final_out[q] = result<uint32_t>(ncap);
-            res.slots.push_back({slot_table[oslots[q]], hp.final_out[q]});
-        }
         // joins keyed by the source run fused, smallest build table first (its Bloom filter
         // is the one held in LDS); the others run afterwards over the survivors
         for (int st = 0; st < hp.nst; ++st) {
@@ -516,7 +535,36 @@ class Executor {
         std::stable_sort(hp.order, hp.order + hp.nstream, [&](int a, int b) {
             return hb.job[hp.st[a].job].rows < hb.job[hp.st[b].job].rows;
         });
-        for (int q = 0; q < hp.nstream; ++q) hp.qpos[hp.order[q]] = q;
+        for (int q = 0; q < hp.nstream; ++q) {
+            const int st = hp.order[q];
+            const BuildJob& jb = hb.job[hp.st[st].job];
+            hp.qpos[st] = q;
+            hp.sj[q] = {hp.st[st].keycol.flat, hp.st[st].keycol.vbits, jb.table, jb.meta, jb.bf};
+        }
+        // output: written directly from the fused pass's staged survivors when every join is
+        // fused (at most one output row per source row: cap rows), else compacted row ids
+        const hj_node& root = plan_->nodes[plan_->root];
+        bool emit = allow_emit && hp.ndep == 0 && root.num_output_attrs == cur.size() &&
+                    cur.size() <= (size_t)kEmitCols;
+        if (emit) {
+            // the direct output's buffers at capacity must leave the arena room for the pass
+            uint64_t need = 0;
+            for (const AttrSlot& a : cur) {
+                const DevColumn& dc = column(slot_table[a.slot], a.col);
+                need += ncap + ncap / 8 + 1024;
+                need += dc.type == HJ_VARCHAR
+                            ? 8 * (ncap + 1) + std::min<uint64_t>(ncap * (uint64_t)dc.max_len,
+                                                                  kCharsBound)
+                            : 4 * ncap;
+            }
+            emit = need <= A_.free_bytes() / 2;
+        }
+        if (!emit)
+            for (size_t q = 0; q < oslots.size(); ++q) {
+                hp.out_slot[q] = oslots[q];
+                hp.final_out[q] = result<uint32_t>(ncap);
+                res.slots.push_back({slot_table[oslots[q]], hp.final_out[q]});
+            }
         {
             // fused pass: contiguous ranges of 64-row groups per block
             const int U = probe_unroll(hp.nstream);
@@ -531,6 +579,18 @@ class Executor {
             for (int q = 0; q <= hp.nstream; ++q) hp.stage[q] = scratch<uint32_t>(cap);
             hp.blk_cnt = scratch<uint32_t>(hp.probe_blocks);
             hp.blk_off = scratch<uint32_t>(hp.probe_blocks);
+        }
+        if (emit) {
+            // output VARCHAR columns: the fused pass counts their bytes per block
+            for (size_t i = 0; i < cur.size(); ++i) {
+                const int ps = cur[i].slot;
+                const DevColumn& dc = column(slot_table[ps], cur[i].col);
+                if (dc.type != HJ_VARCHAR) continue;
+                hp.ostr[hp.nostr] = dc.view();
+                hp.ostr_slot[hp.nostr] = ps ? 1 + hp.qpos[ps - 1] : 0;
+                ++hp.nostr;
+            }
+            hp.blk_bytes = scratch<uint64_t>((uint64_t)hp.probe_blocks * kMaxOutStr);
         }
         if (hp.ndep) {
             hp.l_row = scratch<uint32_t>(ncap);
@@ -550,11 +610,14 @@ class Executor {
         std::memcpy(pin, &hb, bb_bytes);
         std::memcpy(pin + pd_off, &hp, sizeof(PipeDesc));
         HJ_HIP(hipMemcpyAsync(ddesc, pin, pd_off + sizeof(PipeDesc), hipMemcpyHostToDevice, s_));
-        zero_u32(metas, 4 * (joins.size() + 1), s_);
         const BuildBatch* dbb = (const BuildBatch*)ddesc;
         const PipeDesc* dpd = (const PipeDesc*)(ddesc + pd_off);
         { Prof p_(ctx_, "pipe_build"); pipe_build(hb, dbb, s_); }
-        { Prof p_(ctx_, "probe_fused"); pipe_probe_fused(hp, dpd, s_); }
+        {
+            Prof p_(ctx_, "probe_fused", true);
+            pipe_probe_fused(hp, dpd, s_, p_.a, p_.b);
+        }
+        if (emit) return emit_result(hp, oslots, slot_table, cur, metas, dres);
         { Prof p_(ctx_, "probe_tail"); pipe_probe_tail(hp, dpd, s_); }
         struct F4 {
             uint32_t v[4];
@@ -569,10 +632,87 @@ class Executor {
             std::fprintf(stderr, "[pipe] fallback %u survivors %u out %u\n", fl.v[0], fl.v[2],
                          fl.v[3]);
         }
-        if (fl.v[0]) return false;
+        if (fl.v[0]) return Pipe::kNo;
         res.n = fl.v[3];
         out = std::move(res);
-        return true;
+        return Pipe::kRel;
+    }
+
+    // the root's columns straight from the fused pass (emit_rows), then one round trip for the
+    // counters: fallback flag, rows, string totals, overflow
+    Pipe emit_result(const PipeDesc& hp, const std::vector<int>& oslots,
+                     const std::vector<uint32_t>& slot_table, const std::vector<AttrSlot>& cur,
+                     uint32_t* metas, hj_dresult** dres) {
+        const hj_node& root = plan_->nodes[plan_->root];
+        const uint64_t cap = hp.src_rows ? hp.src_rows : 1;
+        const uint64_t vbytes = ((cap + 63) / 64) * 8;
+        std::unique_ptr<hj_dresult> res(new hj_dresult());
+        EmitDesc ed;
+        std::memset(&ed, 0, sizeof ed);
+        ed.ncol = (int32_t)cur.size();
+        int nstr = 0;  // VARCHAR columns in output order: the fused pass's byte count entries
+        for (size_t i = 0; i < cur.size(); ++i) {
+            const int ps = cur[i].slot;
+            const DevColumn& dc = column(slot_table[ps], cur[i].col);
+            if (root.output_attrs[i].type != dc.type)
+                throw Error(HJ_E_TYPE, "output attribute type does not match its column");
+            if (dc.type != HJ_INT32 && dc.type != HJ_VARCHAR)
+                throw Error(HJ_E_TYPE, "unsupported output column type (only INT32 and VARCHAR)");
+            EmitCol& ec = ed.c[i];
+            ec.col = dc.view();
+            ec.slot = ps ? 1 + hp.qpos[ps - 1] : 0;
+            ec.type = dc.type;
+            if (dc.type == HJ_VARCHAR) ec.sidx = nstr++;
+            ec.valid8 = scratch<uint8_t>(cap);
+            hj_dresult::Col c;
+            c.type = dc.type;
+            c.validity = result<uint8_t>(vbytes);
+            ec.validity = c.validity;
+            if (dc.type == HJ_INT32) {
+                c.values = result<int32_t>(cap);
+                ec.values = c.values;
+            } else {
+                c.offsets = result<uint64_t>(cap + 1);
+                ec.offsets = c.offsets;
+                ec.chars_cap = std::min<uint64_t>(cap * (uint64_t)dc.max_len, kCharsBound);
+                c.chars = result<char>(ec.chars_cap ? ec.chars_cap : 1);
+                ec.chars = c.chars;
+            }
+            res->cols.push_back(c);
+        }
+        (void)oslots;
+        ed.nstr = nstr;
+        ed.blocks = hp.probe_blocks;
+        ed.rows_per_block = hp.probe_gpb * 64;
+        ed.blk_cnt = hp.blk_cnt;
+        for (int q = 0; q <= hp.nstream; ++q) ed.stage[q] = hp.stage[q];
+        ed.meta = hp.meta;
+        ed.cap_rows = cap;
+        ed.group = kEmitGroup;
+        ed.blk_bytes = hp.blk_bytes;
+        ed.bytes_stride = kMaxOutStr;
+        ed.out = (uint64_t*)(metas + kMetaWords);
+        { Prof p_(ctx_, "emit"); emit_rows(ed, s_); }
+        // one round trip: pipeline counters and the direct output's totals
+        uint8_t* pin = (uint8_t*)ctx_->pinned;
+        const size_t bytes = kMetaWords * 4 + kEmitOut * 8;
+        HJ_HIP(hipMemcpyAsync(pin, metas, bytes, hipMemcpyDeviceToHost, s_));
+        HJ_HIP(hipStreamSynchronize(s_));
+        uint32_t m[kMetaWords];
+        uint64_t o[kEmitOut];
+        std::memcpy(m, pin, sizeof m);
+        std::memcpy(o, pin + kMetaWords * 4, sizeof o);
+        if (std::getenv("HJ_DEBUG_PIPE"))
+            std::fprintf(stderr, "[pipe] fallback %u rows %llu overflow %llu\n", m[0],
+                         (unsigned long long)o[0], (unsigned long long)o[1 + kEmitCols]);
+        if (m[0]) return Pipe::kNo;
+        if (o[1 + kEmitCols]) return Pipe::kRetry;
+        res->num_rows = o[0];
+        for (size_t i = 0; i < cur.size(); ++i)
+            if (res->cols[i].type == HJ_VARCHAR) res->cols[i].num_chars = o[1 + i];
+        res->live = true;
+        *dres = res.release();
+        return Pipe::kResult;
     }
 
     Rel empty_like(Rel out, const std::vector<std::pair<int, uint32_t>>& used, const Rel& L,

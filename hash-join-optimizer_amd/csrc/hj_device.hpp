@@ -138,6 +138,17 @@ __device__ inline bool value_index(const DCol& c, uint32_t p, const HJ_GLOBAL ui
     return true;
 }
 
+// total bytes of the long string starting at its 0xffff page p
+__device__ inline uint64_t long_length(const DCol& c, uint32_t p) {
+    uint64_t total = *(const HJ_GLOBAL uint32_t*)page_ptr(c, p) >> 16;
+    for (uint32_t q = p + 1; q < c.num_pages; ++q) {
+        const uint32_t h = *(const HJ_GLOBAL uint32_t*)page_ptr(c, q);
+        if ((h & 0xffffu) != 0xfffeu) break;
+        total += h >> 16;
+    }
+    return total;
+}
+
 // validity of row r of an INT32 column's row-space mirror
 __device__ inline bool flat_valid(const DCol& c, uint32_t r) {
     return !c.vbits || ((gp(c.vbits)[r >> 6] >> (r & 63u)) & 1ull);

@@ -158,10 +158,13 @@ struct hj_context {
 };
 
 namespace hj {
-// brackets the launches issued during its lifetime with events when the name is profiled
+// brackets the launches issued during its lifetime with events when the name is profiled. With
+// embed, the scope holds one launch that records the events itself (hipExtLaunchKernelGGL with
+// a and b), so that timing adds no packets of its own to the stream.
 struct Prof {
     hj_context* ctx;
     const char* name;
+    bool embed;
     hipEvent_t a = nullptr, b = nullptr;
     static hipEvent_t get(hj_context* c) {
         if (!c->event_pool.empty()) {
@@ -173,16 +176,20 @@ struct Prof {
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
         return e;
     }
-    Prof(hj_context* c, const char* n) : ctx(c), name(n) {
+    Prof(hj_context* c, const char* n, bool embed_events = false) : ctx(c), name(n), embed(embed_events) {
         if (!ctx->prof_on || std::strncmp(n, ctx->prof_prefix.c_str(), ctx->prof_prefix.size()))
             return;
         a = get(ctx);
         b = get(ctx);
-        if (a) (void)hipEventRecord(a, ctx->stream);
+        if (!a || !b) {
+            a = b = nullptr;
+            return;
+        }
+        if (!embed) (void)hipEventRecord(a, ctx->stream);
     }
     ~Prof() {
         if (!a || !b) return;
-        (void)hipEventRecord(b, ctx->stream);
+        if (!embed) (void)hipEventRecord(b, ctx->stream);
         ctx->prof.push_back({name, a, b});
     }
 };

@@ -17,6 +17,7 @@
 namespace hj {
 
 constexpr int kMatCols = 8;
+constexpr int kMaxStageArrays = 9;  // stage arrays of the fused probe pass (source + 8 joins)
 constexpr uint32_t kMatTile = 256;  // rows per block: one per thread
 
 struct MatCol {
@@ -38,6 +39,50 @@ struct MatBatch {
     uint32_t tiles;
     uint64_t n;
 };
+
+// Output of the fused probe pass (joins keyed by the source only) straight from its staged
+// survivors: block b of the pass staged blk_cnt[b] rows (each output slot's row ids) from row
+// b * rows_per_block of the stage arrays and counted their string bytes per output VARCHAR
+// column (blk_bytes). Two launches, no host round trip:
+//   emit       per `group` blocks of the pass: output row and string byte offsets from the
+//              counts of the blocks before them, then INT32 values, string offsets and bytes,
+//              validity bytes
+//   emit bits  validity bitmaps from the validity bytes; totals are in `out`
+// Output buffers hold cap_rows rows (an upper bound of the row count, known on the host), each
+// chars buffer chars_cap bytes; a larger string total sets the overflow word instead of writing.
+constexpr int kEmitCols = 16;
+
+struct EmitCol {
+    DCol col;
+    int32_t slot;         // stage array of the row ids
+    int32_t type;         // HJ_INT32 (0) or HJ_VARCHAR (3)
+    int32_t sidx;         // VARCHAR: its entry in the pass's per-block string byte counts
+    int32_t* values;      // INT32
+    uint64_t* offsets;    // VARCHAR, cap_rows + 1
+    char* chars;          // VARCHAR, chars_cap bytes
+    uint64_t chars_cap;
+    uint8_t* valid8;      // scratch: validity byte per output row
+    uint8_t* validity;    // bitmap, 64-bit words
+};
+
+struct EmitDesc {
+    EmitCol c[kEmitCols];
+    int32_t ncol;
+    int32_t nstr;                // VARCHAR columns (blk_bytes entries 0 .. nstr-1)
+    uint32_t blocks;             // blocks of the fused pass ...
+    uint32_t group;              // ... per block of the emit pass (<= 16)
+    uint32_t rows_per_block;     // stage rows per block of the fused pass
+    uint32_t bytes_stride;       // entries per block in blk_bytes
+    uint64_t cap_rows;
+    const uint32_t* blk_cnt;     // survivors per block of the fused pass
+    const uint64_t* blk_bytes;   // string bytes per block of the fused pass and VARCHAR column
+    const uint32_t* stage[kMaxStageArrays];
+    const uint32_t* meta;        // pipeline counters: [0] != 0 voids the pass (nothing written)
+    uint64_t* out;               // [0] rows, [1 + c] string bytes of column c, [1 + kEmitCols]
+                                 // overflow of a chars buffer (zeroed)
+};
+
+void emit_rows(const EmitDesc& d, hipStream_t s);
 
 // passes a and b; pass c separately, once chars buffers exist
 void mat_gather(const MatBatch& b, hipStream_t s);

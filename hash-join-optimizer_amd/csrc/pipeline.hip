@@ -1,11 +1,12 @@
 // pipeline.hip - builds and the fused probe of the join pipeline (gfx950, wave64).
 //
-// Build, all build sides in the same launches (blockIdx.y = job):
-//   filter  rows passing the scan's filters with a non-NULL key -> staged (key, row), appended
-//           through a wave-aggregated cursor                        reads key + filter columns
-//   clear   the 2^k table entries (2^k >= 2n for n staged tuples) and the Bloom words
-//   insert  linear probing with a 64-bit compare-and-swap of row << 32 | key; a key found twice
-//           flags the build (the caller then falls back to the general join path)
+// Build, all build sides in the same three launches:
+//   filter  rows passing the scan's filters with a non-NULL key -> staged (key, row) at the
+//           start of the block's row range                          reads key + filter columns
+//   clear   the job's tuple count, then its 2^k key slots (2^k >= 4n) and Bloom words
+//   insert  a 32-bit compare-and-swap of the key into a slot of its home bucket (then the other
+//           slots, then the next buckets), the row next to it; a key found twice flags the
+//           build (the caller then falls back to the general join path)
 // Probe: one fused pass over the source rows (a wave takes U consecutive 64-row groups, one row
 // per lane). The keys of every streamed join are loaded first (coalesced, row-space mirror),
 // then each join in turn: validity, the LDS Bloom filter of the first (smallest) build, and
@@ -14,6 +15,8 @@
 // Joins keyed by a column of a build table run afterwards over the survivor list, marking
 // failures, and an ordered compaction writes the output row ids.
 #include "pipeline.hpp"
+
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 
@@ -83,8 +86,8 @@ __device__ inline u32x4 load_bucket(const HJ_GLOBAL uint32_t* K, uint32_t b) {
 }
 
 // slot of `key` (!= kEmptyKey), kDead when absent, given the keys v of its home bucket b;
-// occupied slots form a prefix of the probe sequence, so an empty slot ends the search, and at
-// load factor <= 1/8 the home bucket nearly always does
+// a key leaves its home bucket only when that bucket is full, so a bucket with a free slot ends
+// the search, and at load factor <= 1/4 the home bucket nearly always does
 __device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t bmask, uint32_t b,
                                           uint32_t key, u32x4 v) {
     for (;;) {
@@ -99,143 +102,14 @@ __device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t 
     }
 }
 
-// ---------------------------------------------------------------- build
-constexpr int kBU = kBuildTile / kB;  // rows per thread per iteration of the filter pass
-
-// block-local append of U predicates per lane through an LDS counter (one LDS atomic per wave)
-template <int U>
-__device__ inline void block_append(uint32_t* lds_cnt, const bool (&pred)[U], uint32_t (&pos)[U]) {
-    uint64_t b[U];
-    uint32_t tot = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        b[u] = __ballot(pred[u]);
-        tot += (uint32_t)__popcll(b[u]);
-    }
-    if (!tot) return;
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(lds_cnt, tot);
-    base = __shfl(base, 0, HJ_WAVE);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        pos[u] = base + (uint32_t)__popcll(b[u] & lanemask_lt());
-        base += (uint32_t)__popcll(b[u]);
-    }
-}
-
-// filter pass: block x of job y takes rows [x * range, (x + 1) * range) and stages the tuples
-// that pass at the start of that range
-__global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__ bb) {
-    __shared__ uint32_t s_cnt;
-    const BuildJob& jb = bb->job[blockIdx.y];
-    const uint32_t rows = jb.rows;
-    const uint32_t range = build_range(rows, gridDim.x);
-    const uint32_t r0 = blockIdx.x * range;
-    const uint32_t r1 = r0 + range < rows ? r0 + range : rows;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    for (uint32_t base = r0; base < r1; base += kB * kBU) {
-        bool ok[kBU];
-        int32_t key[kBU];
-        uint32_t r[kBU], pos[kBU];
-#pragma unroll
-        for (int u = 0; u < kBU; ++u) {
-            r[u] = base + u * kB + threadIdx.x;
-            key[u] = 0;
-            ok[u] = r[u] < r1 && get_int32(jb.key, r[u], &key[u]);
-        }
-        for (int f = 0; f < jb.nf; ++f) {
-#pragma unroll
-            for (int u = 0; u < kBU; ++u) {
-                int32_t v = 0;
-                ok[u] = ok[u] && get_int32(jb.fcol[f], r[u], &v) && v >= jb.flo[f] && v <= jb.fhi[f];
-            }
-        }
-        block_append<kBU>(&s_cnt, ok, pos);
-#pragma unroll
-        for (int u = 0; u < kBU; ++u) {
-            if (ok[u]) {
-                gp(jb.st_key)[r0 + pos[u]] = (uint32_t)key[u];
-                gp(jb.st_row)[r0 + pos[u]] = r[u];
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) gp(jb.blk)[blockIdx.x] = s_cnt;
-}
-
-// tuple count of each job = sum of its filter blocks' counts (one block per job)
-__global__ __launch_bounds__(1024) void k_sb_count(const BuildBatch* __restrict__ bb) {
-    __shared__ uint32_t lds[1024 / HJ_WAVE];
-    const BuildJob& jb = bb->job[blockIdx.x];
-    uint32_t c = 0;
-    for (uint32_t i = threadIdx.x; i < bb->grid; i += 1024) c += gp(jb.blk)[i];
-    const uint32_t n = block_sum<1024>(c, lds);
-    if (threadIdx.x == 0) gp(jb.meta)[0] = n;
-}
-
-__global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ bb) {
-    const BuildJob& jb = bb->job[blockIdx.y];
-    const uint32_t n = gp(jb.meta)[0];
-    const uint32_t nbk = (1u << table_bits(n)) / kBucketKeys;
-    const uint32_t g = blockIdx.x * kB + threadIdx.x, stride = gridDim.x * kB;
-    HJ_GLOBAL u32x4* T = (HJ_GLOBAL u32x4*)gp(jb.table);
-    const u32x4 empty = {kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
-    for (uint32_t i = g; i < 2u * nbk; i += stride) T[i] = empty;  // whole buckets: full lines
-    if (bf_enabled(n)) {
-        const uint32_t W = 1u << bf_log2w(n);
-        for (uint32_t i = g; i < W; i += stride) gp(jb.bf)[i] = 0;
-    }
-}
-
-// insert: block x of job y inserts the tuples staged by block x of the filter pass
-__global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__ bb) {
-    const BuildJob& jb = bb->job[blockIdx.y];
-    const uint32_t n = gp(jb.meta)[0];
-    const uint32_t k = table_bits(n), bmask = (1u << (k - 2)) - 1u;
-    const bool use_bf = bf_enabled(n);
-    const uint32_t lw = bf_log2w(n);
-    const uint32_t r0 = blockIdx.x * build_range(jb.rows, gridDim.x);
-    const uint32_t cnt = gp(jb.blk)[blockIdx.x];
-    HJ_GLOBAL uint32_t* K = gp(jb.table);
-    for (uint32_t i = threadIdx.x; i < cnt; i += kB) {
-        const uint32_t key = gp(jb.st_key)[r0 + i], row = gp(jb.st_row)[r0 + i];
-        if (key == kEmptyKey) {  // kept aside in slot 2^k
-            if (atomicCAS((uint32_t*)&gp(jb.meta)[2], 0u, 1u) != 0u)
-                gp(jb.meta)[1] = 1;
-            else
-                K[row_word(1u << k)] = row;
-        } else {
-            bool done = false;
-            for (uint32_t b = home_bucket(key, k); !done; b = (b + 1u) & bmask) {
-                for (uint32_t j = 0; j < kBucketKeys && !done; ++j) {
-                    const uint32_t s = 4u * b + j;
-                    const uint32_t old = atomicCAS((uint32_t*)&K[8u * b + j], kEmptyKey, key);
-                    if (old == kEmptyKey) {
-                        K[row_word(s)] = row;
-                        done = true;
-                    } else if (old == key) {
-                        gp(jb.meta)[1] = 1;
-                        done = true;
-                    }
-                }
-            }
-        }
-        if (use_bf) {
-            const uint32_t h = fib32(key);
-            atomicOr((uint32_t*)&gp(jb.bf)[bf_word(h, lw)], bf_mask(h));
-        }
-    }
-}
-
-// ---------------------------------------------------------------- fused probe
-// The streaming loop keeps liveness as 64-bit wave masks (one per 64-row group, in scalar
-// registers): validity words are lane masks already, the LDS filter's result is a ballot, and
-// the lanes of a mask run a step with exactly that execution mask (inverse ballot). Source
-// columns are read with buffer loads: one per-lane offset serves every column and group of an
-// iteration (the group is the instruction's immediate offset), and rows past the end of a
-// column read 0 instead of faulting (the loads of the last groups need no clamp).
+// ---------------------------------------------------------------- wave masks, buffer loads
+// Liveness is kept as 64-bit wave masks (one per 64-row group, in scalar registers): validity
+// words are lane masks already, a predicate becomes one with a ballot, and the lanes of a mask
+// run a step with exactly that execution mask (inverse ballot). Columns are read with buffer
+// loads: one per-lane offset serves every column and group of an iteration (the group is the
+// instruction's immediate offset), and rows past the end of a column read 0 instead of faulting.
 constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, 32-bit data format (gfx9 family)
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
@@ -255,6 +129,224 @@ __device__ inline uint64_t lane_word(uint32_t lo, uint32_t hi, int u) {
            (uint32_t)__builtin_amdgcn_readlane((int)lo, u);
 }
 
+// lanes [0, rem) of a group (rem clamped to 64)
+__device__ inline uint64_t range_mask(uint32_t rem) { return rem >= 64u ? ~0ull : (1ull << rem) - 1ull; }
+
+// block-local append of the lanes of U (uniform) masks through an LDS counter: pos[u] is the
+// lane's position among this block's appends for mask u (valid on the lanes of m[u])
+template <int U>
+__device__ inline void block_append_masks(uint32_t* lds_cnt, const uint64_t (&m)[U],
+                                          uint32_t (&pos)[U]) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) tot += (uint32_t)__popcll(m[u]);
+    uint32_t base = 0;
+    if (lane_id() == 0 && tot) base = atomicAdd(lds_cnt, tot);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        pos[u] = base + mask_rank(m[u]);
+        base += (uint32_t)__popcll(m[u]);
+    }
+}
+
+// ---------------------------------------------------------------- build
+// The filter and insert passes run one 1-D grid over all jobs: job j owns blocks
+// [blk0, blk0 + nblk), block x of a job takes its rows [x, x + 1) * kBuildRows and stages the
+// tuples that pass at the start of that range.
+constexpr int kFU = kBuildRows / (kB * 2);  // 64-row groups per wave and iteration (2 rounds)
+static_assert(kFU * kB * 2 == (int)kBuildRows, "build rows per block");
+constexpr int kIU = 4;                       // tuples per thread and round of the insert pass
+
+// job of block b: the number of jobs after the first whose blocks start at or before b (the
+// first blocks of all jobs are one contiguous array, unused entries ~0)
+__device__ inline int build_job_of(const BuildBatch* bb, uint32_t b) {
+    int j = 0;
+#pragma unroll
+    for (int i = 1; i < kMaxStages; ++i) j += b >= bb->blk0[i] ? 1 : 0;
+    return j;
+}
+
+// filter pass: rows with a non-NULL key that pass the scan's range filters -> (key, row)
+__global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__ bb) {
+    __shared__ uint32_t s_cnt;
+    const uint32_t t = threadIdx.x, lane = lane_id();
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
+    if (blockIdx.x == 0 && t < bb->nzero) gp(bb->zero)[t] = 0;  // job and pipeline metas
+    const BuildJob& jb = bb->job[build_job_of(bb, blockIdx.x)];
+    const uint32_t x = blockIdx.x - jb.blk0;
+    const uint32_t rows = jb.rows, G = (rows + 63u) >> 6;
+    const uint32_t r0 = x * kBuildRows;
+    const uint32_t r1 = r0 + kBuildRows < rows ? r0 + kBuildRows : rows;
+    if (t == 0) s_cnt = 0;
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(jb.key.flat, rows * 4u);
+    const __amdgpu_buffer_rsrc_t rkv =
+        make_rsrc(jb.key.vbits ? (const void*)jb.key.vbits : (const void*)jb.key.flat,
+                  jb.key.vbits ? G * 8u : 0u);
+    const uint32_t kv_none = jb.key.vbits ? 0u : ~0u;
+    const int nf = jb.nf;
+    // lane u < kFU reads the validity word of group u of the wave's rows
+    const uint32_t lanev = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 32u;
+    for (uint32_t base = r0; base < r1; base += kB * kFU) {
+        // wave w takes groups base/64 + 4u + w, u < kFU
+        const uint32_t vo = (base + w * 64u + lane) * 4u;
+        const uint32_t vv = (base / 64u + w) * 8u + lanev;
+        int32_t key[kFU];
+#pragma unroll
+        for (int u = 0; u < kFU; ++u)
+            key[u] = __builtin_amdgcn_raw_buffer_load_b32(rk, vo + u * 1024u, 0, 0);
+        const u32x2 kvw = __builtin_amdgcn_raw_buffer_load_b64(rkv, vv, 0, 0);
+        uint64_t live[kFU];
+#pragma unroll
+        for (int u = 0; u < kFU; ++u) {
+            const uint32_t g0 = base + u * 256u + w * 64u;
+            live[u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) &
+                      lane_word(kvw.x | kv_none, kvw.y | kv_none, u);
+        }
+        for (int f = 0; f < nf; ++f) {
+            const DCol& fc = jb.fcol[f];
+            const __amdgpu_buffer_rsrc_t rf = make_rsrc(fc.flat, rows * 4u);
+            const __amdgpu_buffer_rsrc_t rfv =
+                make_rsrc(fc.vbits ? (const void*)fc.vbits : (const void*)fc.flat,
+                          fc.vbits ? G * 8u : 0u);
+            const uint32_t fv_none = fc.vbits ? 0u : ~0u;
+            const int32_t lo = jb.flo[f], hi = jb.fhi[f];
+            int32_t v[kFU];
+#pragma unroll
+            for (int u = 0; u < kFU; ++u)
+                v[u] = __builtin_amdgcn_raw_buffer_load_b32(rf, vo + u * 1024u, 0, 0);
+            const u32x2 fvw = __builtin_amdgcn_raw_buffer_load_b64(rfv, vv, 0, 0);
+#pragma unroll
+            for (int u = 0; u < kFU; ++u)
+                live[u] &= __ballot(v[u] >= lo && v[u] <= hi) &
+                           lane_word(fvw.x | fv_none, fvw.y | fv_none, u);
+        }
+        uint32_t pos[kFU];
+        block_append_masks<kFU>(&s_cnt, live, pos);
+#pragma unroll
+        for (int u = 0; u < kFU; ++u) {
+            if (in_mask(live[u])) {
+                gp(jb.st_key)[r0 + pos[u]] = (uint32_t)key[u];
+                gp(jb.st_row)[r0 + pos[u]] = base + u * 256u + w * 64u + lane;
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) gp(jb.blk)[x] = s_cnt;
+}
+
+// tuple count of a job (sum of its filter blocks' counts; every block of the clear pass
+// computes it, block 0 records it), then the table's 2^k key slots and the Bloom words
+__global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ bb) {
+    const BuildJob& jb = bb->job[blockIdx.y];
+    __shared__ uint32_t s_part[kNW];
+    {
+        uint32_t c = 0;
+        for (uint32_t i = threadIdx.x; i < jb.nblk; i += kB) c += gp(jb.blk)[i];
+        c = wave_sum(c);
+        if (lane_id() == 0) s_part[threadIdx.x / HJ_WAVE] = c;
+    }
+    __syncthreads();
+    uint32_t n = 0;
+#pragma unroll
+    for (int i = 0; i < (int)kNW; ++i) n += s_part[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) gp(jb.meta)[0] = n;
+    const uint32_t nbk = (1u << table_bits(n)) / kBucketKeys;
+    const uint32_t g = blockIdx.x * kB + threadIdx.x, stride = gridDim.x * kB;
+    HJ_GLOBAL u32x4* T = (HJ_GLOBAL u32x4*)gp(jb.table);
+    const u32x4 empty = {kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey};
+    for (uint32_t i = g; i < 2u * nbk; i += stride) T[i] = empty;  // whole buckets: full lines
+    if (bf_enabled(n)) {
+        const uint32_t W = 1u << bf_log2w(n);
+        for (uint32_t i = g; i < W; i += stride) gp(jb.bf)[i] = 0;
+    }
+}
+
+// one compare-and-swap of an insert at key slot sl: true once the key is placed (or found
+// there: a duplicate build key, flagged in meta[1])
+__device__ inline bool insert_at(HJ_GLOBAL uint32_t* K, uint32_t sl, uint32_t key, uint32_t row,
+                                 HJ_GLOBAL uint32_t* meta) {
+    const uint32_t old = atomicCAS((uint32_t*)&K[8u * (sl >> 2) + (sl & 3u)], kEmptyKey, key);
+    if (old == kEmptyKey) {
+        K[row_word(sl)] = row;
+        return true;
+    }
+    if (old == key) {
+        meta[1] = 1;
+        return true;
+    }
+    return false;
+}
+
+// the rest of an insert whose first compare-and-swap (slot s of its home bucket) found another
+// key: the other slots of the home bucket, then the following buckets whole (the table always
+// has free slots: load factor <= 1/4)
+__device__ inline void insert_rest(HJ_GLOBAL uint32_t* K, uint32_t bmask, uint32_t s,
+                                   uint32_t key, uint32_t row, HJ_GLOBAL uint32_t* meta) {
+    const uint32_t b0 = s >> 2;
+    for (uint32_t i = 1; i < kBucketKeys; ++i)
+        if (insert_at(K, 4u * b0 + ((s + i) & 3u), key, row, meta)) return;
+    for (uint32_t b = (b0 + 1u) & bmask;; b = (b + 1u) & bmask)
+        for (uint32_t j = 0; j < kBucketKeys; ++j)
+            if (insert_at(K, 4u * b + j, key, row, meta)) return;
+}
+
+// insert: block x of a job inserts the tuples staged by block x of the filter pass. A key's
+// home slot is the top k bits of its Fibonacci hash (bucket = the top k - 2, then the slot in
+// the bucket); kIU independent compare-and-swaps per thread are in flight at once.
+__global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__ bb) {
+    const BuildJob& jb = bb->job[build_job_of(bb, blockIdx.x)];
+    const uint32_t x = blockIdx.x - jb.blk0;
+    HJ_GLOBAL uint32_t* meta = gp(jb.meta);
+    const uint32_t n = meta[0];
+    const uint32_t k = table_bits(n), bmask = (1u << (k - 2)) - 1u;
+    const bool use_bf = bf_enabled(n);
+    const uint32_t lw = bf_log2w(n);
+    const uint32_t r0 = x * kBuildRows;
+    const uint32_t cnt = gp(jb.blk)[x];
+    HJ_GLOBAL uint32_t* K = gp(jb.table);
+    for (uint32_t i0 = 0; i0 < cnt; i0 += kB * kIU) {
+        uint32_t key[kIU], row[kIU], s[kIU], old[kIU];
+        bool ok[kIU];
+#pragma unroll
+        for (int u = 0; u < kIU; ++u) {
+            const uint32_t i = i0 + u * kB + threadIdx.x;
+            ok[u] = i < cnt;
+            key[u] = ok[u] ? gp(jb.st_key)[r0 + i] : 0u;
+            row[u] = ok[u] ? gp(jb.st_row)[r0 + i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kIU; ++u) {
+            s[u] = fib32(key[u]) >> (32u - k);
+            old[u] = ~0u;
+            if (ok[u] && key[u] != kEmptyKey)
+                old[u] = atomicCAS((uint32_t*)&K[8u * (s[u] >> 2) + (s[u] & 3u)], kEmptyKey, key[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kIU; ++u) {
+            if (!ok[u]) continue;
+            if (key[u] == kEmptyKey) {  // kept aside in slot 2^k
+                if (atomicCAS((uint32_t*)&meta[2], 0u, 1u) != 0u)
+                    meta[1] = 1;
+                else
+                    K[row_word(1u << k)] = row[u];
+            } else if (old[u] == kEmptyKey) {
+                K[row_word(s[u])] = row[u];
+            } else if (old[u] == key[u]) {
+                meta[1] = 1;
+            } else {
+                insert_rest(K, bmask, s[u], key[u], row[u], meta);
+            }
+            if (use_bf) {
+                const uint32_t h = fib32(key[u]);
+                atomicOr((uint32_t*)&gp(jb.bf)[bf_word(h, lw)], bf_mask(h));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- fused probe
 // key slot of kk from its home bucket's keys v: the slot, kDead (absent), or kDead - 1 (the
 // home bucket is full: continue in the next buckets). The key kEmptyKey lives in slot 2^k when
 // the build holds it (sp).
@@ -284,21 +376,21 @@ __device__ inline uint32_t table_lookup(const HJ_GLOBAL uint32_t* K, uint32_t k,
     return sl;
 }
 
-// block-local append of the lanes of n (uniform) masks through an LDS counter: pos[u] is lane's
-// position among this block's appends for mask u (valid on the lanes of m[u])
-template <int U>
-__device__ inline void block_append_masks(uint32_t* lds_cnt, const uint64_t (&m)[U],
-                                          uint32_t (&pos)[U]) {
-    uint32_t tot = 0;
+// string bytes of one survivor (source row, build row per join) in each output VARCHAR column,
+// added to the block's LDS totals
+template <int NS>
+__device__ inline void count_bytes(const PipeDesc& pd, uint64_t* s_bytes, uint32_t row,
+                                   const uint32_t (&brow)[NS]) {
+    for (int c = 0; c < pd.nostr; ++c) {
+        const int sl = pd.ostr_slot[c];
+        uint32_t rid = row;
 #pragma unroll
-    for (int u = 0; u < U; ++u) tot += (uint32_t)__popcll(m[u]);
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(lds_cnt, tot);
-    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        pos[u] = base + mask_rank(m[u]);
-        base += (uint32_t)__popcll(m[u]);
+        for (int q = 0; q < NS; ++q) rid = sl == 1 + q ? brow[q] : rid;
+        const uint64_t ref = gp(pd.ostr[c].sref)[rid];
+        if (ref == kNullRef) continue;
+        const uint64_t len =
+            (ref & kLongRef) ? long_length(pd.ostr[c], (uint32_t)ref) : (ref & 0xffffu);
+        if (len) atomicAdd((unsigned long long*)&s_bytes[c], (unsigned long long)len);
     }
 }
 
@@ -314,7 +406,8 @@ template <int NS>
 __device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, uint32_t lane,
                                    const HJ_GLOBAL uint32_t* const (&tkey)[NS],
                                    const uint32_t (&tk)[NS], const uint32_t (&tsp)[NS],
-                                   uint32_t* s_cnt, const PipeDesc& pd, uint32_t st0) {
+                                   uint32_t* s_cnt, uint64_t* s_bytes, const PipeDesc& pd,
+                                   uint32_t st0) {
     __builtin_amdgcn_wave_barrier();
     uint64_t alive = n >= HJ_WAVE ? ~0ull : (1ull << n) - 1ull;
     const uint32_t row = qb[0][lane];
@@ -333,10 +426,13 @@ __device__ inline void queue_flush(uint32_t (&qb)[NS + 1][kQueue], uint32_t n, u
     uint32_t pos[1];
     block_append_masks<1>(s_cnt, am, pos);
     if (in_mask(alive)) {
+        uint32_t brow[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) brow[q] = tkey[q][row_word(slot[q])];
         gp(pd.stage[0])[st0 + pos[0]] = row;
 #pragma unroll
-        for (int q = 0; q < NS; ++q)
-            gp(pd.stage[1 + q])[st0 + pos[0]] = tkey[q][row_word(slot[q])];
+        for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[0]] = brow[q];
+        count_bytes<NS>(pd, s_bytes, row, brow);
     }
 }
 
@@ -358,30 +454,39 @@ template <int NS, int U, bool FILT>
 __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pdp) {
     __shared__ uint32_t bfl[kBloomWords];
     __shared__ uint32_t qbuf[kNW][NS + 1][kQueue];  // [wave][row, key of each join][entry]
-    __shared__ uint32_t s_bad, s_cnt;
+    __shared__ uint32_t s_cnt;
+    __shared__ uint64_t s_bytes[kMaxOutStr];  // string bytes of this block's survivors
     const PipeDesc& pd = *pdp;
     const uint32_t t = threadIdx.x, lane = lane_id();
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
-    if (t == 0) {
-        s_cnt = 0;
-        uint32_t bad = 0;
-        for (int j = 0; j < pd.nst; ++j) bad |= gp((gp(pd.jobs) + pd.st[j].job)->meta)[1];
-        s_bad = bad;
-    }
-    const HJ_GLOBAL BuildJob* j0p = gp(pd.jobs) + pd.st[pd.order[0]].job;
-    const uint32_t n0 = gp(j0p->meta)[0];
-    const bool use_bf = bf_enabled(n0);
-    const uint32_t lw = bf_log2w(n0);
-    if (use_bf)
-        for (uint32_t i = t; i < (1u << lw); i += kB) bfl[i] = gp(j0p->bf)[i];
-    __syncthreads();
-    if (s_bad) {
+    // a build with a duplicate key voids the pass (the counters of all kMaxStages jobs follow
+    // the pipeline's in one zero-initialized block: independent loads)
+    uint32_t bad = 0;
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) bad |= gp(pd.meta)[4 + 4 * j + 1];
+    if (bad) {
         if (t == 0) {
             if (blockIdx.x == 0) gp(pd.meta)[0] = 1;
             gp(pd.blk_cnt)[blockIdx.x] = 0;
         }
         return;
     }
+    const uint32_t n0 = gp(pd.sj[0].meta)[0];
+    const bool use_bf = bf_enabled(n0);
+    const uint32_t lw = bf_log2w(n0);
+    {
+        // the whole Bloom word array (kBloomWords, used or not): independent 16-byte loads
+        constexpr int kC = kBloomWords / 4 / kB;
+        const HJ_GLOBAL u32x4* src = (const HJ_GLOBAL u32x4*)gp(pd.sj[0].bf);
+        u32x4 c[kC];
+#pragma unroll
+        for (int i = 0; i < kC; ++i) c[i] = src[t + i * kB];
+#pragma unroll
+        for (int i = 0; i < kC; ++i) ((u32x4*)bfl)[t + i * kB] = c[i];
+    }
+    if (t == 0) s_cnt = 0;
+    if (t < (uint32_t)kMaxOutStr) s_bytes[t] = 0;
+    __syncthreads();
     const uint32_t nsrc = pd.src_rows;
     const uint32_t G = (nsrc + 63u) >> 6;
     __amdgpu_buffer_rsrc_t rk[NS], rv[NS];  // key columns, their validity words (0 bytes: none)
@@ -390,16 +495,14 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     uint32_t tk[NS], tsp[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-        const PipeStage& ps = pd.st[pd.order[q]];
-        const int32_t* kf = ps.keycol.flat;
-        const uint64_t* kv = ps.keycol.vbits;
-        rk[q] = make_rsrc(kf, nsrc * 4u);
-        rv[q] = make_rsrc(kv ? (const void*)kv : (const void*)kf, kv ? G * 8u : 0u);
-        nonull[q] = kv ? 0u : ~0u;
-        const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
-        tkey[q] = gp(jp->table);
-        tk[q] = table_bits(gp(jp->meta)[0]);
-        tsp[q] = gp(jp->meta)[2];
+        const StreamJoin& sj = pd.sj[q];
+        rk[q] = make_rsrc(sj.flat, nsrc * 4u);
+        rv[q] = make_rsrc(sj.vbits ? (const void*)sj.vbits : (const void*)sj.flat,
+                          sj.vbits ? G * 8u : 0u);
+        nonull[q] = sj.vbits ? 0u : ~0u;
+        tkey[q] = gp(sj.table);
+        tk[q] = table_bits(gp(sj.meta)[0]);
+        tsp[q] = gp(sj.meta)[2];
     }
     const int nf = FILT ? pd.nf : 0;  // filters of the source scan (their loads are not
                                        // prefetched: a separate instantiation)
@@ -414,7 +517,6 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     // are filtered and looked up. Lane u < U holds the validity word of group u.
     const uint32_t lane4 = lane * 4u;
     const uint32_t lanev = (lane < (uint32_t)U ? lane : (uint32_t)U - 1u) * 8u;
-    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
     int32_t ka[NS][U], kb[NS][U];
     u32x2 va[NS], vb[NS];
     auto fetch = [&](int32_t (&kx)[NS][U], u32x2 (&vx)[NS], uint32_t gx) {
@@ -477,7 +579,7 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
                 }
                 qc += (uint32_t)__popcll(live[u]);
                 if (qc >= HJ_WAVE) {
-                    queue_flush<NS>(qbuf[w], HJ_WAVE, lane, tkey, tk, tsp, &s_cnt, pd, st0);
+                    queue_flush<NS>(qbuf[w], HJ_WAVE, lane, tkey, tk, tsp, &s_cnt, s_bytes, pd, st0);
                     queue_shift<NS>(qbuf[w], qc - HJ_WAVE, lane);
                     qc -= HJ_WAVE;
                 }
@@ -514,10 +616,13 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (in_mask(live[u])) {
+                uint32_t br[NS];
+#pragma unroll
+                for (int q = 0; q < NS; ++q) br[q] = tkey[q][row_word(brow[q][u])];
                 gp(pd.stage[0])[st0 + pos[u]] = r[u];
 #pragma unroll
-                for (int q = 0; q < NS; ++q)
-                    gp(pd.stage[1 + q])[st0 + pos[u]] = tkey[q][row_word(brow[q][u])];
+                for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[u]] = br[q];
+                count_bytes<NS>(pd, s_bytes, r[u], br);
             }
         }
     };
@@ -532,9 +637,11 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
         step(kb, vb, gb);
         gb += kNW;
     }
-    if (qc) queue_flush<NS>(qbuf[w], qc, lane, tkey, tk, tsp, &s_cnt, pd, st0);
+    if (qc) queue_flush<NS>(qbuf[w], qc, lane, tkey, tk, tsp, &s_cnt, s_bytes, pd, st0);
     __syncthreads();
     if (t == 0) gp(pd.blk_cnt)[blockIdx.x] = s_cnt;
+    if (pd.blk_bytes && t < (uint32_t)pd.nostr)
+        gp(pd.blk_bytes)[(uint64_t)blockIdx.x * kMaxOutStr + t] = s_bytes[t];
 }
 
 // the fused pass's staged survivors -> the outputs (no joins keyed by a build table) or the
@@ -681,13 +788,23 @@ __global__ __launch_bounds__(kB) void k_pp_fwrite(const PipeDesc* __restrict__ p
     }
 }
 
+template <class K>
+void launch_timed(K kernel, const PipeDesc& hp, const PipeDesc* dp, hipStream_t s, hipEvent_t a,
+                  hipEvent_t b) {
+    if (a && b)
+        hipExtLaunchKernelGGL(kernel, dim3(hp.probe_blocks), dim3(kB), 0, s, a, b, 0, dp);
+    else
+        hipLaunchKernelGGL(kernel, dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
+}
+
 template <int NS>
-void launch_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
+void launch_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s, hipEvent_t a,
+                  hipEvent_t b) {
     constexpr int U = probe_unroll(NS);
     if (hp.nf)
-        hipLaunchKernelGGL((k_sp_probe<NS, U, true>), dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
+        launch_timed(k_sp_probe<NS, U, true>, hp, dp, s, a, b);
     else
-        hipLaunchKernelGGL((k_sp_probe<NS, U, false>), dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
+        launch_timed(k_sp_probe<NS, U, false>, hp, dp, s, a, b);
 }
 
 template <int NS>
@@ -724,23 +841,22 @@ uint32_t probe_resident_blocks(int ns, uint32_t num_cus) {
 
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s) {
     if (hb.njobs == 0) return;
-    const dim3 grid(hb.grid, hb.njobs);
-    hipLaunchKernelGGL(k_sb_filter, grid, dim3(kB), 0, s, db);
-    hipLaunchKernelGGL(k_sb_count, dim3(hb.njobs), dim3(1024), 0, s, db);
-    hipLaunchKernelGGL(k_sb_clear, grid, dim3(kB), 0, s, db);
-    hipLaunchKernelGGL(k_sb_insert, grid, dim3(kB), 0, s, db);
+    hipLaunchKernelGGL(k_sb_filter, dim3(hb.grid), dim3(kB), 0, s, db);
+    hipLaunchKernelGGL(k_sb_clear, dim3(hb.cgrid, hb.njobs), dim3(kB), 0, s, db);
+    hipLaunchKernelGGL(k_sb_insert, dim3(hb.grid), dim3(kB), 0, s, db);
 }
 
-void pipe_probe_fused(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s) {
+void pipe_probe_fused(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s, hipEvent_t ev_a,
+                      hipEvent_t ev_b) {
     switch (hp.nstream) {
-        case 1: launch_probe<1>(hp, dp, s); break;
-        case 2: launch_probe<2>(hp, dp, s); break;
-        case 3: launch_probe<3>(hp, dp, s); break;
-        case 4: launch_probe<4>(hp, dp, s); break;
-        case 5: launch_probe<5>(hp, dp, s); break;
-        case 6: launch_probe<6>(hp, dp, s); break;
-        case 7: launch_probe<7>(hp, dp, s); break;
-        case 8: launch_probe<8>(hp, dp, s); break;
+        case 1: launch_probe<1>(hp, dp, s, ev_a, ev_b); break;
+        case 2: launch_probe<2>(hp, dp, s, ev_a, ev_b); break;
+        case 3: launch_probe<3>(hp, dp, s, ev_a, ev_b); break;
+        case 4: launch_probe<4>(hp, dp, s, ev_a, ev_b); break;
+        case 5: launch_probe<5>(hp, dp, s, ev_a, ev_b); break;
+        case 6: launch_probe<6>(hp, dp, s, ev_a, ev_b); break;
+        case 7: launch_probe<7>(hp, dp, s, ev_a, ev_b); break;
+        case 8: launch_probe<8>(hp, dp, s, ev_a, ev_b); break;
         default: return;
     }
 }

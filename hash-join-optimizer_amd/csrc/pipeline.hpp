@@ -18,17 +18,21 @@ constexpr uint32_t kEmptyKey = 0x80000000u;  // a free key slot (INT32_MIN itsel
 constexpr uint32_t kBloomWords = 4096;       // 16 KiB: the LDS-resident filter of a small build side
 constexpr uint32_t kBucketKeys = 4;          // one 16-byte load reads a whole bucket
 constexpr uint32_t kMinTableBits = 6;
+constexpr int kMaxOutStr = 16;                // output VARCHAR columns counted by the fused pass
 
 // word index of slot s's row in a table
 __host__ __device__ inline uint32_t row_word(uint32_t s) { return 8u * (s >> 2) + 4u + (s & 3u); }
 
-// key slots for n build tuples: 2^bits >= 8n (load factor <= 1/8, about half a key per
-// 4-key bucket, so a lookup nearly always ends in its home bucket)
+// key slots for n build tuples: 2^bits >= 4n (load factor <= 1/4, at most one key per 4-key
+// bucket on average, so a lookup nearly always ends in its home bucket)
 __host__ __device__ inline uint32_t table_bits(uint32_t n) {
     uint32_t k = 0;
     while ((1ull << k) < (uint64_t)n) ++k;
-    return k + 3 < kMinTableBits ? kMinTableBits : k + 3;
+    return k + 2 < kMinTableBits ? kMinTableBits : k + 2;
 }
+
+// rows per block of the build's filter and insert passes
+constexpr uint32_t kBuildRows = 8192;
 
 // One build side: the rows of a (filtered) base scan with a non-NULL key, in a bucketized
 // open-addressing table (linear probing over buckets). A bucket is 32 bytes: kBucketKeys keys,
@@ -43,9 +47,11 @@ struct BuildJob {
     int32_t nf;
     uint32_t rows;      // table rows (bound of the tuple count)
     uint32_t kmax;      // the table holds 2^kmax key slots
+    uint32_t blk0;      // first block of this job in the filter and insert grids ...
+    uint32_t nblk;      // ... and its blocks, ceil(rows / kBuildRows) (at least 1)
     uint32_t* meta;     // [0] tuples, [1] a key occurs twice, [2] kEmptyKey present, [3] unused
-    uint32_t* st_key;   // staged tuples, each block of the filter pass at the start of its
-    uint32_t* st_row;   // own row range (capacity: rows rounded up to whole ranges)
+    uint32_t* st_key;   // staged tuples, block x of the filter pass from x * kBuildRows on
+    uint32_t* st_row;   // (capacity nblk * kBuildRows)
     uint32_t* blk;      // tuples staged per block of the filter pass
     uint32_t* table;    // 2 * 2^kmax + 8 words; 2^table_bits(tuples) key slots used
     uint32_t* bf;       // blocked Bloom filter, kBloomWords words (small build sides only)
@@ -53,14 +59,28 @@ struct BuildJob {
 
 struct BuildBatch {
     BuildJob job[kMaxStages];
+    uint32_t blk0[kMaxStages];  // job[j].blk0, ~0 for j >= njobs
     int32_t njobs;
-    uint32_t grid;      // blocks per job of the grid-stride kernels
+    uint32_t grid;      // blocks of the filter and insert passes (every job's nblk)
+    uint32_t cgrid;     // blocks per job of the clear pass (grid-stride)
+    uint32_t nzero;     // words of `zero` cleared by the filter pass (<= 256): the jobs' and the
+    uint32_t* zero;     // pipeline's metas
 };
 
 struct PipeStage {
     int32_t job;        // build job of this join
     int32_t key_slot;   // 0: key is a column of the probe source; j+1: of stage j's build table
     DCol keycol;
+};
+
+// a join of the fused pass, in probe order (filled by the host, so that the pass reaches every
+// pointer it needs in one load from the descriptor)
+struct StreamJoin {
+    const int32_t* flat;    // key column of the source: values ...
+    const uint64_t* vbits;  // ... and validity words (nullptr: no NULL)
+    const uint32_t* table;  // the build's table ...
+    const uint32_t* meta;   // ... its counters ([0] tuples, [2] kEmptyKey present) ...
+    const uint32_t* bf;     // ... and Bloom words
 };
 
 struct PipeDesc {
@@ -90,29 +110,31 @@ struct PipeDesc {
     uint32_t* tile_cnt;           // final compaction, per 4096 survivors
     uint32_t* tile_off;
     uint32_t* meta;               // [0] fall back (a build key occurs twice), [1] unused,
-                                  // [2] survivors of the fused pass, [3] output rows
+                                  // [2] survivors of the fused pass, [3] output rows; then the
+                                  // counters of kMaxStages build jobs (4 words each)
+    StreamJoin sj[kMaxStages];    // the fused joins in probe order
+    // direct output: string bytes of the survivors of each block per output VARCHAR column
+    DCol ostr[kMaxOutStr];        // the columns ...
+    int32_t ostr_slot[kMaxOutStr];  // ... their row ids' stage array (0: source, 1 + q: join q)
+    int32_t nostr;
+    uint64_t* blk_bytes;          // per block, kMaxOutStr totals
     const BuildJob* jobs;
     uint32_t grid;                // blocks of the grid-stride kernels
 };
 
 // 64-row groups per wave and iteration of the fused pass over NS streamed joins
 constexpr int probe_unroll(int ns) { return ns == 1 ? 8 : (ns == 2 ? 6 : (ns <= 4 ? 4 : 2)); }
-// rows per block-range of the build filter pass
-constexpr uint32_t kBuildTile = 1024;
-__host__ __device__ inline uint32_t build_range(uint32_t rows, uint32_t blocks) {
-    const uint32_t tiles = (rows + kBuildTile - 1) / kBuildTile;
-    return ((tiles + blocks - 1) / blocks) * kBuildTile;
-}
 
 // blocks of the fused pass over ns streamed joins that are resident at once on the device (the
 // pass gives each a contiguous share of the source, so a second round of blocks would double
 // its time)
 uint32_t probe_resident_blocks(int ns, uint32_t num_cus);
 
-// build: filter + stage, clear tables, insert
+// build: filter + stage, count + clear tables, insert
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s);
-// the fused pass over the source (one launch) ...
-void pipe_probe_fused(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s);
+// the fused pass over the source (one launch; ev_a/ev_b, when set, record its start and end) ...
+void pipe_probe_fused(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s,
+                      hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr);
 // ... then the survivor compaction, the joins keyed by build tables and the final compaction
 void pipe_probe_tail(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s);
 
