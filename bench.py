@@ -111,6 +111,25 @@ def kernel_bytes(name, info, launches):
     return None
 
 
+# ------------------------------------------------------------------ measured HBM traffic
+TRAFFIC_FILES = {("probe_fused", "star4"): "k_sp_probe_star4_traffic.md"}
+
+
+def measured_traffic(kernel, workload_name, scale):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None when there is none"""
+    f = TRAFFIC_FILES.get((kernel, workload_name))
+    if f is None or scale != 1.0:
+        return None
+    try:
+        for line in open(os.path.join(ROOT, "profiles", f)):
+            if line.strip().startswith("bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 # ------------------------------------------------------------------ CPU baseline (oracle)
 def cpu_baseline(d, workload_name, tuples, threads):
     """the oracle (CPU restatement of the reference's execute()), used only as the baseline"""
@@ -201,7 +220,8 @@ def main():
         avg_s = dom_ms / launches / 1e3
         achieved = per_launch_bytes / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
+                    "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9),
+                    "traffic": measured_traffic(dom, args.workload, args.scale),
                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_s * 1e3}
 
     cpu = None
