@@ -425,9 +425,16 @@ class Executor {
         std::memset(&hp, 0, sizeof hp);
         hb.njobs = (int32_t)joins.size();
         hp.nst = (int32_t)joins.size();
-        // the pipeline's and every job's counters (cleared by the build's filter pass)
-        // pipeline and job counters, then the direct output's totals (all cleared by the build)
-        uint32_t* metas = scratch<uint32_t>(kMetaWords + 2 * kEmitOut);
+        // descriptors (build batch, pipeline) and the counters (pipeline, jobs, direct output) in
+        // one block, uploaded together before the first kernel (the counters as zeros)
+        const size_t bb_bytes = sizeof(BuildBatch);
+        const size_t pd_off = (bb_bytes + 255) & ~(size_t)255;
+        const size_t meta_off = (pd_off + sizeof(PipeDesc) + 255) & ~(size_t)255;
+        const size_t desc_bytes = meta_off + (kMetaWords + 2 * kEmitOut) * 4;
+        if (desc_bytes > hj_context::kPinnedBytes - hj_context::kDescOffset)
+            throw Error(HJ_E_INVALID, "pipeline descriptors exceed the staging buffer");
+        uint8_t* ddesc = (uint8_t*)A_.scratch(desc_bytes);
+        uint32_t* metas = (uint32_t*)(ddesc + meta_off);
         hp.meta = metas;
         for (size_t i = 0; i < joins.size(); ++i) {
             const hj_node& jn = plan_->nodes[joins[i]];
@@ -494,8 +501,19 @@ class Executor {
             cur.swap(nxt);
         }
         for (int j = 0; j < kMaxStages; ++j) hb.blk0[j] = j < hb.njobs ? hb.job[j].blk0 : ~0u;
-        hb.zero = metas;  // cleared by the filter pass
-        hb.nzero = kMetaWords + 2 * kEmitOut;
+        // partitioned build (segments built in LDS) when every job has at most 2^kMaxFineBits
+        // fine partitions and kMaxPartBlocks filter blocks; else clear + compare-and-swap inserts
+        hb.part = ctx_->part_build ? 1 : 0;
+        for (int j = 0; j < hb.njobs; ++j)
+            if (fine_bits(hb.job[j].kmax) > kMaxFineBits || hb.job[j].nblk > kMaxPartBlocks)
+                hb.part = 0;
+        if (hb.part)
+            for (int j = 0; j < hb.njobs; ++j) {
+                BuildJob& jb = hb.job[j];
+                const uint64_t P = 1ull << fine_bits(jb.kmax);
+                jb.poff = scratch<uint16_t>((uint64_t)jb.nblk * (P + 1));
+            }
+        hb.sgrid = (uint32_t)ctx_->num_cus * 4;  // segment pass: a grid-stride loop
 
         hp.nf = (int32_t)srel.filters.size();
         for (size_t f = 0; f < srel.filters.size(); ++f) {
@@ -600,16 +618,13 @@ class Executor {
         }
         hp.grid = (uint32_t)ctx_->num_cus * 8;
 
-        // descriptors -> pinned staging -> device
+        // descriptors and zeroed counters -> pinned staging -> device
         uint8_t* pin = (uint8_t*)ctx_->pinned + hj_context::kDescOffset;
-        const size_t bb_bytes = sizeof(BuildBatch), pd_off = (bb_bytes + 255) & ~(size_t)255;
-        if (pd_off + sizeof(PipeDesc) > hj_context::kPinnedBytes - hj_context::kDescOffset)
-            throw Error(HJ_E_INVALID, "pipeline descriptors exceed the staging buffer");
-        uint8_t* ddesc = (uint8_t*)A_.scratch(pd_off + sizeof(PipeDesc));
         hp.jobs = (const BuildJob*)(ddesc + offsetof(BuildBatch, job));
+        std::memset(pin, 0, desc_bytes);
         std::memcpy(pin, &hb, bb_bytes);
         std::memcpy(pin + pd_off, &hp, sizeof(PipeDesc));
-        HJ_HIP(hipMemcpyAsync(ddesc, pin, pd_off + sizeof(PipeDesc), hipMemcpyHostToDevice, s_));
+        HJ_HIP(hipMemcpyAsync(ddesc, pin, desc_bytes, hipMemcpyHostToDevice, s_));
         const BuildBatch* dbb = (const BuildBatch*)ddesc;
         const PipeDesc* dpd = (const PipeDesc*)(ddesc + pd_off);
         { Prof p_(ctx_, "pipe_build"); pipe_build(hb, dbb, s_); }
@@ -1096,6 +1111,10 @@ int hj_set_option(hj_context* ctx, const char* name, int64_t value) {
     if (!ctx || !name) return HJ_E_INVALID;
     if (std::strcmp(name, "pipeline") == 0) {
         ctx->use_pipeline = value != 0;
+        return HJ_OK;
+    }
+    if (std::strcmp(name, "partitioned_build") == 0) {
+        ctx->part_build = value != 0;
         return HJ_OK;
     }
     ctx->last_error = std::string("unknown option ") + name;

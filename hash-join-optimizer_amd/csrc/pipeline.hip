@@ -88,7 +88,7 @@ __device__ inline u32x4 load_bucket(const HJ_GLOBAL uint32_t* K, uint32_t b) {
 // slot of `key` (!= kEmptyKey), kDead when absent, given the keys v of its home bucket b;
 // a key leaves its home bucket only when that bucket is full, so a bucket with a free slot ends
 // the search, and at load factor <= 1/4 the home bucket nearly always does
-__device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t bmask, uint32_t b,
+__device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t smask, uint32_t b,
                                           uint32_t key, u32x4 v) {
     for (;;) {
         if (v.x == key) return 4u * b;
@@ -97,7 +97,7 @@ __device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t 
         if (v.w == key) return 4u * b + 3u;
         if (v.x == kEmptyKey || v.y == kEmptyKey || v.z == kEmptyKey || v.w == kEmptyKey)
             return kDead;
-        b = (b + 1u) & bmask;
+        b = next_bucket(b, smask);
         v = load_bucket(K, b);
     }
 }
@@ -172,7 +172,6 @@ __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__
     __shared__ uint32_t s_cnt;
     const uint32_t t = threadIdx.x, lane = lane_id();
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
-    if (blockIdx.x == 0 && t < bb->nzero) gp(bb->zero)[t] = 0;  // job and pipeline metas
     const BuildJob& jb = bb->job[build_job_of(bb, blockIdx.x)];
     const uint32_t x = blockIdx.x - jb.blk0;
     const uint32_t rows = jb.rows, G = (rows + 63u) >> 6;
@@ -236,6 +235,257 @@ __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__
     if (t == 0) gp(jb.blk)[x] = s_cnt;
 }
 
+// ---------------------------------------------------------------- partitioned build
+// Filter pass of the partitioned build: as k_sb_filter, and the block's tuples are staged in
+// order of their fine hash partition (top fine_bits(kmax) bits of the Fibonacci hash), counted
+// with LDS atomics; poff records where each partition starts in the block's range. The first
+// block of a job also clears the job's small Bloom filter (set by the segment pass).
+constexpr int kFR = (int)(kBuildRows / (kB * kFU));  // rounds per block
+static_assert(kFR * kB * kFU == (int)kBuildRows, "build rows per block");
+
+__global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restrict__ bb) {
+    __shared__ uint32_t hist[(1u << kMaxFineBits) + 1];
+    __shared__ uint32_t lds[kNW];
+    const uint32_t t = threadIdx.x, lane = lane_id();
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
+    const BuildJob& jb = bb->job[build_job_of(bb, blockIdx.x)];
+    const uint32_t x = blockIdx.x - jb.blk0;
+    const uint32_t rows = jb.rows, G = (rows + 63u) >> 6;
+    const uint32_t r0 = x * kBuildRows;
+    const uint32_t r1 = r0 + kBuildRows < rows ? r0 + kBuildRows : rows;
+    const uint32_t fb = fine_bits(jb.kmax), P = 1u << fb;
+    if (x == 0)
+        for (uint32_t i = t; i < kBloomWords; i += kB) gp(jb.bf)[i] = 0;
+    for (uint32_t i = t; i <= P; i += kB) hist[i] = 0;
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(jb.key.flat, rows * 4u);
+    const __amdgpu_buffer_rsrc_t rkv =
+        make_rsrc(jb.key.vbits ? (const void*)jb.key.vbits : (const void*)jb.key.flat,
+                  jb.key.vbits ? G * 8u : 0u);
+    const uint32_t kv_none = jb.key.vbits ? 0u : ~0u;
+    const int nf = jb.nf;
+    const uint32_t lanev = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 32u;
+    int32_t key[kFR][kFU];
+    uint32_t rank[kFR][kFU];
+    uint32_t alive = 0;  // bit r * kFU + u: this lane's row of round r, group u passes
+    // every round's keys and validity words at once, then the filters' columns the same way
+    uint64_t live[kFR][kFU];
+    {
+        u32x2 kvw[kFR];
+#pragma unroll
+        for (int rd = 0; rd < kFR; ++rd) {
+            const uint32_t base = r0 + rd * (kB * kFU);
+            const uint32_t vo = (base + w * 64u + lane) * 4u;
+#pragma unroll
+            for (int u = 0; u < kFU; ++u)
+                key[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(rk, vo + u * 1024u, 0, 0);
+            kvw[rd] = __builtin_amdgcn_raw_buffer_load_b64(rkv, (base / 64u + w) * 8u + lanev, 0, 0);
+        }
+#pragma unroll
+        for (int rd = 0; rd < kFR; ++rd)
+#pragma unroll
+            for (int u = 0; u < kFU; ++u) {
+                const uint32_t g0 = r0 + rd * (kB * kFU) + u * 256u + w * 64u;
+                live[rd][u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) &
+                              lane_word(kvw[rd].x | kv_none, kvw[rd].y | kv_none, u);
+            }
+    }
+    for (int f = 0; f < nf; ++f) {
+        const DCol& fc = jb.fcol[f];
+        const __amdgpu_buffer_rsrc_t rf = make_rsrc(fc.flat, rows * 4u);
+        const __amdgpu_buffer_rsrc_t rfv =
+            make_rsrc(fc.vbits ? (const void*)fc.vbits : (const void*)fc.flat,
+                      fc.vbits ? G * 8u : 0u);
+        const uint32_t fv_none = fc.vbits ? 0u : ~0u;
+        const int32_t lo = jb.flo[f], hi = jb.fhi[f];
+        int32_t v[kFR][kFU];
+        u32x2 fvw[kFR];
+#pragma unroll
+        for (int rd = 0; rd < kFR; ++rd) {
+            const uint32_t base = r0 + rd * (kB * kFU);
+            const uint32_t vo = (base + w * 64u + lane) * 4u;
+#pragma unroll
+            for (int u = 0; u < kFU; ++u)
+                v[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(rf, vo + u * 1024u, 0, 0);
+            fvw[rd] = __builtin_amdgcn_raw_buffer_load_b64(rfv, (base / 64u + w) * 8u + lanev, 0, 0);
+        }
+#pragma unroll
+        for (int rd = 0; rd < kFR; ++rd)
+#pragma unroll
+            for (int u = 0; u < kFU; ++u)
+                live[rd][u] &= __ballot(v[rd][u] >= lo && v[rd][u] <= hi) &
+                               lane_word(fvw[rd].x | fv_none, fvw[rd].y | fv_none, u);
+    }
+#pragma unroll
+    for (int rd = 0; rd < kFR; ++rd)
+#pragma unroll
+        for (int u = 0; u < kFU; ++u) {
+            rank[rd][u] = 0;
+            if (in_mask(live[rd][u])) {
+                const uint32_t fp = fb ? fib32((uint32_t)key[rd][u]) >> (32u - fb) : 0u;
+                rank[rd][u] = atomicAdd(&hist[fp], 1u);
+                alive |= 1u << (rd * kFU + u);
+            }
+        }
+    __syncthreads();
+    // partition starts: exclusive scan of the P counts, kPB per thread; hist[P] = the total
+    {
+        constexpr int kPB = (1 << kMaxFineBits) / kB;
+        const uint32_t per = (P + kB - 1) / kB;
+        uint32_t loc[kPB], sum = 0;
+#pragma unroll
+        for (int c = 0; c < kPB; ++c) {
+            const uint32_t bin = t * per + c;
+            const uint32_t v = (uint32_t)c < per && bin < P ? hist[bin] : 0u;
+            loc[c] = sum;
+            sum += v;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<kB>(sum, lds, &tot);
+#pragma unroll
+        for (int c = 0; c < kPB; ++c) {
+            const uint32_t bin = t * per + c;
+            if ((uint32_t)c < per && bin < P) hist[bin] = ex + loc[c];
+        }
+        if (t == 0) hist[P] = tot;
+    }
+    __syncthreads();
+    HJ_GLOBAL uint16_t* po = gp(jb.poff) + (uint64_t)x * (P + 1u);
+    for (uint32_t i = t; i <= P; i += kB) po[i] = (uint16_t)hist[i];
+#pragma unroll
+    for (int rd = 0; rd < kFR; ++rd)
+#pragma unroll
+        for (int u = 0; u < kFU; ++u) {
+            if (!((alive >> (rd * kFU + u)) & 1u)) continue;
+            const uint32_t fp = fb ? fib32((uint32_t)key[rd][u]) >> (32u - fb) : 0u;
+            const uint32_t pos = hist[fp] + rank[rd][u];
+            gp(jb.st_key)[r0 + pos] = (uint32_t)key[rd][u];
+            gp(jb.st_row)[r0 + pos] = r0 + rd * (kB * kFU) + u * 256u + w * 64u + lane;
+        }
+    if (t == 0) {
+        gp(jb.blk)[x] = hist[P];
+        atomicAdd((uint32_t*)gp(jb.meta), hist[P]);  // the job's tuple count
+    }
+}
+
+// Segment pass: the table segments of every job (2^(fine bits of the table) per job, from the
+// tuple count), a grid-stride loop over them. A block gathers a segment's tuples from every
+// filter block's staged range (their fine partitions are contiguous there), inserts them into
+// the segment held in LDS, and writes the segment out whole (the table needs no clearing). A
+// segment that runs out of slots flags the build, as a repeated key does.
+__global__ __launch_bounds__(kB) void k_sb_segbuild(const BuildBatch* __restrict__ bb) {
+    __shared__ uint32_t T[8u << kSegBits];  // the segment: [4 keys | 4 rows] per bucket
+    __shared__ uint32_t s_lo[kMaxPartBlocks], s_pre[kMaxPartBlocks + 1];
+    __shared__ uint32_t lds[kNW];
+    const uint32_t t = threadIdx.x;
+    const int nj = bb->njobs;
+    uint32_t first[kMaxStages + 1];  // first flat segment of each job
+    first[0] = 0;
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) {
+        const uint32_t nseg =
+            j < nj ? 1u << fine_bits(table_bits(gp(bb->job[j].meta)[0])) : 0u;
+        first[j + 1] = first[j] + nseg;
+    }
+    for (uint32_t f = blockIdx.x; f < first[kMaxStages]; f += gridDim.x) {
+        int j = 0;
+#pragma unroll
+        for (int i = 1; i < kMaxStages; ++i) j += f >= first[i] ? 1 : 0;
+        const BuildJob& jb = bb->job[j];
+        HJ_GLOBAL uint32_t* meta = gp(jb.meta);
+        const uint32_t n = meta[0];
+        const uint32_t fb = fine_bits(jb.kmax), P = 1u << fb;
+        const uint32_t k = table_bits(n), sb = fine_bits(k), S = 1u << (k - 2 - sb);
+        const uint32_t per = 1u << (fb - sb);  // fine partitions per segment
+        const uint32_t seg = f - first[j], y = seg * per;
+        const uint32_t nb = jb.nblk;
+        __syncthreads();  // the previous segment's LDS is written out
+        // this segment's range in each filter block (thread t: blocks 2t, 2t + 1), their
+        // exclusive scan, so that the segment's tuples are spread over all threads
+        uint32_t cnt[2] = {0, 0};
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const uint32_t b = 2u * t + i;
+            if (b < nb) {
+                const HJ_GLOBAL uint16_t* po = gp(jb.poff) + (uint64_t)b * (P + 1u);
+                const uint32_t lo = po[y], hi = po[y + per];
+                s_lo[b] = lo;
+                cnt[i] = hi - lo;
+            }
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<kB>(cnt[0] + cnt[1], lds, &tot);
+        if (2u * t < nb) s_pre[2u * t] = ex;
+        if (2u * t + 1u < nb) s_pre[2u * t + 1u] = ex + cnt[0];
+        if (t == 0) s_pre[nb] = tot;
+        for (uint32_t i = t; i < 8u * S; i += kB) T[i] = (i & 4u) ? 0u : kEmptyKey;
+        __syncthreads();
+        const uint32_t smask = S - 1u;
+        const bool use_bf = bf_enabled(n);
+        const uint32_t lw = bf_log2w(n);
+        HJ_GLOBAL uint32_t* K = gp(jb.table);
+        constexpr int kSU = 4;  // items per thread and round: their gathers in flight at once
+        for (uint32_t i0 = 0; i0 < tot; i0 += kB * kSU) {
+            uint32_t key[kSU], row[kSU];
+            bool have[kSU];
+#pragma unroll
+            for (int u = 0; u < kSU; ++u) {
+                const uint32_t i = i0 + u * kB + t;
+                have[u] = i < tot;
+                key[u] = row[u] = 0;
+                if (!have[u]) continue;
+                // filter block holding item i: the last b with s_pre[b] <= i
+                uint32_t lo = 0, hi = nb;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_pre[mid] <= i)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                const uint64_t src = (uint64_t)lo * kBuildRows + s_lo[lo] + (i - s_pre[lo]);
+                key[u] = gp(jb.st_key)[src];
+                row[u] = gp(jb.st_row)[src];
+            }
+#pragma unroll
+            for (int u = 0; u < kSU; ++u) {
+                if (!have[u]) continue;
+                const uint32_t h = fib32(key[u]);
+                if (use_bf) atomicOr((uint32_t*)&gp(jb.bf)[bf_word(h, lw)], bf_mask(h));
+                if (key[u] == kEmptyKey) {  // kept aside in slot 2^k
+                    if (atomicCAS((uint32_t*)&meta[2], 0u, 1u) != 0u)
+                        meta[1] = 1;
+                    else
+                        K[row_word(1u << k)] = row[u];
+                    continue;
+                }
+                const uint32_t s0 = h >> (32u - k);  // home slot
+                uint32_t bk = (s0 >> 2) & smask;
+                const uint32_t j0 = s0 & 3u;
+                bool done = false;
+                for (uint32_t m = 0; m <= smask && !done; ++m, bk = (bk + 1u) & smask) {
+                    for (uint32_t jj = 0; jj < kBucketKeys && !done; ++jj) {
+                        const uint32_t sl = m == 0 ? (j0 + jj) & 3u : jj;
+                        const uint32_t old = atomicCAS(&T[8u * bk + sl], kEmptyKey, key[u]);
+                        if (old == kEmptyKey) {
+                            T[8u * bk + 4u + sl] = row[u];
+                            done = true;
+                        } else if (old == key[u]) {
+                            meta[1] = 1;
+                            done = true;
+                        }
+                    }
+                }
+                if (!done) meta[1] = 1;  // the segment is full
+            }
+        }
+        __syncthreads();
+        HJ_GLOBAL u32x4* dst = (HJ_GLOBAL u32x4*)(K + 8u * (seg * S));
+        const u32x4* srcv = (const u32x4*)T;
+        for (uint32_t i = t; i < 2u * S; i += kB) dst[i] = srcv[i];
+    }
+}
+
 // tuple count of a job (sum of its filter blocks' counts; every block of the clear pass
 // computes it, block 0 records it), then the table's 2^k key slots and the Bloom words
 __global__ __launch_bounds__(kB) void k_sb_clear(const BuildBatch* __restrict__ bb) {
@@ -280,16 +530,17 @@ __device__ inline bool insert_at(HJ_GLOBAL uint32_t* K, uint32_t sl, uint32_t ke
 }
 
 // the rest of an insert whose first compare-and-swap (slot s of its home bucket) found another
-// key: the other slots of the home bucket, then the following buckets whole (the table always
-// has free slots: load factor <= 1/4)
-__device__ inline void insert_rest(HJ_GLOBAL uint32_t* K, uint32_t bmask, uint32_t s,
+// key: the other slots of the home bucket, then the following buckets of its segment whole (a
+// segment that has no free slot left flags the build, as a repeated key does)
+__device__ inline void insert_rest(HJ_GLOBAL uint32_t* K, uint32_t smask, uint32_t s,
                                    uint32_t key, uint32_t row, HJ_GLOBAL uint32_t* meta) {
     const uint32_t b0 = s >> 2;
     for (uint32_t i = 1; i < kBucketKeys; ++i)
         if (insert_at(K, 4u * b0 + ((s + i) & 3u), key, row, meta)) return;
-    for (uint32_t b = (b0 + 1u) & bmask;; b = (b + 1u) & bmask)
+    for (uint32_t b = next_bucket(b0, smask), m = 0; m < smask; b = next_bucket(b, smask), ++m)
         for (uint32_t j = 0; j < kBucketKeys; ++j)
             if (insert_at(K, 4u * b + j, key, row, meta)) return;
+    meta[1] = 1;  // the segment is full
 }
 
 // insert: block x of a job inserts the tuples staged by block x of the filter pass. A key's
@@ -300,7 +551,7 @@ __global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__
     const uint32_t x = blockIdx.x - jb.blk0;
     HJ_GLOBAL uint32_t* meta = gp(jb.meta);
     const uint32_t n = meta[0];
-    const uint32_t k = table_bits(n), bmask = (1u << (k - 2)) - 1u;
+    const uint32_t k = table_bits(n), smask = seg_mask(k);
     const bool use_bf = bf_enabled(n);
     const uint32_t lw = bf_log2w(n);
     const uint32_t r0 = x * kBuildRows;
@@ -336,7 +587,7 @@ __global__ __launch_bounds__(kB) void k_sb_insert(const BuildBatch* __restrict__
             } else if (old[u] == key[u]) {
                 meta[1] = 1;
             } else {
-                insert_rest(K, bmask, s[u], key[u], row[u], meta);
+                insert_rest(K, smask, s[u], key[u], row[u], meta);
             }
             if (use_bf) {
                 const uint32_t h = fib32(key[u]);
@@ -369,9 +620,9 @@ __device__ inline uint32_t table_lookup(const HJ_GLOBAL uint32_t* K, uint32_t k,
     const uint32_t hb = home_bucket(kk, k);
     uint32_t sl = resolve_home(load_bucket(K, hb), kk, hb, k, sp);
     if (sl == kDead - 1u) {
-        const uint32_t bmask = (1u << (k - 2)) - 1u;
-        const uint32_t b = (hb + 1u) & bmask;
-        sl = bucket_resolve(K, bmask, b, kk, load_bucket(K, b));
+        const uint32_t smask = seg_mask(k);
+        const uint32_t b = next_bucket(hb, smask);
+        sl = bucket_resolve(K, smask, b, kk, load_bucket(K, b));
     }
     return sl;
 }
@@ -676,7 +927,7 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
     const PipeStage& ps = pd.st[s];
     const HJ_GLOBAL BuildJob* jp = gp(pd.jobs) + ps.job;
     const HJ_GLOBAL uint32_t* tkey = gp(jp->table);
-    const uint32_t k = table_bits(gp(jp->meta)[0]), bmask = (1u << (k - 2)) - 1u;
+    const uint32_t k = table_bits(gp(jp->meta)[0]), smask = seg_mask(k);
     const uint32_t special = gp(jp->meta)[2] ? (1u << k) : kDead;
     const HJ_GLOBAL uint32_t* prev = previous < 0 ? nullptr : gp(pd.l_b[previous]);
     const HJ_GLOBAL uint32_t* src =
@@ -713,7 +964,7 @@ __global__ __launch_bounds__(kB) void k_pp_stage(const PipeDesc* __restrict__ pd
             uint32_t slot = kDead;
             if (ok[u]) {
                 const uint32_t kk = (uint32_t)key[u];
-                slot = kk == kEmptyKey ? special : bucket_resolve(tkey, bmask, hb[u], kk, v[u]);
+                slot = kk == kEmptyKey ? special : bucket_resolve(tkey, smask, hb[u], kk, v[u]);
             }
             outb[jx[u]] = slot == kDead ? kDead : tkey[row_word(slot)];
         }
@@ -841,6 +1092,11 @@ uint32_t probe_resident_blocks(int ns, uint32_t num_cus) {
 
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s) {
     if (hb.njobs == 0) return;
+    if (hb.part) {
+        hipLaunchKernelGGL(k_sb_filter_part, dim3(hb.grid), dim3(kB), 0, s, db);
+        hipLaunchKernelGGL(k_sb_segbuild, dim3(hb.sgrid), dim3(kB), 0, s, db);
+        return;
+    }
     hipLaunchKernelGGL(k_sb_filter, dim3(hb.grid), dim3(kB), 0, s, db);
     hipLaunchKernelGGL(k_sb_clear, dim3(hb.cgrid, hb.njobs), dim3(kB), 0, s, db);
     hipLaunchKernelGGL(k_sb_insert, dim3(hb.grid), dim3(kB), 0, s, db);

@@ -31,6 +31,22 @@ __host__ __device__ inline uint32_t table_bits(uint32_t n) {
     return k + 2 < kMinTableBits ? kMinTableBits : k + 2;
 }
 
+// Linear probing runs over buckets within a segment of at most 2^kSegBits buckets (the next
+// bucket after a segment's last is its first), so that one workgroup can build a segment in LDS.
+constexpr uint32_t kSegBits = 10;
+__host__ __device__ inline uint32_t seg_mask(uint32_t k) {
+    return (k - 2 < kSegBits ? (1u << (k - 2)) : (1u << kSegBits)) - 1u;
+}
+__host__ __device__ inline uint32_t next_bucket(uint32_t b, uint32_t smask) {
+    return (b & ~smask) | ((b + 1u) & smask);
+}
+// fine hash partitions of a job whose table has at most 2^kmax key slots: segments nest in them
+__host__ __device__ inline uint32_t fine_bits(uint32_t kmax) {
+    return kmax - 2 > kSegBits ? kmax - 2 - kSegBits : 0u;
+}
+constexpr uint32_t kMaxFineBits = 12;   // partitioned builds: at most 4096 fine partitions ...
+constexpr uint32_t kMaxPartBlocks = 512;  // ... and 512 filter blocks per job
+
 // rows per block of the build's filter and insert passes
 constexpr uint32_t kBuildRows = 8192;
 
@@ -54,17 +70,20 @@ struct BuildJob {
     uint32_t* st_row;   // (capacity nblk * kBuildRows)
     uint32_t* blk;      // tuples staged per block of the filter pass
     uint32_t* table;    // 2 * 2^kmax + 8 words; 2^table_bits(tuples) key slots used
+    uint16_t* poff;     // partitioned build: per filter block, the start of each of the
+                        // 2^fine_bits(kmax) fine partitions in its staged range, then its total
     uint32_t* bf;       // blocked Bloom filter, kBloomWords words (small build sides only)
 };
 
 struct BuildBatch {
     BuildJob job[kMaxStages];
     uint32_t blk0[kMaxStages];  // job[j].blk0, ~0 for j >= njobs
+    int32_t part;               // partitioned build (filter + segment passes) instead of
+                                // filter + clear + insert
+    uint32_t sgrid;             // blocks of the segment pass (a grid-stride loop)
     int32_t njobs;
     uint32_t grid;      // blocks of the filter and insert passes (every job's nblk)
     uint32_t cgrid;     // blocks per job of the clear pass (grid-stride)
-    uint32_t nzero;     // words of `zero` cleared by the filter pass (<= 256): the jobs' and the
-    uint32_t* zero;     // pipeline's metas
 };
 
 struct PipeStage {

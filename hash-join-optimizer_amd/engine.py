@@ -103,7 +103,9 @@ class Context:
         return DeviceResult(self, h)
 
     def set_option(self, name, value):
-        """engine options: "pipeline" (1 = fused probe pipeline for left-deep chains, 0 = off)"""
+        """engine options: "pipeline" (1 = fused probe pipeline for left-deep chains, 0 = off),
+        "partitioned_build" (1 = pipeline tables built segment by segment in LDS, 0 = by
+        compare-and-swap inserts into cleared tables)"""
         self._check(self.lib.hj_set_option(self.handle, name.encode(), int(value)))
 
     def profile(self, prefix):

@@ -178,7 +178,10 @@ int hj_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint
 
 /* ---- engine options (no counterpart in the reference):
  *   "pipeline" 1 (default) / 0: run left-deep chains whose build sides are base scans as one
- *              fused probe pipeline; 0 forces the join-at-a-time path for every plan. ---- */
+ *              fused probe pipeline; 0 forces the join-at-a-time path for every plan.
+ *   "partitioned_build" 1 (default) / 0: the pipeline builds its hash tables segment by segment
+ *              in LDS (build sides of up to 4M rows); 0 uses device-wide compare-and-swap inserts
+ *              into cleared tables. ---- */
 int hj_set_option(hj_context* ctx, const char* name, int64_t value);
 
 /* ---- kernel timing (no counterpart in the reference): when enabled, every engine kernel launch

@@ -11,12 +11,15 @@ import hjtest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["pipeline", "join_at_a_time"])
+@pytest.fixture(params=["pipeline", "pipeline_cas", "join_at_a_time"])
 def mode(request, gpu_ctx):
-    """run each test with the fused probe pipeline enabled and disabled"""
-    gpu_ctx.set_option("pipeline", 1 if request.param == "pipeline" else 0)
+    """run each test with the fused probe pipeline (tables built segment by segment in LDS, or
+    by compare-and-swap inserts into cleared tables) and with join-at-a-time"""
+    gpu_ctx.set_option("pipeline", 0 if request.param == "join_at_a_time" else 1)
+    gpu_ctx.set_option("partitioned_build", 0 if request.param == "pipeline_cas" else 1)
     yield request.param
     gpu_ctx.set_option("pipeline", 1)
+    gpu_ctx.set_option("partitioned_build", 1)
 
 
 @pytest.mark.parametrize("case", hjtest.load_kat_cases(), ids=lambda c: c["name"])
