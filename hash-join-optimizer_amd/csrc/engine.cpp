@@ -514,6 +514,7 @@ class Executor {
                 jb.poff = scratch<uint16_t>((uint64_t)jb.nblk * (P + 1));
             }
         hb.sgrid = (uint32_t)ctx_->num_cus * 4;  // segment pass: a grid-stride loop
+        hb.jmeta = metas + 4;
 
         hp.nf = (int32_t)srel.filters.size();
         for (size_t f = 0; f < srel.filters.size(); ++f) {

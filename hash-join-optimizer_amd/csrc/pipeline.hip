@@ -90,7 +90,7 @@ __device__ inline u32x4 load_bucket(const HJ_GLOBAL uint32_t* K, uint32_t b) {
 // the search, and at load factor <= 1/4 the home bucket nearly always does
 __device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t smask, uint32_t b,
                                           uint32_t key, u32x4 v) {
-    for (;;) {
+    for (uint32_t m = 0; m <= smask; ++m) {  // at most the whole segment
         if (v.x == key) return 4u * b;
         if (v.y == key) return 4u * b + 1u;
         if (v.z == key) return 4u * b + 2u;
@@ -100,6 +100,7 @@ __device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t 
         b = next_bucket(b, smask);
         v = load_bucket(K, b);
     }
+    return kDead;
 }
 
 // ---------------------------------------------------------------- wave masks, buffer loads
@@ -380,13 +381,13 @@ __global__ __launch_bounds__(kB) void k_sb_segbuild(const BuildBatch* __restrict
     const uint32_t t = threadIdx.x;
     const int nj = bb->njobs;
     uint32_t first[kMaxStages + 1];  // first flat segment of each job
+    uint32_t nj_t[kMaxStages];       // tuples per job (all kMaxStages counters: independent loads)
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) nj_t[j] = gp(bb->jmeta)[4 * j];
     first[0] = 0;
 #pragma unroll
-    for (int j = 0; j < kMaxStages; ++j) {
-        const uint32_t nseg =
-            j < nj ? 1u << fine_bits(table_bits(gp(bb->job[j].meta)[0])) : 0u;
-        first[j + 1] = first[j] + nseg;
-    }
+    for (int j = 0; j < kMaxStages; ++j)
+        first[j + 1] = first[j] + (j < nj ? 1u << fine_bits(table_bits(nj_t[j])) : 0u);
     for (uint32_t f = blockIdx.x; f < first[kMaxStages]; f += gridDim.x) {
         int j = 0;
 #pragma unroll

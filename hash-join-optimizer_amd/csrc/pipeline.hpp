@@ -81,6 +81,7 @@ struct BuildBatch {
     int32_t part;               // partitioned build (filter + segment passes) instead of
                                 // filter + clear + insert
     uint32_t sgrid;             // blocks of the segment pass (a grid-stride loop)
+    const uint32_t* jmeta;      // the counters of kMaxStages jobs, 4 words each (job[j].meta)
     int32_t njobs;
     uint32_t grid;      // blocks of the filter and insert passes (every job's nblk)
     uint32_t cgrid;     // blocks per job of the clear pass (grid-stride)
