@@ -28,6 +28,7 @@
 #include "hj_internal.hpp"
 #include "kernels.hpp"
 #include "pagecodec.hpp"
+#include "pagewrite.hpp"
 #include "pipeline.hpp"
 #include "materialize.hpp"
 
@@ -999,8 +1000,96 @@ void free_dresult(hj_context* ctx, hj_dresult* r) {
     delete r;
 }
 
+// Device page encoder: plan every column's pages (one wave per column), read the page counts,
+// then write the pages in HBM in bounded batches and copy each batch into the result columns'
+// malloc'd page blocks. Byte-identical to the host writer (pagecodec.cpp) on the same columns.
+void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
+    constexpr uint64_t kStagePages = 32768;  // 256 MiB of pages per writer launch
+    const uint64_t n = r->num_rows;
+    const size_t ncols = r->cols.size();
+    out->num_rows = n;
+    out->num_columns = (uint32_t)ncols;
+    out->columns = (hj_out_column*)std::calloc(ncols ? ncols : 1, sizeof(hj_out_column));
+    if (!out->columns) throw std::bad_alloc();
+    Arena& A = ctx->arena;
+    const size_t mark = A.mark();
+    try {
+        for (size_t c0 = 0; c0 < ncols; c0 += kPageCols) {
+            const uint32_t nc = (uint32_t)std::min<size_t>(kPageCols, ncols - c0);
+            PageBatch pb{};
+            pb.ncol = nc;
+            pb.counts = (uint64_t*)A.scratch(kPageCols * sizeof(uint64_t));
+            for (uint32_t i = 0; i < nc; ++i) {
+                const auto& col = r->cols[c0 + i];
+                PageCol& pc = pb.c[i];
+                pc.type = col.type;
+                pc.n = n;
+                pc.validity = col.validity;
+                pc.values = col.values;
+                pc.offsets = col.offsets;
+                pc.chars = col.chars;
+                // every page holds at least one row or one 8188-byte piece of a long string
+                pc.cap = col.type == HJ_INT32 ? int32_page_bound(n)
+                                              : n + col.num_chars / kLongPiece + 2;
+                pc.refs = (PageRef*)A.scratch(pc.cap * sizeof(PageRef));
+            }
+            {
+                Prof p_(ctx, "page_plan");
+                page_plan(pb, ctx->stream);
+            }
+            HJ_HIP(hipMemcpyAsync(ctx->pinned, pb.counts, nc * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HJ_HIP(hipStreamSynchronize(ctx->stream));
+            uint64_t total = 0;
+            for (uint32_t i = 0; i < nc; ++i) {
+                const uint64_t cnt = ctx->pinned[i];
+                if (cnt > pb.c[i].cap) throw Error(HJ_E_INVALID, "page planner exceeded its bound");
+                pb.c[i].first = total;
+                total += cnt;
+                hj_out_column& oc = out->columns[c0 + i];
+                oc.type = pb.c[i].type;
+                oc.num_pages = (uint32_t)cnt;
+                if (cnt) {
+                    oc.pages = (uint8_t*)std::malloc(cnt * HJ_PAGE_SIZE);
+                    if (!oc.pages) throw std::bad_alloc();
+                }
+            }
+            if (!total) continue;
+            const uint64_t stage = std::min(total, kStagePages);
+            pb.pages = (uint8_t*)A.scratch(stage * kPageBytes);
+            for (uint64_t p0 = 0; p0 < total; p0 += stage) {
+                const uint64_t cnt = std::min(stage, total - p0);
+                {
+                    Prof p_(ctx, "page_write");
+                    page_write(pb, p0, cnt, ctx->stream);
+                }
+                // copy the batch into the columns it covers
+                for (uint32_t i = 0; i < nc; ++i) {
+                    const uint64_t a = std::max(p0, pb.c[i].first);
+                    const uint64_t e = std::min(p0 + cnt, pb.c[i].first + out->columns[c0 + i].num_pages);
+                    if (a >= e) continue;
+                    HJ_HIP(hipMemcpyAsync(out->columns[c0 + i].pages + (a - pb.c[i].first) * kPageBytes,
+                                          pb.pages + (a - p0) * kPageBytes, (e - a) * kPageBytes,
+                                          hipMemcpyDeviceToHost, ctx->stream));
+                }
+                HJ_HIP(hipStreamSynchronize(ctx->stream));
+            }
+            A.reset(mark);
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        A.reset(mark);
+        throw;
+    }
+    A.reset(mark);
+}
+
 void to_host(hj_context* ctx, const hj_dresult* r, hj_result* out) {
     HJ_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->gpu_pages) {
+        to_host_pages(ctx, r, out);
+        return;
+    }
     const uint64_t n = r->num_rows;
     out->num_rows = n;
     out->num_columns = (uint32_t)r->cols.size();
@@ -1116,6 +1205,10 @@ int hj_set_option(hj_context* ctx, const char* name, int64_t value) {
     }
     if (std::strcmp(name, "partitioned_build") == 0) {
         ctx->part_build = value != 0;
+        return HJ_OK;
+    }
+    if (std::strcmp(name, "gpu_pages") == 0) {
+        ctx->gpu_pages = value != 0;
         return HJ_OK;
     }
     ctx->last_error = std::string("unknown option ") + name;

@@ -146,6 +146,7 @@ struct hj_context {
     static constexpr size_t kDescOffset = 4096;
     bool use_pipeline = true;  // fused probe pipeline for left-deep chains (hj_set_option)
     bool part_build = true;    // pipeline builds partitioned in LDS (hj_set_option)
+    bool gpu_pages = true;     // result pages written on the device (hj_set_option)
     int num_cus = 256;
     // kernel timing (hj_profile_enable)
     bool prof_on = false;

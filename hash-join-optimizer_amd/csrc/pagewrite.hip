@@ -1,0 +1,299 @@
+// pagewrite.hip - device page encoder of the result columns (planner + writer, see pagewrite.hpp).
+//
+// Page rules restated from include/plan.h:265-396 and src/execute.cpp:447-569 (the host writer
+// in pagecodec.cpp follows the same rules). Counting the row being added, row j of a page fits
+//   INT32    while 4 + 4 * values + j / 8 + 1 <= 8192
+//   VARCHAR  while 4 + 2 * strings + chars + j / 8 + 1 <= 8192
+// Both left sides only grow with j, so the page ends before the first row whose cost exceeds
+// 8192, and that row opens the next page.
+#include <hip/hip_runtime.h>
+
+#include "../../include/hjoin.h"
+#include "hj_device.hpp"
+#include "pagewrite.hpp"
+
+namespace hj {
+namespace {
+
+constexpr int kWriteBlock = 256;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void put_ref(const PageCol& c, uint64_t i, uint64_t row0, uint32_t rows,
+                               uint32_t kind) {
+    if (i < c.cap) {
+        HJ_GLOBAL PageRef* r = gp(c.refs) + i;
+        r->row0 = row0;
+        r->rows = rows;
+        r->kind = kind;
+    }
+}
+
+// INT32: lane i holds validity word i of a 2048-row window (rows b + 32i .. b + 32i + 31).
+// g = valid rows in [s, b) of the open page starting at row s (negative: minus the valid rows
+// in [b, s) once s is inside the window).
+__device__ uint64_t plan_int32(const PageCol& c) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t n = c.n;
+    const HJ_GLOBAL uint32_t* vw = (const HJ_GLOBAL uint32_t*)gp(c.validity);
+    uint64_t np = 0, s = 0;
+    int64_t g = 0;
+    for (uint64_t b = 0; b < n; b += 2048) {
+        const uint64_t r0 = b + 32ull * lane;
+        uint32_t word = 0, nrow = 0;
+        if (r0 < n) {
+            nrow = (uint32_t)(n - r0 < 32 ? n - r0 : 32);
+            word = vw[r0 >> 5];
+            if (nrow < 32) word &= (1u << nrow) - 1u;
+        }
+        const uint32_t cnt = (uint32_t)__popc(word);
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t excl = incl - cnt;
+        const uint64_t e = r0 + nrow;
+        for (;;) {
+            // cost of the open page if it ended with this lane's last row
+            const bool cand = nrow && e > s;
+            const int64_t cost = 4 * (g + (int64_t)incl) + (int64_t)((e - 1 - s) >> 3) + 5;
+            const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
+            if (!m) break;
+            // the first row that does not fit lies in word L: lanes 0..31 test one row each
+            const uint32_t L = (uint32_t)__ffsll((unsigned long long)m) - 1;
+            const uint32_t wl = __shfl(word, L), xl = __shfl(excl, L);
+            const uint64_t rl = b + 32ull * L;
+            const uint64_t x = rl + lane;
+            const uint32_t upto = lane < 31 ? (2u << lane) - 1u : 0xffffffffu;
+            const int64_t cx =
+                4 * (g + (int64_t)(xl + __popc(wl & upto))) + (int64_t)((x - s) >> 3) + 5;
+            const uint64_t m2 = __ballot(lane < 32 && x >= s && x < n && cx > (int64_t)kPageBytes);
+            const uint32_t t = (uint32_t)__ffsll((unsigned long long)m2) - 1;
+            const uint64_t xb = rl + t;
+            if (lane == 0) put_ref(c, np, s, (uint32_t)(xb - s), 0);
+            ++np;
+            g = -(int64_t)(xl + __popc(wl & ((1u << t) - 1u)));
+            s = xb;
+        }
+        g += (int64_t)__shfl(incl, 63);
+    }
+    if (n > s) {
+        if (lane == 0) put_ref(c, np, s, (uint32_t)(n - s), 0);
+        ++np;
+    }
+    return np;
+}
+
+// VARCHAR: lane i holds row b + i. A short string adds 2 + len bytes, a NULL row only its bit.
+// g = bytes of [s, b) in the open page (negative once s is inside the window).
+__device__ uint64_t plan_varchar(const PageCol& c) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t n = c.n;
+    const HJ_GLOBAL uint8_t* vb = gp(c.validity);
+    const HJ_GLOBAL uint64_t* off = gp(c.offsets);
+    uint64_t np = 0, s = 0;
+    int64_t g = 0;
+    for (uint64_t b = 0; b < n; b += 64) {
+        const uint64_t r = b + lane;
+        const bool has = r < n;
+        const bool valid = has && ((vb[r >> 3] >> (r & 7)) & 1);
+        const uint64_t len = valid ? off[r + 1] - off[r] : 0;
+        const bool lng = valid && len > kLongStr;
+        const uint32_t a = (valid && !lng) ? 2u + (uint32_t)len : 0u;
+        const uint32_t A = wave_incl_scan(a);
+        for (;;) {
+            const uint64_t lm = __ballot(lng && r >= s);
+            const uint32_t lim = lm ? (uint32_t)__ffsll((unsigned long long)lm) - 1 : 64u;
+            const bool cand = has && r >= s && lane < lim;
+            const int64_t cost = 5 + g + (int64_t)A + (int64_t)((r - s) >> 3);
+            const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
+            if (m) {
+                const uint32_t L = (uint32_t)__ffsll((unsigned long long)m) - 1;
+                const uint64_t xb = b + L;
+                if (lane == 0) put_ref(c, np, s, (uint32_t)(xb - s), 0);
+                ++np;
+                g = -(int64_t)(__shfl(A, L) - __shfl(a, L));
+                s = xb;
+                continue;
+            }
+            if (lim == 64) break;
+            // a long string: close the open page, then one page per piece of 8188 bytes
+            const uint64_t rl = b + lim;
+            if (rl > s) {
+                if (lane == 0) put_ref(c, np, s, (uint32_t)(rl - s), 0);
+                ++np;
+            }
+            const uint64_t ll = __shfl(len, lim);
+            const uint64_t pieces = (ll + kLongPiece - 1) / kLongPiece;
+            for (uint64_t p0 = 0; p0 < pieces; p0 += 64) {
+                const uint64_t p = p0 + lane;
+                if (p < pieces) put_ref(c, np + p, rl, (uint32_t)p, p == 0 ? 1u : 2u);
+            }
+            np += pieces;
+            g = -(int64_t)__shfl(A, lim);
+            s = rl + 1;
+        }
+        g += (int64_t)__shfl(A, 63);
+    }
+    if (n > s) {
+        if (lane == 0) put_ref(c, np, s, (uint32_t)(n - s), 0);
+        ++np;
+    }
+    return np;
+}
+
+__global__ __launch_bounds__(64) void k_page_plan(PageBatch b) {
+    const PageCol& c = b.c[blockIdx.x];
+    const uint64_t np = c.type == HJ_INT32 ? plan_int32(c) : plan_varchar(c);
+    if (threadIdx.x == 0) gp(b.counts)[blockIdx.x] = np;
+}
+
+// 8 validity bits starting at bit q (bit i of byte i/8); nbytes = bytes holding the n rows
+__device__ inline uint32_t valid8(const HJ_GLOBAL uint8_t* vb, uint64_t nbytes, uint64_t q) {
+    const uint64_t k = q >> 3;
+    const uint32_t sh = (uint32_t)(q & 7);
+    uint32_t v = vb[k];
+    if (sh && k + 1 < nbytes) v |= (uint32_t)vb[k + 1] << 8;
+    return (v >> sh) & 0xffu;
+}
+
+// One block per page. Thread t handles rows f0 + 8t .. f0 + 8t + 7 of the page per 2048-row
+// step: their validity is bitmap byte (f0 / 8 + t) of the page.
+__global__ __launch_bounds__(kWriteBlock) void k_page_write(PageBatch b, uint64_t p0) {
+    __shared__ uint32_t pg[kPageBytes / 4];
+    __shared__ uint64_t s_src[kPageBytes / 2];  // VARCHAR: source offset of each string
+    __shared__ uint64_t s_red[kWriteBlock / 64];
+    uint8_t* pgb = (uint8_t*)pg;
+    const uint32_t t = threadIdx.x;
+    const uint64_t pid = p0 + blockIdx.x;
+    uint32_t ci = 0;
+    for (uint32_t i = 1; i < b.ncol; ++i) ci = pid >= b.c[i].first ? i : ci;
+    const PageCol& c = b.c[ci];
+    const HJ_GLOBAL PageRef* rp = gp(c.refs) + (pid - c.first);
+    PageRef ref;
+    ref.row0 = rp->row0;
+    ref.rows = rp->rows;
+    ref.kind = rp->kind;
+    for (uint32_t i = t; i < kPageBytes / 4; i += kWriteBlock) pg[i] = 0;
+    __syncthreads();
+    const HJ_GLOBAL uint8_t* vb = gp(c.validity);
+    const uint64_t nbytes = (c.n + 7) >> 3;
+    if (ref.kind) {
+        // piece ref.rows of the long string in row ref.row0
+        const HJ_GLOBAL uint64_t* off = gp(c.offsets);
+        const uint64_t st = off[ref.row0], len = off[ref.row0 + 1] - st;
+        const uint64_t po = (uint64_t)ref.rows * kLongPiece;
+        const uint32_t l = (uint32_t)(len - po < kLongPiece ? len - po : kLongPiece);
+        if (t == 0) pg[0] = (ref.kind == 1 ? 0xffffu : 0xfffeu) | (l << 16);
+        const HJ_GLOBAL char* src = gp(c.chars) + st + po;
+        for (uint32_t x = t; x < l; x += kWriteBlock) pgb[4 + x] = (uint8_t)src[x];
+    } else if (c.type == HJ_INT32) {
+        const uint64_t r0 = ref.row0;
+        const uint32_t R = ref.rows, nb = (R + 7) / 8;
+        const HJ_GLOBAL int32_t* vals = gp(c.values);
+        uint32_t nv = 0;
+        for (uint32_t f0 = 0; f0 < R; f0 += 8 * kWriteBlock) {
+            const uint32_t f = f0 + 8 * t;
+            uint32_t byte = 0;
+            if (f < R) {
+                byte = valid8(vb, nbytes, r0 + f);
+                if (R - f < 8) byte &= (1u << (R - f)) - 1u;
+                pgb[kPageBytes - nb + (f >> 3)] = (uint8_t)byte;
+            }
+            uint64_t tot;
+            uint32_t k = nv + (uint32_t)block_excl_scan<kWriteBlock>((uint64_t)__popc(byte), s_red, &tot);
+            int32_t v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = ((byte >> i) & 1) ? vals[r0 + f + i] : 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if ((byte >> i) & 1) pg[1 + k++] = (uint32_t)v[i];
+            nv += (uint32_t)tot;
+        }
+        if (t == 0) pg[0] = R | (nv << 16);
+    } else {
+        const uint64_t r0 = ref.row0;
+        const uint32_t R = ref.rows, nb = (R + 7) / 8;
+        const HJ_GLOBAL uint64_t* off = gp(c.offsets);
+        // strings on the page first: the chars start after their end offsets
+        uint32_t ns = 0;
+        for (uint32_t f0 = 0; f0 < R; f0 += 8 * kWriteBlock) {
+            const uint32_t f = f0 + 8 * t;
+            uint32_t byte = 0;
+            if (f < R) {
+                byte = valid8(vb, nbytes, r0 + f);
+                if (R - f < 8) byte &= (1u << (R - f)) - 1u;
+                pgb[kPageBytes - nb + (f >> 3)] = (uint8_t)byte;
+            }
+            ns += (uint32_t)block_sum<kWriteBlock>((uint64_t)__popc(byte), s_red);
+        }
+        // end offsets (u16 at byte 4 + 2i) and source offsets of the strings, in row order
+        uint32_t si = 0, cp = 0;
+        for (uint32_t f0 = 0; f0 < R; f0 += 8 * kWriteBlock) {
+            const uint32_t f = f0 + 8 * t;
+            uint32_t byte = 0;
+            uint64_t st[8];
+            uint32_t ln[8];
+            uint32_t sum = 0;
+            if (f < R) {
+                byte = valid8(vb, nbytes, r0 + f);
+                if (R - f < 8) byte &= (1u << (R - f)) - 1u;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                st[i] = 0;
+                ln[i] = 0;
+                if ((byte >> i) & 1) {
+                    st[i] = off[r0 + f + i];
+                    ln[i] = (uint32_t)(off[r0 + f + i + 1] - st[i]);
+                }
+                sum += ln[i];
+            }
+            uint64_t tot;
+            const uint64_t ex = block_excl_scan<kWriteBlock>(
+                ((uint64_t)sum << 32) | (uint64_t)__popc(byte), s_red, &tot);
+            uint32_t k = si + (uint32_t)ex, e = cp + (uint32_t)(ex >> 32);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if ((byte >> i) & 1) {
+                    e += ln[i];
+                    *(uint16_t*)(pgb + 4 + 2 * k) = (uint16_t)e;
+                    s_src[k] = st[i];
+                    ++k;
+                }
+            }
+            si += (uint32_t)tot;
+            cp += (uint32_t)(tot >> 32);
+        }
+        if (t == 0) pg[0] = R | (ns << 16);
+        __syncthreads();
+        const uint16_t* ends = (const uint16_t*)(pgb + 4);
+        const HJ_GLOBAL char* chars = gp(c.chars);
+        uint8_t* dst = pgb + 4 + 2 * ns;
+        for (uint32_t x = t; x < cp; x += kWriteBlock) {
+            // string holding page char x: the first whose end offset exceeds x
+            uint32_t lo = 0, hi = ns - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (ends[mid] > x) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t s0 = lo ? ends[lo - 1] : 0u;
+            dst[x] = (uint8_t)chars[s_src[lo] + (x - s0)];
+        }
+    }
+    __syncthreads();
+    HJ_GLOBAL u32x4* out = (HJ_GLOBAL u32x4*)gp(b.pages + (uint64_t)blockIdx.x * kPageBytes);
+    const u32x4* src = (const u32x4*)pg;
+    for (uint32_t i = t; i < kPageBytes / 16; i += kWriteBlock) out[i] = src[i];
+}
+
+}  // namespace
+
+void page_plan(const PageBatch& b, hipStream_t s) {
+    if (!b.ncol) return;
+    hipLaunchKernelGGL(k_page_plan, dim3(b.ncol), dim3(64), 0, s, b);
+}
+
+void page_write(const PageBatch& b, uint64_t p0, uint64_t count, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_page_write, dim3((uint32_t)count), dim3(kWriteBlock), 0, s, b, p0);
+}
+
+}  // namespace hj

@@ -105,7 +105,8 @@ class Context:
     def set_option(self, name, value):
         """engine options: "pipeline" (1 = fused probe pipeline for left-deep chains, 0 = off),
         "partitioned_build" (1 = pipeline tables built segment by segment in LDS, 0 = by
-        compare-and-swap inserts into cleared tables)"""
+        compare-and-swap inserts into cleared tables), "gpu_pages" (1 = result pages written on
+        the device, 0 = flat columns copied out and paged on the host; same bytes)"""
         self._check(self.lib.hj_set_option(self.handle, name.encode(), int(value)))
 
     def profile(self, prefix):

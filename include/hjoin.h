@@ -181,7 +181,10 @@ int hj_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint
  *              fused probe pipeline; 0 forces the join-at-a-time path for every plan.
  *   "partitioned_build" 1 (default) / 0: the pipeline builds its hash tables segment by segment
  *              in LDS (build sides of up to 4M rows); 0 uses device-wide compare-and-swap inserts
- *              into cleared tables. ---- */
+ *              into cleared tables.
+ *   "gpu_pages" 1 (default) / 0: hj_execute and hj_dresult_to_host write the result pages on the
+ *              device and copy them out; 0 copies the flat columns and pages them on the host.
+ *              Both give the same bytes. ---- */
 int hj_set_option(hj_context* ctx, const char* name, int64_t value);
 
 /* ---- kernel timing (no counterpart in the reference): when enabled, every engine kernel launch
