@@ -9,7 +9,9 @@ time; pct_hbm_peak = algorithmic bytes / step time / 8 TB/s (keys and filter att
 validity bits, output columns gathered and written once). The roofline object is the dominant
 kernel timed with HIP events on the engine's stream during the timed steps. cpu_baseline is the
 oracle (a CPU restatement of the reference's execute()) on the host, timed around execute() with
-pre-filtered dimensions, as the reference filters before execute().
+pre-filtered dimensions, as the reference filters before execute(). result_pages times the copy of
+the result into the reference's page format (outside the timed steps): pages written on the
+device against flat columns paged on the host.
 """
 import argparse
 import ctypes as C
@@ -130,6 +132,39 @@ def measured_traffic(kernel, workload_name, scale):
     return None
 
 
+# ------------------------------------------------------------------ result pages
+def result_pages(ctx, plan, dts, reps=3):
+    """wall time of hj_dresult_to_host (the result's 8 KiB pages into host memory) with the
+    pages written on the device (gpu_pages = 1) and with flat columns paged on the host (0); the
+    page kernels' own time from HIP events"""
+    lib = ctx.lib
+    r = ctx.execute_device(plan, dts)
+    ctx.synchronize()
+    out = {"rows": r.num_rows}
+    for mode in (1, 0):
+        ctx.set_option("gpu_pages", mode)
+        if mode:
+            ctx.profile("page_")
+        ts = []
+        for _ in range(reps):
+            res = hjo.abi.hj_result()
+            t0 = time.perf_counter()
+            ctx._check(lib.hj_dresult_to_host(ctx.handle, r.handle, C.byref(res)))
+            ts.append(time.perf_counter() - t0)
+            pages = sum(res.columns[j].num_pages for j in range(res.num_columns))
+            lib.hj_result_free(C.byref(res))
+        if mode:
+            prof = ctx.profile_report()
+            ctx.profile(None)
+            out["kernel_ms"] = {k: ms / n for k, (n, ms) in prof.items()}
+        out["pages"] = pages
+        out["gpu_pages_ms" if mode else "host_pages_ms"] = min(ts) * 1e3
+    ctx.set_option("gpu_pages", 1)
+    r.free()
+    out["bytes"] = out["pages"] * 8192
+    return out
+
+
 # ------------------------------------------------------------------ CPU baseline (oracle)
 def cpu_baseline(d, workload_name, tuples, threads):
     """the oracle (CPU restatement of the reference's execute()), used only as the baseline"""
@@ -224,6 +259,9 @@ def main():
                     "traffic": measured_traffic(dom, args.workload, args.scale),
                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_s * 1e3}
 
+    pages = result_pages(ctx, plan, dts)
+    log(f"result pages: {pages}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -237,7 +275,7 @@ def main():
         "pct_hbm_peak": 100.0 * info["b_alg"] / step_s / HBM_PEAK,
         "alg_bytes_per_step": info["b_alg"], "output_rows_per_s": info["out_rows"] / step_s,
         "kernel_ms_per_step": {k: v[1] for k, v in per_step.items()},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "result_pages": pages,
     }
     for t_ in dts:
         t_.free()

@@ -21,6 +21,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -1000,17 +1001,53 @@ void free_dresult(hj_context* ctx, hj_dresult* r) {
     delete r;
 }
 
+// copies segments (dst, src, bytes) with up to 8 threads for large totals
+struct CopySeg {
+    uint8_t* dst;
+    const uint8_t* src;
+    size_t bytes;
+};
+void parallel_copy(const std::vector<CopySeg>& segs) {
+    size_t total = 0;
+    for (const auto& g : segs) total += g.bytes;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = (unsigned)std::min<size_t>({8, hw, total / (4u << 20) + 1});
+    auto run = [&segs](size_t lo, size_t hi) {  // bytes [lo, hi) of the concatenated segments
+        size_t base = 0;
+        for (const auto& g : segs) {
+            const size_t a = std::max(lo, base), e = std::min(hi, base + g.bytes);
+            if (a < e) std::memcpy(g.dst + (a - base), g.src + (a - base), e - a);
+            base += g.bytes;
+        }
+    };
+    if (nt <= 1) {
+        run(0, total);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t step = (total + nt - 1) / nt;
+    for (unsigned t = 1; t < nt; ++t)
+        th.emplace_back(run, std::min(total, t * step), std::min(total, (t + 1) * step));
+    run(0, std::min(total, step));
+    for (auto& x : th) x.join();
+}
+
 // Device page encoder: plan every column's pages (one wave per column), read the page counts,
-// then write the pages in HBM in bounded batches and copy each batch into the result columns'
+// then write the pages in HBM in batches of up to 32 MiB. Each batch goes to one half of a
+// pinned double buffer while the host copies the previous batch into the result columns'
 // malloc'd page blocks. Byte-identical to the host writer (pagecodec.cpp) on the same columns.
 void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
-    constexpr uint64_t kStagePages = 32768;  // 256 MiB of pages per writer launch
+    constexpr uint64_t kStagePages = hj_context::kPageStageHalf / kPageBytes;
     const uint64_t n = r->num_rows;
     const size_t ncols = r->cols.size();
     out->num_rows = n;
     out->num_columns = (uint32_t)ncols;
     out->columns = (hj_out_column*)std::calloc(ncols ? ncols : 1, sizeof(hj_out_column));
     if (!out->columns) throw std::bad_alloc();
+    if (!ctx->page_stage) {
+        HJ_HIP(hipHostMalloc(&ctx->page_stage, 2 * hj_context::kPageStageHalf, hipHostMallocDefault));
+        for (auto& e : ctx->page_ev) HJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     Arena& A = ctx->arena;
     const size_t mark = A.mark();
     try {
@@ -1018,7 +1055,9 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
             const uint32_t nc = (uint32_t)std::min<size_t>(kPageCols, ncols - c0);
             PageBatch pb{};
             pb.ncol = nc;
-            pb.counts = (uint64_t*)A.scratch(kPageCols * sizeof(uint64_t));
+            pb.counts = (uint64_t*)A.scratch(2 * kPageCols * sizeof(uint64_t));
+            pb.valid_rows = pb.counts + kPageCols;
+            HJ_HIP(hipMemsetAsync(pb.valid_rows, 0, kPageCols * sizeof(uint64_t), ctx->stream));
             for (uint32_t i = 0; i < nc; ++i) {
                 const auto& col = r->cols[c0 + i];
                 PageCol& pc = pb.c[i];
@@ -1035,7 +1074,7 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
             }
             {
                 Prof p_(ctx, "page_plan");
-                page_plan(pb, ctx->stream);
+                page_plan(pb, n, ctx->stream);
             }
             HJ_HIP(hipMemcpyAsync(ctx->pinned, pb.counts, nc * sizeof(uint64_t),
                                   hipMemcpyDeviceToHost, ctx->stream));
@@ -1056,23 +1095,36 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
             }
             if (!total) continue;
             const uint64_t stage = std::min(total, kStagePages);
+            const uint64_t nbatch = (total + stage - 1) / stage;
             pb.pages = (uint8_t*)A.scratch(stage * kPageBytes);
-            for (uint64_t p0 = 0; p0 < total; p0 += stage) {
-                const uint64_t cnt = std::min(stage, total - p0);
+            auto issue = [&](uint64_t k) {
+                const uint64_t p0 = k * stage, cnt = std::min(stage, total - p0);
                 {
                     Prof p_(ctx, "page_write");
                     page_write(pb, p0, cnt, ctx->stream);
                 }
-                // copy the batch into the columns it covers
+                HJ_HIP(hipMemcpyAsync(ctx->page_stage + (k & 1) * hj_context::kPageStageHalf,
+                                      pb.pages, cnt * kPageBytes, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+                HJ_HIP(hipEventRecord(ctx->page_ev[k & 1], ctx->stream));
+            };
+            issue(0);
+            for (uint64_t k = 0; k < nbatch; ++k) {
+                // half (k + 1) & 1 was last read by the host copy of batch k - 1, finished above
+                if (k + 1 < nbatch) issue(k + 1);
+                HJ_HIP(hipEventSynchronize(ctx->page_ev[k & 1]));
+                const uint64_t p0 = k * stage, cnt = std::min(stage, total - p0);
+                const uint8_t* src = ctx->page_stage + (k & 1) * hj_context::kPageStageHalf;
+                std::vector<CopySeg> segs;
                 for (uint32_t i = 0; i < nc; ++i) {
                     const uint64_t a = std::max(p0, pb.c[i].first);
-                    const uint64_t e = std::min(p0 + cnt, pb.c[i].first + out->columns[c0 + i].num_pages);
-                    if (a >= e) continue;
-                    HJ_HIP(hipMemcpyAsync(out->columns[c0 + i].pages + (a - pb.c[i].first) * kPageBytes,
-                                          pb.pages + (a - p0) * kPageBytes, (e - a) * kPageBytes,
-                                          hipMemcpyDeviceToHost, ctx->stream));
+                    const uint64_t e =
+                        std::min(p0 + cnt, pb.c[i].first + out->columns[c0 + i].num_pages);
+                    if (a < e)
+                        segs.push_back({out->columns[c0 + i].pages + (a - pb.c[i].first) * kPageBytes,
+                                        src + (a - p0) * kPageBytes, (size_t)((e - a) * kPageBytes)});
                 }
-                HJ_HIP(hipStreamSynchronize(ctx->stream));
+                parallel_copy(segs);
             }
             A.reset(mark);
         }
@@ -1189,6 +1241,9 @@ int hj_context_destroy(hj_context* ctx) {
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->page_stage) (void)hipHostFree(ctx->page_stage);
+    for (auto e : ctx->page_ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->arena_mem) (void)hipFree(ctx->arena_mem);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -8,6 +8,8 @@
 // 8192, and that row opens the next page.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "../../include/hjoin.h"
 #include "hj_device.hpp"
 #include "pagewrite.hpp"
@@ -28,119 +30,241 @@ __device__ inline void put_ref(const PageCol& c, uint64_t i, uint64_t row0, uint
     }
 }
 
-// INT32: lane i holds validity word i of a 2048-row window (rows b + 32i .. b + 32i + 31).
-// g = valid rows in [s, b) of the open page starting at row s (negative: minus the valid rows
-// in [b, s) once s is inside the window).
+// inclusive prefix sum over the wave with DPP moves: shifts by 1, 2, 4, 8 within each row of 16
+// lanes, then the row totals broadcast from lane 15 and lane 31
+__device__ inline uint32_t wave_scan(uint32_t x) {
+    int v = (int)x;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 -> rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 -> rows 2, 3
+    return (uint32_t)v;
+}
+
+__device__ inline uint32_t lane_u32(uint32_t x, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)l);
+}
+__device__ inline uint64_t lane_u64(uint64_t x, uint32_t l) {
+    return (uint64_t)lane_u32((uint32_t)x, l) | ((uint64_t)lane_u32((uint32_t)(x >> 32), l) << 32);
+}
+__device__ inline uint32_t first_lane(uint64_t m) {
+    return (uint32_t)__builtin_ctzll(m);
+}
+
+// Planner state of one column: the open page starts at row s; g = its bytes (VARCHAR) or valid
+// rows (INT32) in [s, b) for the window starting at row b (negative: minus those in [b, s) once
+// s is inside the window); np = pages cut so far.
+struct PlanState {
+    uint64_t s = 0;
+    int64_t g = 0;
+    uint64_t np = 0;
+};
+
+// INT32 window of 2048 rows: lane i holds validity word i (rows b + 32i .. b + 32i + 31)
+__device__ inline void int32_window(const PageCol& c, PlanState& st, uint64_t b, uint32_t vword) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t n = c.n;
+    const uint64_t r0 = b + 32ull * lane;
+    uint32_t word = 0, nrow = 0;
+    if (r0 < n) {
+        nrow = (uint32_t)(n - r0 < 32 ? n - r0 : 32);
+        word = nrow < 32 ? vword & ((1u << nrow) - 1u) : vword;
+    }
+    const uint32_t cnt = (uint32_t)__popc(word);
+    const uint32_t incl = wave_scan(cnt);
+    const uint32_t excl = incl - cnt;
+    const uint64_t e = r0 + nrow;
+    for (;;) {
+        // cost of the open page if it ended with this lane's last row
+        const bool cand = nrow && e > st.s;
+        const int64_t cost = 4 * (st.g + (int64_t)incl) + (int64_t)((e - 1 - st.s) >> 3) + 5;
+        const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
+        if (!m) break;
+        // the first row that does not fit lies in word L: lanes 0..31 test one row each
+        const uint32_t L = first_lane(m);
+        const uint32_t wl = lane_u32(word, L), xl = lane_u32(excl, L);
+        const uint64_t rl = b + 32ull * L;
+        const uint64_t x = rl + lane;
+        const uint32_t upto = lane < 31 ? (2u << lane) - 1u : 0xffffffffu;
+        const int64_t cx =
+            4 * (st.g + (int64_t)(xl + __popc(wl & upto))) + (int64_t)((x - st.s) >> 3) + 5;
+        const uint64_t m2 = __ballot(lane < 32 && x >= st.s && x < n && cx > (int64_t)kPageBytes);
+        const uint32_t t = first_lane(m2);
+        const uint64_t xb = rl + t;
+        if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(xb - st.s), 0);
+        ++st.np;
+        st.g = -(int64_t)(xl + __popc(wl & ((1u << t) - 1u)));
+        st.s = xb;
+    }
+    st.g += (int64_t)lane_u32(incl, 63);
+}
+
+// VARCHAR window of 64 rows: lane i holds row b + i. A short string adds 2 + len bytes, a NULL
+// row only its bit; a string longer than 8185 bytes closes the open page and takes pages of its
+// own.
+__device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t b, uint32_t vbyte,
+                                      uint64_t o0, uint64_t o1) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t n = c.n;
+    const uint64_t r = b + lane;
+    const bool has = r < n;
+    const bool valid = has && ((vbyte >> (r & 7)) & 1);
+    const uint64_t len = valid ? o1 - o0 : 0;
+    const bool lng = valid && len > kLongStr;
+    const uint32_t a = (valid && !lng) ? 2u + (uint32_t)len : 0u;
+    const uint32_t A = wave_scan(a);
+    for (;;) {
+        const uint64_t lm = __ballot(lng && r >= st.s);
+        const uint32_t lim = lm ? first_lane(lm) : 64u;
+        const bool cand = has && r >= st.s && lane < lim;
+        const int64_t cost = 5 + st.g + (int64_t)A + (int64_t)((r - st.s) >> 3);
+        const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
+        if (m) {
+            const uint32_t L = first_lane(m);
+            const uint64_t xb = b + L;
+            if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(xb - st.s), 0);
+            ++st.np;
+            st.g = -(int64_t)(lane_u32(A, L) - lane_u32(a, L));
+            st.s = xb;
+            continue;
+        }
+        if (lim == 64) break;
+        const uint64_t rl = b + lim;
+        if (rl > st.s) {
+            if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(rl - st.s), 0);
+            ++st.np;
+        }
+        const uint64_t ll = lane_u64(len, lim);
+        const uint64_t pieces = (ll + kLongPiece - 1) / kLongPiece;
+        for (uint64_t p0 = 0; p0 < pieces; p0 += 64) {
+            const uint64_t p = p0 + lane;
+            if (p < pieces) put_ref(c, st.np + p, rl, (uint32_t)p, p == 0 ? 1u : 2u);
+        }
+        st.np += pieces;
+        st.g = -(int64_t)lane_u32(A, lim);
+        st.s = rl + 1;
+    }
+    st.g += (int64_t)lane_u32(A, 63);
+}
+
+// The cut itself is a short chain of wave operations per window, so the loads run a group of
+// windows ahead: the next group's data is requested before the current group is cut.
+constexpr int kIntGroup = 16;  // INT32 windows per group (32768 rows)
+constexpr int kStrGroup = 8;   // VARCHAR windows per group (512 rows)
+
 __device__ uint64_t plan_int32(const PageCol& c) {
     const uint32_t lane = threadIdx.x;
     const uint64_t n = c.n;
     const HJ_GLOBAL uint32_t* vw = (const HJ_GLOBAL uint32_t*)gp(c.validity);
-    uint64_t np = 0, s = 0;
-    int64_t g = 0;
-    for (uint64_t b = 0; b < n; b += 2048) {
-        const uint64_t r0 = b + 32ull * lane;
-        uint32_t word = 0, nrow = 0;
-        if (r0 < n) {
-            nrow = (uint32_t)(n - r0 < 32 ? n - r0 : 32);
-            word = vw[r0 >> 5];
-            if (nrow < 32) word &= (1u << nrow) - 1u;
-        }
-        const uint32_t cnt = (uint32_t)__popc(word);
-        const uint32_t incl = wave_incl_scan(cnt);
-        const uint32_t excl = incl - cnt;
-        const uint64_t e = r0 + nrow;
-        for (;;) {
-            // cost of the open page if it ended with this lane's last row
-            const bool cand = nrow && e > s;
-            const int64_t cost = 4 * (g + (int64_t)incl) + (int64_t)((e - 1 - s) >> 3) + 5;
-            const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
-            if (!m) break;
-            // the first row that does not fit lies in word L: lanes 0..31 test one row each
-            const uint32_t L = (uint32_t)__ffsll((unsigned long long)m) - 1;
-            const uint32_t wl = __shfl(word, L), xl = __shfl(excl, L);
-            const uint64_t rl = b + 32ull * L;
-            const uint64_t x = rl + lane;
-            const uint32_t upto = lane < 31 ? (2u << lane) - 1u : 0xffffffffu;
-            const int64_t cx =
-                4 * (g + (int64_t)(xl + __popc(wl & upto))) + (int64_t)((x - s) >> 3) + 5;
-            const uint64_t m2 = __ballot(lane < 32 && x >= s && x < n && cx > (int64_t)kPageBytes);
-            const uint32_t t = (uint32_t)__ffsll((unsigned long long)m2) - 1;
-            const uint64_t xb = rl + t;
-            if (lane == 0) put_ref(c, np, s, (uint32_t)(xb - s), 0);
-            ++np;
-            g = -(int64_t)(xl + __popc(wl & ((1u << t) - 1u)));
-            s = xb;
-        }
-        g += (int64_t)__shfl(incl, 63);
+    constexpr uint64_t kRows = 2048ull * kIntGroup;
+    uint32_t cur[kIntGroup], nxt[kIntGroup];
+#pragma unroll
+    for (int j = 0; j < kIntGroup; ++j) {
+        const uint64_t r0 = 2048ull * j + 32ull * lane;
+        nxt[j] = r0 < n ? vw[r0 >> 5] : 0u;
     }
-    if (n > s) {
-        if (lane == 0) put_ref(c, np, s, (uint32_t)(n - s), 0);
-        ++np;
+    PlanState st;
+    for (uint64_t B = 0; B < n; B += kRows) {
+#pragma unroll
+        for (int j = 0; j < kIntGroup; ++j) cur[j] = nxt[j];
+#pragma unroll
+        for (int j = 0; j < kIntGroup; ++j) {
+            const uint64_t r0 = B + kRows + 2048ull * j + 32ull * lane;
+            if (r0 < n) nxt[j] = vw[r0 >> 5];
+        }
+#pragma unroll
+        for (int j = 0; j < kIntGroup; ++j)
+            if (B + 2048ull * j < n) int32_window(c, st, B + 2048ull * j, cur[j]);
     }
-    return np;
+    if (n > st.s) {
+        if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(n - st.s), 0);
+        ++st.np;
+    }
+    return st.np;
 }
 
-// VARCHAR: lane i holds row b + i. A short string adds 2 + len bytes, a NULL row only its bit.
-// g = bytes of [s, b) in the open page (negative once s is inside the window).
 __device__ uint64_t plan_varchar(const PageCol& c) {
     const uint32_t lane = threadIdx.x;
     const uint64_t n = c.n;
     const HJ_GLOBAL uint8_t* vb = gp(c.validity);
     const HJ_GLOBAL uint64_t* off = gp(c.offsets);
-    uint64_t np = 0, s = 0;
-    int64_t g = 0;
-    for (uint64_t b = 0; b < n; b += 64) {
-        const uint64_t r = b + lane;
-        const bool has = r < n;
-        const bool valid = has && ((vb[r >> 3] >> (r & 7)) & 1);
-        const uint64_t len = valid ? off[r + 1] - off[r] : 0;
-        const bool lng = valid && len > kLongStr;
-        const uint32_t a = (valid && !lng) ? 2u + (uint32_t)len : 0u;
-        const uint32_t A = wave_incl_scan(a);
-        for (;;) {
-            const uint64_t lm = __ballot(lng && r >= s);
-            const uint32_t lim = lm ? (uint32_t)__ffsll((unsigned long long)lm) - 1 : 64u;
-            const bool cand = has && r >= s && lane < lim;
-            const int64_t cost = 5 + g + (int64_t)A + (int64_t)((r - s) >> 3);
-            const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
-            if (m) {
-                const uint32_t L = (uint32_t)__ffsll((unsigned long long)m) - 1;
-                const uint64_t xb = b + L;
-                if (lane == 0) put_ref(c, np, s, (uint32_t)(xb - s), 0);
-                ++np;
-                g = -(int64_t)(__shfl(A, L) - __shfl(a, L));
-                s = xb;
-                continue;
-            }
-            if (lim == 64) break;
-            // a long string: close the open page, then one page per piece of 8188 bytes
-            const uint64_t rl = b + lim;
-            if (rl > s) {
-                if (lane == 0) put_ref(c, np, s, (uint32_t)(rl - s), 0);
-                ++np;
-            }
-            const uint64_t ll = __shfl(len, lim);
-            const uint64_t pieces = (ll + kLongPiece - 1) / kLongPiece;
-            for (uint64_t p0 = 0; p0 < pieces; p0 += 64) {
-                const uint64_t p = p0 + lane;
-                if (p < pieces) put_ref(c, np + p, rl, (uint32_t)p, p == 0 ? 1u : 2u);
-            }
-            np += pieces;
-            g = -(int64_t)__shfl(A, lim);
-            s = rl + 1;
+    constexpr uint64_t kRows = 64ull * kStrGroup;
+    uint32_t cb[kStrGroup], nb[kStrGroup];
+    uint64_t c0[kStrGroup], c1[kStrGroup], n0[kStrGroup], n1[kStrGroup];
+#pragma unroll
+    for (int j = 0; j < kStrGroup; ++j) {
+        const uint64_t r = 64ull * j + lane;
+        nb[j] = 0;
+        n0[j] = n1[j] = 0;
+        if (r < n) {
+            nb[j] = vb[r >> 3];
+            n0[j] = off[r];
+            n1[j] = off[r + 1];
         }
-        g += (int64_t)__shfl(A, 63);
     }
-    if (n > s) {
-        if (lane == 0) put_ref(c, np, s, (uint32_t)(n - s), 0);
-        ++np;
+    PlanState st;
+    for (uint64_t B = 0; B < n; B += kRows) {
+#pragma unroll
+        for (int j = 0; j < kStrGroup; ++j) {
+            cb[j] = nb[j];
+            c0[j] = n0[j];
+            c1[j] = n1[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kStrGroup; ++j) {
+            const uint64_t r = B + kRows + 64ull * j + lane;
+            if (r < n) {
+                nb[j] = vb[r >> 3];
+                n0[j] = off[r];
+                n1[j] = off[r + 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kStrGroup; ++j)
+            if (B + 64ull * j < n) varchar_window(c, st, B + 64ull * j, cb[j], c0[j], c1[j]);
     }
-    return np;
+    if (n > st.s) {
+        if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(n - st.s), 0);
+        ++st.np;
+    }
+    return st.np;
+}
+
+// valid rows of each INT32 column (blockIdx.y), added into b.valid_rows (zeroed by the caller)
+__global__ __launch_bounds__(256) void k_page_valid(PageBatch b) {
+    __shared__ uint64_t red[4];
+    const PageCol c = b.c[blockIdx.y];
+    if (c.type != HJ_INT32) return;
+    const HJ_GLOBAL uint32_t* vw = (const HJ_GLOBAL uint32_t*)gp(c.validity);
+    const uint64_t words = (c.n + 31) / 32;
+    uint64_t cnt = 0;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = vw[w];
+        const uint64_t r0 = 32 * w;
+        if (c.n - r0 < 32) v &= (1u << (c.n - r0)) - 1u;
+        cnt += (uint64_t)__popc(v);
+    }
+    cnt = block_sum<256>(cnt, red);
+    if (threadIdx.x == 0 && cnt) atomicAdd((unsigned long long*)(b.valid_rows + blockIdx.y),
+                                           (unsigned long long)cnt);
 }
 
 __global__ __launch_bounds__(64) void k_page_plan(PageBatch b) {
-    const PageCol& c = b.c[blockIdx.x];
-    const uint64_t np = c.type == HJ_INT32 ? plan_int32(c) : plan_varchar(c);
+    const PageCol c = b.c[blockIdx.x];
+    uint64_t np;
+    if (c.type == HJ_INT32 && gp(b.valid_rows)[blockIdx.x] == c.n) {
+        // no NULL: every page but the last holds exactly 1984 rows
+        np = (c.n + 1983) / 1984;
+        for (uint64_t p = threadIdx.x; p < np; p += 64) {
+            const uint64_t r0 = 1984 * p;
+            put_ref(c, p, r0, (uint32_t)(c.n - r0 < 1984 ? c.n - r0 : 1984), 0);
+        }
+    } else {
+        np = c.type == HJ_INT32 ? plan_int32(c) : plan_varchar(c);
+    }
     if (threadIdx.x == 0) gp(b.counts)[blockIdx.x] = np;
 }
 
@@ -286,8 +410,11 @@ __global__ __launch_bounds__(kWriteBlock) void k_page_write(PageBatch b, uint64_
 
 }  // namespace
 
-void page_plan(const PageBatch& b, hipStream_t s) {
+void page_plan(const PageBatch& b, uint64_t max_rows, hipStream_t s) {
     if (!b.ncol) return;
+    const uint64_t words = (max_rows + 31) / 32;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, words / 2048));
+    hipLaunchKernelGGL(k_page_valid, dim3(blocks, b.ncol), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_page_plan, dim3(b.ncol), dim3(64), 0, s, b);
 }
 

@@ -3,6 +3,8 @@
 // The reference writes its output ColumnarTable on the host, page by page, after the last join
 // (table_to_columnar, src/execute.cpp:431-573, page rules include/plan.h:265-396). Here the flat
 // device result columns are encoded in HBM and copied out page by page:
+//   k_page_valid  counts each INT32 column's valid rows; a column without NULL is cut by
+//                 arithmetic (every page but the last holds 1984 rows)
 //   k_page_plan   one wave per column walks the rows and cuts pages exactly where the host
 //                 writer would. A page's byte count is monotone in the rows it holds, so a wave
 //                 finds the first row that does not fit among 64 candidates with one ballot:
@@ -49,11 +51,13 @@ struct PageBatch {
     uint32_t ncol;
     uint32_t pad;
     uint64_t* counts;   // planner: pages per column
+    uint64_t* valid_rows;  // planner: valid rows per INT32 column (zeroed before page_plan)
     uint8_t* pages;     // writer: all columns' pages, column after column
 };
 
-// plan every column (one wave each); counts[c] = pages of column c (refs beyond cap unwritten)
-void page_plan(const PageBatch& b, hipStream_t s);
+// plan every column (one wave each; INT32 columns without NULL by arithmetic); counts[c] =
+// pages of column c (refs beyond cap unwritten); max_rows = the largest column's rows
+void page_plan(const PageBatch& b, uint64_t max_rows, hipStream_t s);
 // write pages p0 .. p0 + count - 1 of the batch (columns' pages in order, b.c[i].first set from
 // the counts) to b.pages[0 .. count)
 void page_write(const PageBatch& b, uint64_t p0, uint64_t count, hipStream_t s);

@@ -130,14 +130,15 @@ def test_scan_many_columns(paged, oracle):
 
 
 def test_scan_staged_batches(paged, oracle):
-    # more pages than one writer launch stages (32768): the copy-out spans batches
-    n = 1984 * 33000 + 77
+    # more pages than two writer batches (4096 pages each): the copy-out cycles through both
+    # halves of the pinned double buffer
+    n = 1984 * 9000 + 77
     rng = np.random.default_rng(8)
     v = rng.integers(-2**31, 2**31, size=n, dtype=np.int64).astype(np.int32)
     ok = np.ones(n, np.uint8)
     ok[rng.integers(0, n, 1000)] = 0
     col = oracle.codec.encode_int32(v, ok)
-    assert col.num_pages > 32768
+    assert col.num_pages > 2 * 4096
     res = paged.execute(_scan_plan(hjtest.ColumnarTable(n, [col])))
     assert res.columns[0].num_pages == col.num_pages
     assert np.array_equal(res.columns[0].pages, col.pages)
