@@ -512,7 +512,8 @@ class Executor {
         if (hb.part)
             for (int j = 0; j < hb.njobs; ++j) {
                 BuildJob& jb = hb.job[j];
-                const uint64_t P = 1ull << fine_bits(jb.kmax);
+                jb.pbits = std::min(fine_bits(jb.kmax), kPartBits);
+                const uint64_t P = 1ull << jb.pbits;
                 jb.poff = scratch<uint16_t>((uint64_t)jb.nblk * (P + 1));
             }
         hb.sgrid = (uint32_t)ctx_->num_cus * 4;  // segment pass: a grid-stride loop

@@ -238,14 +238,14 @@ __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__
 
 // ---------------------------------------------------------------- partitioned build
 // Filter pass of the partitioned build: as k_sb_filter, and the block's tuples are staged in
-// order of their fine hash partition (top fine_bits(kmax) bits of the Fibonacci hash), counted
-// with LDS atomics; poff records where each partition starts in the block's range. The first
-// block of a job also clears the job's small Bloom filter (set by the segment pass).
+// order of their hash partition (top pbits bits of the Fibonacci hash), counted with LDS
+// atomics; poff records where each partition starts in the block's range. The first block of a
+// job also clears the job's small Bloom filter (set by the segment pass).
 constexpr int kFR = (int)(kBuildRows / (kB * kFU));  // rounds per block
 static_assert(kFR * kB * kFU == (int)kBuildRows, "build rows per block");
 
 __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restrict__ bb) {
-    __shared__ uint32_t hist[(1u << kMaxFineBits) + 1];
+    __shared__ uint32_t hist[(1u << kPartBits) + 1];
     __shared__ uint32_t lds[kNW];
     const uint32_t t = threadIdx.x, lane = lane_id();
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
     const uint32_t rows = jb.rows, G = (rows + 63u) >> 6;
     const uint32_t r0 = x * kBuildRows;
     const uint32_t r1 = r0 + kBuildRows < rows ? r0 + kBuildRows : rows;
-    const uint32_t fb = fine_bits(jb.kmax), P = 1u << fb;
+    const uint32_t fb = jb.pbits, P = 1u << fb;
     if (x == 0)
         for (uint32_t i = t; i < kBloomWords; i += kB) gp(jb.bf)[i] = 0;
     for (uint32_t i = t; i <= P; i += kB) hist[i] = 0;
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
     __syncthreads();
     // partition starts: exclusive scan of the P counts, kPB per thread; hist[P] = the total
     {
-        constexpr int kPB = (1 << kMaxFineBits) / kB;
+        constexpr int kPB = (1 << kPartBits) / kB;
         const uint32_t per = (P + kB - 1) / kB;
         uint32_t loc[kPB], sum = 0;
 #pragma unroll
@@ -369,11 +369,15 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
     }
 }
 
-// Segment pass: the table segments of every job (2^(fine bits of the table) per job, from the
-// tuple count), a grid-stride loop over them. A block gathers a segment's tuples from every
-// filter block's staged range (their fine partitions are contiguous there), inserts them into
-// the segment held in LDS, and writes the segment out whole (the table needs no clearing). A
-// segment that runs out of slots flags the build, as a repeated key does.
+// Segment pass: the table segments of every job (2^fine_bits(k) per job, k from the tuple
+// count), a grid-stride loop over them. A block gathers a segment's tuples from every filter
+// block's staged range (its partitions are contiguous there; a partition that holds several
+// segments is read by each of them, each keeping its own tuples), inserts them into the segment
+// held in LDS, and writes the segment out whole (the table needs no clearing). A segment that
+// runs out of slots flags the build, as a repeated key does. Loop index i maps to segment
+// (i % 8) * C + i / 8 (C = ceil(segments / 8)): with a grid that is a multiple of 8, the blocks
+// on one XCD (blocks are dealt to the 8 XCDs in turn) take neighbouring segments, whose staged
+// tuples and partition starts share cache lines.
 __global__ __launch_bounds__(kB) void k_sb_segbuild(const BuildBatch* __restrict__ bb) {
     __shared__ uint32_t T[8u << kSegBits];  // the segment: [4 keys | 4 rows] per bucket
     __shared__ uint32_t s_lo[kMaxPartBlocks], s_pre[kMaxPartBlocks + 1];
@@ -388,17 +392,24 @@ __global__ __launch_bounds__(kB) void k_sb_segbuild(const BuildBatch* __restrict
 #pragma unroll
     for (int j = 0; j < kMaxStages; ++j)
         first[j + 1] = first[j] + (j < nj ? 1u << fine_bits(table_bits(nj_t[j])) : 0u);
-    for (uint32_t f = blockIdx.x; f < first[kMaxStages]; f += gridDim.x) {
+    const uint32_t F = first[kMaxStages], C = (F + 7u) / 8u;
+    for (uint32_t i = blockIdx.x; i < 8u * C; i += gridDim.x) {
+        const uint32_t f = (i & 7u) * C + (i >> 3);
+        if (f >= F) continue;
         int j = 0;
 #pragma unroll
         for (int i = 1; i < kMaxStages; ++i) j += f >= first[i] ? 1 : 0;
         const BuildJob& jb = bb->job[j];
         HJ_GLOBAL uint32_t* meta = gp(jb.meta);
         const uint32_t n = meta[0];
-        const uint32_t fb = fine_bits(jb.kmax), P = 1u << fb;
+        const uint32_t fb = jb.pbits, P = 1u << fb;
         const uint32_t k = table_bits(n), sb = fine_bits(k), S = 1u << (k - 2 - sb);
-        const uint32_t per = 1u << (fb - sb);  // fine partitions per segment
-        const uint32_t seg = f - first[j], y = seg * per;
+        const uint32_t seg = f - first[j];
+        // partitions [y, y + per) hold the segment; with fewer partitions than segments, keep
+        // only the tuples whose top sb hash bits are seg
+        const bool pick = sb > fb;
+        const uint32_t per = pick ? 1u : 1u << (fb - sb);
+        const uint32_t y = pick ? seg >> (sb - fb) : seg * per;
         const uint32_t nb = jb.nblk;
         __syncthreads();  // the previous segment's LDS is written out
         // this segment's range in each filter block (thread t: blocks 2t, 2t + 1), their
@@ -452,6 +463,7 @@ __global__ __launch_bounds__(kB) void k_sb_segbuild(const BuildBatch* __restrict
             for (int u = 0; u < kSU; ++u) {
                 if (!have[u]) continue;
                 const uint32_t h = fib32(key[u]);
+                if (pick && (h >> (32u - sb)) != seg) continue;
                 if (use_bf) atomicOr((uint32_t*)&gp(jb.bf)[bf_word(h, lw)], bf_mask(h));
                 if (key[u] == kEmptyKey) {  // kept aside in slot 2^k
                     if (atomicCAS((uint32_t*)&meta[2], 0u, 1u) != 0u)

@@ -40,12 +40,17 @@ __host__ __device__ inline uint32_t seg_mask(uint32_t k) {
 __host__ __device__ inline uint32_t next_bucket(uint32_t b, uint32_t smask) {
     return (b & ~smask) | ((b + 1u) & smask);
 }
-// fine hash partitions of a job whose table has at most 2^kmax key slots: segments nest in them
+// segment bits of a table with 2^k key slots (2^fine_bits(k) segments); a segment is the
+// key slots whose hash shares its top fine_bits(k) bits
 __host__ __device__ inline uint32_t fine_bits(uint32_t kmax) {
     return kmax - 2 > kSegBits ? kmax - 2 - kSegBits : 0u;
 }
-constexpr uint32_t kMaxFineBits = 12;   // partitioned builds: at most 4096 fine partitions ...
+constexpr uint32_t kMaxFineBits = 12;   // partitioned builds: tables of at most 4096 segments ...
 constexpr uint32_t kMaxPartBlocks = 512;  // ... and 512 filter blocks per job
+// The filter pass orders each block's tuples by the top min(fine_bits(kmax), kPartBits) hash
+// bits (kmax from the unfiltered rows). A segment of a table with more segments than partitions
+// reads its partition's tuples and keeps its own (at most 4 segments per partition).
+constexpr uint32_t kPartBits = 10;
 
 // rows per block of the build's filter and insert passes
 constexpr uint32_t kBuildRows = 8192;
@@ -63,6 +68,7 @@ struct BuildJob {
     int32_t nf;
     uint32_t rows;      // table rows (bound of the tuple count)
     uint32_t kmax;      // the table holds 2^kmax key slots
+    uint32_t pbits;     // partitioned build: hash bits of the filter pass's staging order
     uint32_t blk0;      // first block of this job in the filter and insert grids ...
     uint32_t nblk;      // ... and its blocks, ceil(rows / kBuildRows) (at least 1)
     uint32_t* meta;     // [0] tuples, [1] a key occurs twice, [2] kEmptyKey present, [3] unused
@@ -71,7 +77,7 @@ struct BuildJob {
     uint32_t* blk;      // tuples staged per block of the filter pass
     uint32_t* table;    // 2 * 2^kmax + 8 words; 2^table_bits(tuples) key slots used
     uint16_t* poff;     // partitioned build: per filter block, the start of each of the
-                        // 2^fine_bits(kmax) fine partitions in its staged range, then its total
+                        // 2^pbits partitions in its staged range, then its total
     uint32_t* bf;       // blocked Bloom filter, kBloomWords words (small build sides only)
 };
 
