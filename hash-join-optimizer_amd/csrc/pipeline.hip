@@ -269,10 +269,19 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
     int32_t key[kFR][kFU];
     uint32_t rank[kFR][kFU];
     uint32_t alive = 0;  // bit r * kFU + u: this lane's row of round r, group u passes
-    // every round's keys and validity words at once, then the filters' columns the same way
+    // every round's keys and validity words at once, with the first filter's column, then the
+    // other filters' columns the same way
     uint64_t live[kFR][kFU];
     {
-        u32x2 kvw[kFR];
+        u32x2 kvw[kFR], fvw[kFR];
+        int32_t v[kFR][kFU];
+        const DCol& fc = jb.fcol[0];
+        const __amdgpu_buffer_rsrc_t rf = make_rsrc(fc.flat, nf ? rows * 4u : 0u);
+        const __amdgpu_buffer_rsrc_t rfv =
+            make_rsrc(fc.vbits ? (const void*)fc.vbits : (const void*)fc.flat,
+                      nf && fc.vbits ? G * 8u : 0u);
+        const uint32_t fv_none = nf && fc.vbits ? 0u : ~0u;
+        const int32_t lo = nf ? jb.flo[0] : INT32_MIN, hi = nf ? jb.fhi[0] : INT32_MAX;
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd) {
             const uint32_t base = r0 + rd * (kB * kFU);
@@ -281,17 +290,24 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
             for (int u = 0; u < kFU; ++u)
                 key[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(rk, vo + u * 1024u, 0, 0);
             kvw[rd] = __builtin_amdgcn_raw_buffer_load_b64(rkv, (base / 64u + w) * 8u + lanev, 0, 0);
+#pragma unroll
+            for (int u = 0; u < kFU; ++u)
+                v[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(rf, vo + u * 1024u, 0, 0);
+            fvw[rd] = __builtin_amdgcn_raw_buffer_load_b64(rfv, (base / 64u + w) * 8u + lanev, 0, 0);
         }
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd)
 #pragma unroll
             for (int u = 0; u < kFU; ++u) {
                 const uint32_t g0 = r0 + rd * (kB * kFU) + u * 256u + w * 64u;
+                // without a filter the loads read 0 (empty resource) and the range passes all
                 live[rd][u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) &
-                              lane_word(kvw[rd].x | kv_none, kvw[rd].y | kv_none, u);
+                              lane_word(kvw[rd].x | kv_none, kvw[rd].y | kv_none, u) &
+                              __ballot(v[rd][u] >= lo && v[rd][u] <= hi) &
+                              lane_word(fvw[rd].x | fv_none, fvw[rd].y | fv_none, u);
             }
     }
-    for (int f = 0; f < nf; ++f) {
+    for (int f = 1; f < nf; ++f) {
         const DCol& fc = jb.fcol[f];
         const __amdgpu_buffer_rsrc_t rf = make_rsrc(fc.flat, rows * 4u);
         const __amdgpu_buffer_rsrc_t rfv =
@@ -401,7 +417,9 @@ __global__ __launch_bounds__(kB) void k_sb_segbuild(const BuildBatch* __restrict
         for (int i = 1; i < kMaxStages; ++i) j += f >= first[i] ? 1 : 0;
         const BuildJob& jb = bb->job[j];
         HJ_GLOBAL uint32_t* meta = gp(jb.meta);
-        const uint32_t n = meta[0];
+        uint32_t n = 0;  // the job's tuple count, loaded above
+#pragma unroll
+        for (int jj = 0; jj < kMaxStages; ++jj) n = jj == j ? nj_t[jj] : n;
         const uint32_t fb = jb.pbits, P = 1u << fb;
         const uint32_t k = table_bits(n), sb = fine_bits(k), S = 1u << (k - 2 - sb);
         const uint32_t seg = f - first[j];
