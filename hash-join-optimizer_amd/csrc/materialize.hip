@@ -210,6 +210,31 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
     const HJ_GLOBAL uint32_t* rids = gp(d.stage[ec.slot]);
     HJ_GLOBAL uint64_t* offs = gp(ec.offsets);
     if (str && blockIdx.x == 0 && t == 0) offs[0] = 0;
+    if (!str) {
+        // INT32: kIntRows rows per thread and step, their loads in flight together (a group of
+        // a large output holds tens of thousands of rows)
+        constexpr int kIntRows = 8;
+        for (uint32_t f0 = 0; f0 < R; f0 += kB * kIntRows) {
+            uint32_t rid[kIntRows];
+#pragma unroll
+            for (int u = 0; u < kIntRows; ++u) {
+                const uint32_t f = f0 + u * kB + t;
+                uint32_t jj = 0;
+                for (uint32_t x = 1; x < G; ++x) jj = f >= s_rb[x] ? x : jj;
+                rid[u] = f < R ? rids[(uint64_t)(j0 + jj) * d.rows_per_block + (f - s_rb[jj])] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kIntRows; ++u) {
+                const uint32_t f = f0 + u * kB + t;
+                if (f >= R) continue;
+                int32_t v = 0;
+                const bool ok = get_int32(ec.col, rid[u], &v);
+                gp(ec.values)[off + f] = ok ? v : 0;
+                gp(ec.valid8)[off + f] = ok ? 1 : 0;
+            }
+        }
+        return;
+    }
     uint64_t run = s_sum[2];  // string byte offset of the next row (same in all threads)
     for (uint32_t f0 = 0; f0 < R; f0 += kB) {
         const uint32_t f = f0 + t;
