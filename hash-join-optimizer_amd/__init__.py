@@ -5,8 +5,8 @@ Import as importlib.import_module("hash-join-optimizer_amd") with the repository
 """
 from . import abi  # noqa: F401
 from .abi import EngineMissing  # noqa: F401
-from .plan import (Codec, Column, ColumnarTable, DataType, JoinNode, Plan,  # noqa: F401
-                   PlanNode, ScanNode, Table, sorted_rows)
+from .plan import (CacheError, Codec, Column, ColumnarTable, DataType, JoinNode,  # noqa: F401
+                   Plan, PlanNode, ScanNode, Table, read_cache, sorted_rows, write_cache)
 from .engine import Context, DeviceResult, DeviceTable, Directory, EngineError  # noqa: F401
 
 HASH_ALGO = "hip"  # the hash-table implementation this package provides (GPU unchained directory)

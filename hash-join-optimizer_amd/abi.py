@@ -96,6 +96,11 @@ _SIGS = {
     "hj_directory_info": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
     "hj_directory_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hj_free": (None, [C.c_void_p]),
+    "hj_cache_open": (C.c_int, [C.c_char_p, P(C.c_void_p)]),
+    "hj_cache_table": (P(hj_table), [C.c_void_p]),
+    "hj_cache_close": (None, [C.c_void_p]),
+    "hj_cache_write": (C.c_int, [P(hj_table), C.c_char_p]),
+    "hj_table_upload_cache": (C.c_int, [C.c_void_p, C.c_char_p, P(C.c_void_p)]),
     "hj_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "hj_profile_enable": (C.c_int, [C.c_void_p, C.c_char_p]),
     "hj_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64]),

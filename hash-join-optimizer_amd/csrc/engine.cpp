@@ -30,6 +30,7 @@
 #include "kernels.hpp"
 #include "pagecodec.hpp"
 #include "pagewrite.hpp"
+#include "tblcache.hpp"
 #include "pipeline.hpp"
 #include "materialize.hpp"
 
@@ -843,6 +844,72 @@ uint32_t rd16h(const uint8_t* p) {
     return v;
 }
 
+// copies segments (dst, src, bytes) with up to 8 threads for large totals
+struct CopySeg {
+    uint8_t* dst;
+    const uint8_t* src;
+    size_t bytes;
+};
+void parallel_copy(const std::vector<CopySeg>& segs) {
+    size_t total = 0;
+    for (const auto& g : segs) total += g.bytes;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = (unsigned)std::min<size_t>({8, hw, total / (4u << 20) + 1});
+    auto run = [&segs](size_t lo, size_t hi) {  // bytes [lo, hi) of the concatenated segments
+        size_t base = 0;
+        for (const auto& g : segs) {
+            const size_t a = std::max(lo, base), e = std::min(hi, base + g.bytes);
+            if (a < e) std::memcpy(g.dst + (a - base), g.src + (a - base), e - a);
+            base += g.bytes;
+        }
+    };
+    if (nt <= 1) {
+        run(0, total);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t step = (total + nt - 1) / nt;
+    for (unsigned t = 1; t < nt; ++t)
+        th.emplace_back(run, std::min(total, t * step), std::min(total, (t + 1) * step));
+    run(0, std::min(total, step));
+    for (auto& x : th) x.join();
+}
+
+// the context's pinned double buffer for pages on their way between host and HBM
+void ensure_page_stage(hj_context* ctx) {
+    if (ctx->page_stage) return;
+    HJ_HIP(hipHostMalloc(&ctx->page_stage, 2 * hj_context::kPageStageHalf, hipHostMallocDefault));
+    for (auto& e : ctx->page_ev) HJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+// host pages -> contiguous device pages, in batches of 32 MiB through the pinned double buffer:
+// the host fills one half (up to 8 threads) while the other half's copy to HBM runs
+void copy_pages_h2d(hj_context* ctx, uint8_t* dst, const uint8_t* const* pages, uint32_t np) {
+    ensure_page_stage(ctx);
+    constexpr uint64_t kHalfPages = hj_context::kPageStageHalf / HJ_PAGE_SIZE;
+    const uint64_t nb = (np + kHalfPages - 1) / kHalfPages;
+    for (uint64_t k = 0; k < nb; ++k) {
+        uint8_t* half = ctx->page_stage + (k & 1) * hj_context::kPageStageHalf;
+        // the copy out of this half two batches ago must be done before it is refilled
+        if (k >= 2) HJ_HIP(hipEventSynchronize(ctx->page_ev[k & 1]));
+        const uint64_t p0 = k * kHalfPages, cnt = std::min<uint64_t>(kHalfPages, np - p0);
+        std::vector<CopySeg> segs;
+        for (uint64_t j = 0; j < cnt; ++j) {
+            const uint8_t* src = pages[p0 + j];
+            if (!src) throw Error(HJ_E_INVALID, "null page pointer");
+            if (!segs.empty() && segs.back().src + segs.back().bytes == src)
+                segs.back().bytes += HJ_PAGE_SIZE;  // pages adjacent in host memory: one run
+            else
+                segs.push_back({half + j * HJ_PAGE_SIZE, src, HJ_PAGE_SIZE});
+        }
+        parallel_copy(segs);
+        HJ_HIP(hipMemcpyAsync(dst + p0 * HJ_PAGE_SIZE, half, cnt * HJ_PAGE_SIZE,
+                              hipMemcpyHostToDevice, ctx->stream));
+        HJ_HIP(hipEventRecord(ctx->page_ev[k & 1], ctx->stream));
+    }
+    HJ_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj_dtable* t,
                    DevColumn& dc) {
     dc.type = hc.type;
@@ -921,13 +988,7 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
         return p;
     };
     dc.pages = (uint8_t*)dev_alloc(bytes + 256);  // slack for wide loads near the end
-    if (bytes) {
-        std::vector<uint8_t> staging(bytes);
-        for (uint32_t p = 0; p < hc.num_pages; ++p)
-            std::memcpy(staging.data() + (size_t)p * HJ_PAGE_SIZE, hc.pages[p], HJ_PAGE_SIZE);
-        HJ_HIP(hipMemcpyAsync(dc.pages, staging.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
-        HJ_HIP(hipStreamSynchronize(ctx->stream));
-    }
+    if (bytes) copy_pages_h2d(ctx, dc.pages, hc.pages, hc.num_pages);
     dc.row_start = (uint32_t*)dev_alloc(dc.h_row_start.size() * 4);
     HJ_HIP(hipMemcpyAsync(dc.row_start, dc.h_row_start.data(), dc.h_row_start.size() * 4,
                           hipMemcpyHostToDevice, ctx->stream));
@@ -1002,37 +1063,6 @@ void free_dresult(hj_context* ctx, hj_dresult* r) {
     delete r;
 }
 
-// copies segments (dst, src, bytes) with up to 8 threads for large totals
-struct CopySeg {
-    uint8_t* dst;
-    const uint8_t* src;
-    size_t bytes;
-};
-void parallel_copy(const std::vector<CopySeg>& segs) {
-    size_t total = 0;
-    for (const auto& g : segs) total += g.bytes;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const unsigned nt = (unsigned)std::min<size_t>({8, hw, total / (4u << 20) + 1});
-    auto run = [&segs](size_t lo, size_t hi) {  // bytes [lo, hi) of the concatenated segments
-        size_t base = 0;
-        for (const auto& g : segs) {
-            const size_t a = std::max(lo, base), e = std::min(hi, base + g.bytes);
-            if (a < e) std::memcpy(g.dst + (a - base), g.src + (a - base), e - a);
-            base += g.bytes;
-        }
-    };
-    if (nt <= 1) {
-        run(0, total);
-        return;
-    }
-    std::vector<std::thread> th;
-    const size_t step = (total + nt - 1) / nt;
-    for (unsigned t = 1; t < nt; ++t)
-        th.emplace_back(run, std::min(total, t * step), std::min(total, (t + 1) * step));
-    run(0, std::min(total, step));
-    for (auto& x : th) x.join();
-}
-
 // Device page encoder: plan every column's pages (one wave per column), read the page counts,
 // then write the pages in HBM in batches of up to 32 MiB. Each batch goes to one half of a
 // pinned double buffer while the host copies the previous batch into the result columns'
@@ -1045,10 +1075,7 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
     out->num_columns = (uint32_t)ncols;
     out->columns = (hj_out_column*)std::calloc(ncols ? ncols : 1, sizeof(hj_out_column));
     if (!out->columns) throw std::bad_alloc();
-    if (!ctx->page_stage) {
-        HJ_HIP(hipHostMalloc(&ctx->page_stage, 2 * hj_context::kPageStageHalf, hipHostMallocDefault));
-        for (auto& e : ctx->page_ev) HJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
+    ensure_page_stage(ctx);
     Arena& A = ctx->arena;
     const size_t mark = A.mark();
     try {
@@ -1333,6 +1360,32 @@ int hj_table_upload(hj_context* ctx, const hj_table* table, hj_dtable** out) {
     }
     *out = t;
     return HJ_OK;
+}
+
+int hj_table_upload_cache(hj_context* ctx, const char* path, hj_dtable** out) {
+    if (!ctx || !path || !out) return HJ_E_INVALID;
+    *out = nullptr;
+    hj_cache* c = nullptr;
+    int rc = guarded(ctx, [&] { c = open_cache(path); });
+    if (rc) return rc;
+    rc = hj_table_upload(ctx, &c->table, out);
+    close_cache(c);
+    return rc;
+}
+
+int hj_cache_open(const char* path, hj_cache** out) {
+    if (!path || !out) return HJ_E_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] { *out = open_cache(path); });
+}
+
+const hj_table* hj_cache_table(const hj_cache* c) { return c ? &c->table : nullptr; }
+
+void hj_cache_close(hj_cache* c) { close_cache(c); }
+
+int hj_cache_write(const hj_table* table, const char* path) {
+    if (!table || !path) return HJ_E_INVALID;
+    return guarded(nullptr, [&] { write_cache(*table, path); });
 }
 
 int hj_table_free(hj_context* ctx, hj_dtable* t) {

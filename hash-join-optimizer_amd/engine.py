@@ -89,6 +89,13 @@ class Context:
         self._check(self.lib.hj_table_upload(self.handle, C.byref(ht), C.byref(h)))
         return DeviceTable(self, h, table.num_rows)
 
+    def upload_cache(self, path):
+        """Table::from_cache (src/build_table.cpp:147-203) straight into HBM
+        (hj_table_upload_cache)"""
+        h = C.c_void_p()
+        self._check(self.lib.hj_table_upload_cache(self.handle, str(path).encode(), C.byref(h)))
+        return DeviceTable(self, h, int(self.lib.hj_table_num_rows(h)))
+
     def execute_device(self, plan, dtables):
         # the C plan is rebuilt only when the plan changed (node count, root)
         sig = (len(plan.nodes), plan.root)

@@ -169,6 +169,47 @@ def tables_to_c(tables):
     return arr, keep
 
 
+# ------------------------------------------------------------------------------ .tbl cache files
+class CacheError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def write_cache(table, path, lib=None):
+    """DumpTable::dump (include/table.h:200-211): the header page (TableMeta), then every
+    column's pages, to `path` (hj_cache_write)"""
+    lib = lib or abi.load()
+    keep = []
+    ht = table_to_c(table, keep)
+    rc = lib.hj_cache_write(C.byref(ht), str(path).encode())
+    if rc:
+        raise CacheError(rc, lib.hj_global_error().decode())
+
+
+def read_cache(path, lib=None):
+    """Table::from_cache (src/build_table.cpp:147-203): the file's ColumnarTable, its pages
+    copied out of the mapping (hj_cache_open)"""
+    lib = lib or abi.load()
+    h = C.c_void_p()
+    rc = lib.hj_cache_open(str(path).encode(), C.byref(h))
+    if rc:
+        raise CacheError(rc, lib.hj_global_error().decode())
+    try:
+        t = lib.hj_cache_table(h).contents
+        cols = []
+        for j in range(t.num_columns):
+            c = t.columns[j]
+            pages = np.empty((c.num_pages, PAGE_SIZE), dtype=np.uint8)
+            if c.num_pages:  # a column's pages are contiguous in the file
+                C.memmove(pages.ctypes.data, c.pages[0], c.num_pages * PAGE_SIZE)
+            cols.append(Column(c.type, pages))
+        return ColumnarTable(t.num_rows, cols)
+    finally:
+        lib.hj_cache_close(h)
+
+
 # ------------------------------------------------------------------------------ page codec
 class Codec:
     """Page writer/reader behind a C library exposing <prefix>encode_int32 etc. with the

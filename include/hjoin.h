@@ -176,6 +176,24 @@ uint64_t hj_column_num_chars(const hj_column* col);
 int hj_decode_int32(const hj_column* col, int32_t* values, uint8_t* validity);
 int hj_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint8_t* validity);
 
+/* ---- .tbl table cache files (host only, no GPU needed): Table::from_cache
+ * (src/build_table.cpp:147-203) maps a file written by DumpTable::dump (include/table.h:128-212).
+ * Layout: an 8192-byte header page holding TableMeta {uint64 num_rows; uint64 num_cols;
+ * int32 types[16]; uint64 num_pages[16]} (208 bytes, then zeros), then the pages of column 0,
+ * column 1, ... Errors carry the reference's messages ("Failed to open file: <path>", "File is
+ * empty: <path>"); a file with more than 16 columns, an unknown type or a size other than
+ * 8192 x (1 + its page count) is refused. ---- */
+typedef struct hj_cache hj_cache;
+int hj_cache_open(const char* path, hj_cache** out);
+/* the file's table: page pointers into the read-only mapping, valid until hj_cache_close */
+const hj_table* hj_cache_table(const hj_cache* cache);
+void hj_cache_close(hj_cache* cache);
+/* DumpTable::dump (include/table.h:200-211): the header page, then every column's pages */
+int hj_cache_write(const hj_table* table, const char* path);
+/* Table::from_cache straight into HBM: the mapped pages go to the device through a pinned
+ * double buffer, then as hj_table_upload. */
+int hj_table_upload_cache(hj_context* ctx, const char* path, hj_dtable** out);
+
 /* ---- engine options (no counterpart in the reference):
  *   "pipeline" 1 (default) / 0: run left-deep chains whose build sides are base scans as one
  *              fused probe pipeline; 0 forces the join-at-a-time path for every plan.

@@ -15,6 +15,7 @@
  *   Table::to_columnar / from_columnar  src/build_table.cpp:513-882
  *   load-time INT32 range filter ...... src/build_table.cpp:319-371, include/inner_column.h:250-300
  *   UnchainedHash ..................... include/unchained.h:14-247
+ *   DumpTable::dump / Table::from_cache include/table.h:30-37,128-211, src/build_table.cpp:147-203
  * Not restated: the slab allocator's chunk order (include/slab_allocator.h), which only fixes the
  * output row order; parity is on the sorted multiset of rows (tests/read_sql.cpp:1293-1300).
  * Divergence: with one partition the reference shifts a 64-bit hash by 64 (undefined,
@@ -1486,3 +1487,82 @@ uint64_t oracle_unchained_probe(const oracle_unchained* h, uint64_t key, uint64_
     return cnt;
 }
 uint32_t oracle_unchained_dir_size(const oracle_unchained* h) { return h->dir_size; }
+
+/* ------------------------------------------------------------------ .tbl cache files */
+/* DumpTable::dump (include/table.h:200-211): TableMeta (include/table.h:30-35) written field by
+ * field at its C layout offsets, zeros to the end of the 8192-byte header page, then each
+ * column's pages. */
+int oracle_cache_write(const hj_table* t, const char* path) {
+    if (t->num_columns > 16) return -1;
+    unsigned char head[HJ_PAGE_SIZE];
+    memset(head, 0, sizeof head);
+    uint64_t v = t->num_rows;
+    memcpy(head + 0, &v, 8);
+    v = t->num_columns;
+    memcpy(head + 8, &v, 8);
+    for (uint32_t i = 0; i < t->num_columns; ++i) {
+        int32_t ty = t->columns[i].type;
+        uint64_t np = t->columns[i].num_pages;
+        memcpy(head + 16 + 4 * i, &ty, 4);
+        memcpy(head + 80 + 8 * i, &np, 8);
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) return -1;
+    int ok = fwrite(head, 1, sizeof head, f) == sizeof head;
+    for (uint32_t i = 0; ok && i < t->num_columns; ++i)
+        for (uint32_t j = 0; ok && j < t->columns[i].num_pages; ++j)
+            ok = fwrite(t->columns[i].pages[j], 1, HJ_PAGE_SIZE, f) == HJ_PAGE_SIZE;
+    if (fclose(f) != 0) ok = 0;
+    return ok ? 0 : -1;
+}
+
+/* Table::from_cache (src/build_table.cpp:147-203), reading instead of mapping: the header's
+ * counts, then num_pages[i] pages per column in column order. */
+int oracle_cache_read(const char* path, hj_result* out) {
+    memset(out, 0, sizeof *out);
+    FILE* f = fopen(path, "rb");
+    if (!f) return -1;
+    unsigned char head[HJ_PAGE_SIZE];
+    if (fread(head, 1, sizeof head, f) != sizeof head) {
+        fclose(f);
+        return -1;
+    }
+    uint64_t rows, ncols, pages[16], total = 0;
+    int32_t types[16];
+    memcpy(&rows, head + 0, 8);
+    memcpy(&ncols, head + 8, 8);
+    if (ncols > 16) {
+        fclose(f);
+        return -1;
+    }
+    for (uint64_t i = 0; i < ncols; ++i) {
+        memcpy(&types[i], head + 16 + 4 * i, 4);
+        memcpy(&pages[i], head + 80 + 8 * i, 8);
+        total += pages[i];
+    }
+    fseek(f, 0, SEEK_END);
+    const long size = ftell(f);
+    if (size < 0 || (uint64_t)size != (total + 1) * HJ_PAGE_SIZE) {
+        fclose(f);
+        return -1;
+    }
+    fseek(f, HJ_PAGE_SIZE, SEEK_SET);
+    out->num_rows = rows;
+    out->num_columns = (uint32_t)ncols;
+    out->columns = (hj_out_column*)calloc(ncols ? ncols : 1, sizeof(hj_out_column));
+    int ok = out->columns != NULL;
+    for (uint64_t i = 0; ok && i < ncols; ++i) {
+        hj_out_column* c = &out->columns[i];
+        c->type = types[i];
+        c->num_pages = (uint32_t)pages[i];
+        if (!pages[i]) continue;
+        c->pages = (uint8_t*)malloc(pages[i] * HJ_PAGE_SIZE);
+        ok = c->pages && fread(c->pages, 1, pages[i] * HJ_PAGE_SIZE, f) == pages[i] * HJ_PAGE_SIZE;
+    }
+    fclose(f);
+    if (!ok) {
+        oracle_result_free(out);
+        return -1;
+    }
+    return 0;
+}

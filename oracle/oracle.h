@@ -52,6 +52,14 @@ uint64_t oracle_column_chars(const hj_column* col);
 int oracle_decode_int32(const hj_column* col, int32_t* values, uint8_t* validity);
 int oracle_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint8_t* validity);
 
+/* .tbl cache files: DumpTable::dump (include/table.h:128-211) and Table::from_cache
+ * (src/build_table.cpp:147-203). The header page is TableMeta at byte 0 - num_rows (u64) at 0,
+ * num_cols (u64) at 8, types (16 x int32) at 16, num_pages (16 x u64) at 80, 208 bytes - and
+ * zeros to 8192; the pages of column 0, 1, ... follow. The reader returns -1 for a file that is
+ * shorter than its header, has more than 16 columns or a size other than 8192 x (1 + pages). */
+int oracle_cache_write(const hj_table* t, const char* path);
+int oracle_cache_read(const char* path, hj_result* out);
+
 /* PartitionedDirectoryHash build + probe (include/partitioned_directory_hash.h:68-215) over raw
  * keys; writes (probe index, build index) pairs in probe-chunk order. Returns the pair count, or
  * (uint64_t)-1 when cap is too small (pairs then holds the first cap pairs). */

@@ -24,6 +24,7 @@ ColumnarTable = _plan.ColumnarTable
 Codec = _plan.Codec
 sorted_rows = _plan.sorted_rows
 result_from_c = _plan.result_from_c
+table_to_c = _plan.table_to_c
 
 
 def _ensure_built():
@@ -47,6 +48,8 @@ class Oracle:
             "oracle_nested_loop_execute": (C.c_int, [P(abi.hj_plan), P(abi.hj_result),
                                                      C.c_char_p, C.c_size_t]),
             "oracle_result_free": (None, [P(abi.hj_result)]),
+            "oracle_cache_write": (C.c_int, [P(abi.hj_table), C.c_char_p]),
+            "oracle_cache_read": (C.c_int, [C.c_char_p, P(abi.hj_result)]),
             "oracle_value_int32": (C.c_uint64, [C.c_int32]),
             "oracle_value_strref": (C.c_uint64, [C.c_uint64]),
             "oracle_strref_encode": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64,
@@ -88,6 +91,20 @@ class Oracle:
         rc = self.lib.oracle_nested_loop_execute(C.byref(p), C.byref(res), err, 256)
         if rc:
             raise RuntimeError(f"oracle_nested_loop_execute failed ({rc}): {err.value.decode()}")
+        return result_from_c(res, self.lib.oracle_result_free)
+
+    def cache_write(self, table, path):
+        """the oracle's DumpTable::dump"""
+        keep = []
+        ht = table_to_c(table, keep)
+        if self.lib.oracle_cache_write(C.byref(ht), str(path).encode()):
+            raise RuntimeError("oracle_cache_write failed")
+
+    def cache_read(self, path):
+        """the oracle's Table::from_cache; None when it refuses the file"""
+        res = abi.hj_result()
+        if self.lib.oracle_cache_read(str(path).encode(), C.byref(res)):
+            return None
         return result_from_c(res, self.lib.oracle_result_free)
 
     def rows(self, ct):

@@ -328,3 +328,47 @@ def test_star4_reduced_scale_vs_numpy(gpu_ctx, hjo, mode):
         exp.append(tuple(r))
     assert len(got) == len(exp) > 1000
     assert got == hjtest.sorted_rows(hjtest.Table(exp, [0, 0, 0, 3, 3, 3]))
+
+
+def test_cache_file_upload(gpu_ctx, hjo, oracle, tmp_path):
+    # Table::from_cache straight into HBM: the same rows as uploading the ColumnarTables
+    rng = np.random.default_rng(31)
+    plan = hjtest.random_plan(rng, oracle.codec, ntables=3, max_rows=3000, key_domain=400,
+                              long_every=97)
+    paths = []
+    for i, t in enumerate(plan.inputs):
+        p = tmp_path / f"t{i}.tbl"
+        hjo.write_cache(t, p)
+        paths.append(p)
+    dts = [gpu_ctx.upload_cache(p) for p in paths]
+    try:
+        r = gpu_ctx.execute_device(plan, dts)
+        got = oracle.rows(r.to_host())
+        r.free()
+    finally:
+        for d in dts:
+            d.free()
+    assert got == oracle.rows(oracle.execute(plan))
+
+
+def test_cache_file_upload_many_pages(gpu_ctx, hjo, oracle, tmp_path):
+    # 9001 pages: three batches through the pinned double buffer; a root scan returns them
+    n = 1984 * 9000 + 11
+    v = np.arange(n, dtype=np.int64).astype(np.int32)
+    col = oracle.codec.encode_int32(v)
+    t = hjtest.ColumnarTable(n, [col])
+    p = tmp_path / "big.tbl"
+    hjo.write_cache(t, p)
+    dt = gpu_ctx.upload_cache(p)
+    plan = hjtest.Plan()
+    plan.inputs.append(t)
+    plan.new_scan_node(0, [(0, hjtest.DataType.INT32)])
+    plan.root = 0
+    try:
+        r = gpu_ctx.execute_device(plan, [dt])
+        res = r.to_host()
+        r.free()
+    finally:
+        dt.free()
+    assert res.num_rows == n
+    assert np.array_equal(res.columns[0].pages, col.pages)
