@@ -38,12 +38,22 @@ class hj_filter(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class hj_pred(C.Structure):
+    _fields_ = [("op", C.c_int32), ("column", C.c_uint32), ("ival", C.c_int64),
+                ("sval", C.c_char_p), ("slen", C.c_uint64)]
+
+
+PRED_OPS = {"eq": 0, "neq": 1, "lt": 2, "gt": 3, "leq": 4, "geq": 5, "like": 6, "not_like": 7,
+            "is_null": 8, "is_not_null": 9, "and": 16, "or": 17, "not": 18}
+
+
 class hj_node(C.Structure):
     _fields_ = [("kind", C.c_int32), ("build_left", C.c_int32),
                 ("base_table_id", C.c_uint64), ("left", C.c_uint64), ("right", C.c_uint64),
                 ("left_attr", C.c_uint64), ("right_attr", C.c_uint64),
                 ("num_output_attrs", C.c_uint64), ("output_attrs", C.POINTER(hj_attr)),
-                ("num_filters", C.c_uint64), ("filters", C.POINTER(hj_filter))]
+                ("num_filters", C.c_uint64), ("filters", C.POINTER(hj_filter)),
+                ("num_preds", C.c_uint64), ("preds", C.POINTER(hj_pred))]
 
 
 class hj_plan(C.Structure):

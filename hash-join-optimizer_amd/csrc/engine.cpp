@@ -32,6 +32,7 @@
 #include "pagewrite.hpp"
 #include "tblcache.hpp"
 #include "pipeline.hpp"
+#include "predicate.hpp"
 #include "materialize.hpp"
 
 namespace hj {
@@ -89,7 +90,41 @@ struct Rel {
     std::vector<AttrRef> attrs;
     uint32_t table = 0;                // scan only
     std::vector<hj_filter> filters;    // scan only
+    DCol pred_view{};                  // scan only: the predicate program's 0 / 1 per row
 };
+
+// filter column standing for a scan's predicate program (its result, filtered on = 1)
+constexpr uint32_t kPredColumn = 0xffffffffu;
+
+// a LIKE pattern as tokens: UTF-8 characters (the decoding of oracle_like and the device
+// matcher: a byte that does not start a complete sequence is a character of its own), with '%'
+// and '_' as kLikeAnyRun / kLikeAnyOne
+std::vector<uint32_t> like_tokens(const uint8_t* p, uint64_t n) {
+    std::vector<uint32_t> t;
+    for (uint64_t i = 0; i < n;) {
+        const uint32_t b = p[i];
+        uint32_t need = 0, cp = b;
+        if (b >= 0xC2 && b <= 0xDF) { need = 1; cp = b & 0x1F; }
+        else if (b >= 0xE0 && b <= 0xEF) { need = 2; cp = b & 0x0F; }
+        else if (b >= 0xF0 && b <= 0xF4) { need = 3; cp = b & 0x07; }
+        else if (b >= 0x80) { need = 0; cp = 0x110000u + b; }
+        uint32_t len = 1;
+        if (need) {
+            bool ok = true;
+            for (uint32_t k = 1; k <= need && ok; ++k)
+                ok = i + k < n && (p[i + k] & 0xC0) == 0x80;
+            if (ok) {
+                for (uint32_t k = 1; k <= need; ++k) cp = (cp << 6) | (p[i + k] & 0x3F);
+                len = need + 1;
+            } else {
+                cp = 0x110000u + b;
+            }
+        }
+        t.push_back(b == '%' ? kLikeAnyRun : b == '_' ? kLikeAnyOne : cp);
+        i += len;
+    }
+    return t;
+}
 
 class Executor {
   public:
@@ -173,7 +208,7 @@ class Executor {
             if (a.index >= tb->cols.size()) throw Error(HJ_E_INVALID, "scan attribute out of range");
             r.attrs.push_back({0, (uint32_t)a.index, tb->cols[a.index].type});
         }
-        if (nd.num_filters > (uint64_t)kMaxFilters)
+        if (nd.num_filters + (nd.num_preds ? 1 : 0) > (uint64_t)kMaxFilters)
             throw Error(HJ_E_INVALID, "too many filters on one scan");
         for (uint64_t f = 0; f < nd.num_filters; ++f) {
             const hj_filter& fl = nd.filters[f];
@@ -181,7 +216,97 @@ class Executor {
                 throw Error(HJ_E_TYPE, "filter column must be an INT32 column of the scanned table");
             r.filters.push_back(fl);
         }
+        if (nd.num_preds) predicate(nd, tb, r);
         return r;
+    }
+
+    // the scan's predicate program, run once over the table's rows into an INT32 0 / 1 array
+    // that the scan filters on like any range filter
+    void predicate(const hj_node& nd, const hj_dtable* tb, Rel& r) {
+        if (nd.num_preds > (uint64_t)kMaxPredOps)
+            throw Error(HJ_E_INVALID, "a scan predicate has more than 32 entries");
+        if (!nd.preds) throw Error(HJ_E_INVALID, "scan predicate missing");
+        if (tb->cols.empty()) throw Error(HJ_E_INVALID, "predicate on a table without columns");
+        PredProg pp;
+        std::memset(&pp, 0, sizeof pp);
+        std::vector<uint8_t> lit;
+        std::vector<uint32_t> used;
+        int depth = 0;
+        for (uint64_t k = 0; k < nd.num_preds; ++k) {
+            const hj_pred& e = nd.preds[k];
+            PredOp& o = pp.op[k];
+            o.op = e.op;
+            if (e.op == HJ_P_AND || e.op == HJ_P_OR || e.op == HJ_P_NOT) {
+                const int need = e.op == HJ_P_NOT ? 1 : 2;
+                if (depth < need) throw Error(HJ_E_INVALID, "scan predicate: operator without operands");
+                depth -= need - 1;
+                continue;
+            }
+            if (e.op < HJ_P_EQ || e.op > HJ_P_IS_NOT_NULL)
+                throw Error(HJ_E_INVALID, "scan predicate: unknown operator");
+            if (e.column >= tb->cols.size())
+                throw Error(HJ_E_INVALID, "scan predicate: column out of range");
+            const int32_t ty = tb->cols[e.column].type;
+            if (ty != HJ_INT32 && ty != HJ_VARCHAR)
+                throw Error(HJ_E_TYPE, "scan predicates take INT32 and VARCHAR columns");
+            const bool like = e.op == HJ_P_LIKE || e.op == HJ_P_NOT_LIKE;
+            if (like && ty != HJ_VARCHAR) throw Error(HJ_E_TYPE, "LIKE needs a VARCHAR column");
+            size_t slot = 0;
+            while (slot < used.size() && used[slot] != e.column) ++slot;
+            if (slot == used.size()) {
+                if (used.size() == (size_t)kMaxPredCols)
+                    throw Error(HJ_E_INVALID, "a scan predicate reads more than 8 columns");
+                used.push_back(e.column);
+            }
+            o.col = (int32_t)slot;
+            if (ty == HJ_INT32) {
+                o.ival = (int32_t)e.ival;  // the reference's static_cast<int32_t> of the literal
+            } else if (e.op < HJ_P_IS_NULL) {
+                if (e.slen && !e.sval) throw Error(HJ_E_INVALID, "scan predicate: literal missing");
+                while (lit.size() % 4) lit.push_back(0);
+                o.loff = (uint32_t)lit.size();
+                if (like) {
+                    const std::vector<uint32_t> t = like_tokens((const uint8_t*)e.sval, e.slen);
+                    if (t.size() > (size_t)kMaxLikeTokens)
+                        throw Error(HJ_E_INVALID, "LIKE patterns longer than 63 characters are not supported");
+                    o.llen = (uint32_t)t.size();
+                    for (size_t j = 0; j < t.size(); ++j) {
+                        if (t[j] == kLikeAnyRun) o.pct |= 1ull << (j + 1);
+                        if (t[j] == kLikeAnyOne) o.any |= 1ull << (j + 1);
+                    }
+                    const uint8_t* tb8 = (const uint8_t*)t.data();
+                    lit.insert(lit.end(), tb8, tb8 + t.size() * 4);
+                } else {
+                    if (e.slen >= (1ull << 32)) throw Error(HJ_E_INVALID, "scan predicate: literal too long");
+                    o.llen = (uint32_t)e.slen;
+                    lit.insert(lit.end(), (const uint8_t*)e.sval, (const uint8_t*)e.sval + e.slen);
+                }
+            }
+            if (++depth > 32) throw Error(HJ_E_INVALID, "scan predicate deeper than 32");
+        }
+        if (depth != 1) throw Error(HJ_E_INVALID, "a scan predicate must leave exactly one result");
+        pp.nops = (int32_t)nd.num_preds;
+        pp.ncol = (int32_t)used.size();
+        for (size_t i = 0; i < used.size(); ++i) pp.col[i] = tb->cols[used[i]].view();
+        uint8_t* dlit = result<uint8_t>(lit.size() + 4);
+        if (!lit.empty()) {
+            HJ_HIP(hipMemcpyAsync(dlit, lit.data(), lit.size(), hipMemcpyHostToDevice, s_));
+            HJ_HIP(hipStreamSynchronize(s_));
+        }
+        pp.lit = dlit;
+        pp.n = tb->num_rows;
+        pp.out = result<int32_t>(pp.n ? pp.n : 1);
+        { Prof p_(ctx_, "predicate"); eval_predicate(pp, s_); }
+        // row structure (pages, row index) of column 0; values from the 0 / 1 array
+        r.pred_view = tb->cols[0].view();
+        r.pred_view.type = HJ_INT32;
+        r.pred_view.flat = pp.out;
+        r.pred_view.vbits = nullptr;
+        r.filters.push_back({kPredColumn, 1, 1, 0});
+    }
+
+    DCol filter_view(const Rel& r, uint32_t col) const {
+        return col == kPredColumn ? r.pred_view : column(r.table, col).view();
     }
 
     ScanSrc scan_src(const Rel& r, const AttrRef& key) const {
@@ -190,7 +315,7 @@ class Executor {
         s.key = column(r.table, key.col).view();
         s.nf = (int32_t)r.filters.size();
         for (size_t f = 0; f < r.filters.size(); ++f) {
-            s.fcol[f] = column(r.table, r.filters[f].column).view();
+            s.fcol[f] = filter_view(r, r.filters[f].column);
             s.flo[f] = r.filters[f].lo;
             s.fhi[f] = r.filters[f].hi;
         }
@@ -462,7 +587,7 @@ class Executor {
             jb.key = bcol.view();
             jb.nf = (int32_t)brel.filters.size();
             for (size_t f = 0; f < brel.filters.size(); ++f) {
-                jb.fcol[f] = column(brel.table, brel.filters[f].column).view();
+                jb.fcol[f] = filter_view(brel, brel.filters[f].column);
                 jb.flo[f] = brel.filters[f].lo;
                 jb.fhi[f] = brel.filters[f].hi;
             }
@@ -522,7 +647,7 @@ class Executor {
 
         hp.nf = (int32_t)srel.filters.size();
         for (size_t f = 0; f < srel.filters.size(); ++f) {
-            hp.fcol[f] = column(srel.table, srel.filters[f].column).view();
+            hp.fcol[f] = filter_view(srel, srel.filters[f].column);
             hp.flo[f] = srel.filters[f].lo;
             hp.fhi[f] = srel.filters[f].hi;
         }
@@ -748,7 +873,7 @@ class Executor {
     hj_dresult* materialize(Rel& r, const hj_node& root) {
         // a filtered scan at the root is compacted to a row-id list first
         if (r.is_scan && !r.filters.empty()) {
-            AttrRef any{0, r.filters[0].column, HJ_INT32};
+            AttrRef any{0, 0, HJ_INT32};  // column 0: the row structure of the pages
             ScanSrc ss = scan_src(r, any);
             uint32_t nblk = ss.key.num_pages;
             uint64_t n = 0;

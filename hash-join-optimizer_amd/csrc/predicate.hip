@@ -1,0 +1,189 @@
+// predicate.hip - load-time scan predicates on the device (see predicate.hpp).
+//
+// Semantics restated from src/statement.cpp:46-230 and include/inner_column.h:250-560: a
+// comparison of a NULL row is 0 (NOT_LIKE and NEQ included), IS_NULL / IS_NOT_NULL read the
+// validity, NOT flips its operand whatever it is. VARCHAR comparisons are byte-lexicographic
+// (std::string_view), LIKE is Comparison::like_match (include/statement.h:118-161): the pattern
+// as a regular expression with '%' -> ".*" and '_' -> ".", fully matched, '.' being one UTF-8
+// character other than '\n'. The match runs as a bit-parallel automaton over the pattern's
+// tokens: bit j = "the first j tokens match the text read so far".
+#include <hip/hip_runtime.h>
+
+#include "../../include/hjoin.h"
+#include "predicate.hpp"
+
+namespace hj {
+namespace {
+
+// sequential reader of one VARCHAR value: a span of a page, or a long string's 0xffff page and
+// its 0xfffe continuation pages
+struct StrCursor {
+    const HJ_GLOBAL uint8_t* p;
+    uint32_t seg;   // bytes left in the current span
+    uint32_t page;  // current page (long strings)
+    uint64_t rem;   // bytes left in the value
+    __device__ inline uint32_t next(const DCol& c) {
+        if (!seg) {
+            ++page;
+            const HJ_GLOBAL uint8_t* pg = page_ptr(c, page);
+            const uint32_t h = *(const HJ_GLOBAL uint32_t*)pg;
+            p = pg + 4;
+            seg = h >> 16;
+        }
+        --seg;
+        --rem;
+        return *p++;
+    }
+};
+
+__device__ inline bool open_string(const DCol& c, uint32_t row, StrCursor* cur) {
+    const uint64_t ref = gp(c.sref)[row];
+    if (ref == kNullRef) return false;
+    if (ref & kLongRef) {
+        const uint32_t q = (uint32_t)ref;
+        const HJ_GLOBAL uint8_t* pg = page_ptr(c, q);
+        cur->p = pg + 4;
+        cur->seg = *(const HJ_GLOBAL uint32_t*)pg >> 16;
+        cur->page = q;
+        cur->rem = long_length(c, q);
+    } else {
+        cur->p = page_ptr(c, 0) + (ref >> 16);
+        cur->seg = (uint32_t)(ref & 0xffffu);
+        cur->page = 0;
+        cur->rem = ref & 0xffffu;
+    }
+    return true;
+}
+
+// one UTF-8 character from the cursor; a byte that does not start a complete sequence (lead
+// 0xC2-0xF4 and its continuation bytes) is a character of its own, 0x110000 + the byte. An
+// incomplete sequence gives back its lead byte alone: the bytes after it are read again.
+__device__ inline uint32_t next_char(const DCol& c, StrCursor* cur) {
+    const uint32_t b = cur->next(c);
+    uint32_t need = 0, cp = b;
+    if (b >= 0xC2 && b <= 0xDF) { need = 1; cp = b & 0x1F; }
+    else if (b >= 0xE0 && b <= 0xEF) { need = 2; cp = b & 0x0F; }
+    else if (b >= 0xF0 && b <= 0xF4) { need = 3; cp = b & 0x07; }
+    else if (b >= 0x80) return 0x110000u + b;
+    if (!need) return cp;
+    StrCursor save = *cur;
+    for (uint32_t k = 0; k < need; ++k) {
+        if (!cur->rem) {
+            *cur = save;
+            return 0x110000u + b;
+        }
+        const uint32_t x = cur->next(c);
+        if ((x & 0xC0) != 0x80) {
+            *cur = save;
+            return 0x110000u + b;
+        }
+        cp = (cp << 6) | (x & 0x3F);
+    }
+    return cp;
+}
+
+__device__ inline uint64_t like_closure(uint64_t d, uint64_t pct) {
+    for (;;) {
+        const uint64_t e = d | ((d << 1) & pct);
+        if (e == d) return d;
+        d = e;
+    }
+}
+
+__device__ bool like_match(const DCol& c, StrCursor cur, const HJ_GLOBAL uint32_t* tok, uint32_t m,
+                           uint64_t pct, uint64_t any) {
+    uint64_t d = like_closure(1ull, pct);
+    while (cur.rem && d) {
+        const uint32_t ch = next_char(c, &cur);
+        uint64_t lit = 0;
+        for (uint32_t j = 0; j < m; ++j) lit |= (uint64_t)(tok[j] == ch) << (j + 1);
+        const uint64_t nl = ch == '\n' ? 0ull : ~0ull;
+        d = like_closure(((d << 1) & (lit | (any & nl))) | (d & pct & nl), pct);
+    }
+    return (d >> m) & 1ull;
+}
+
+// -1 / 0 / 1: the value against lit[0, ll), bytes as unsigned, a proper prefix first
+__device__ int compare_string(const DCol& c, StrCursor cur, const HJ_GLOBAL uint8_t* lit, uint32_t ll) {
+    uint32_t i = 0;
+    while (cur.rem && i < ll) {
+        const uint32_t a = cur.next(c), b = lit[i++];
+        if (a != b) return a < b ? -1 : 1;
+    }
+    if (cur.rem) return 1;
+    return i < ll ? -1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_predicate(PredProg p) {
+    const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= p.n) return;
+    const uint32_t r = (uint32_t)row;
+    uint32_t st = 0;  // bit stack, top = bit 0
+    for (int k = 0; k < p.nops; ++k) {
+        const PredOp& o = p.op[k];
+        if (o.op == HJ_P_AND) {
+            const uint32_t a = st & 1u;
+            st >>= 1;
+            st = (st & ~1u) | (st & a);
+            continue;
+        }
+        if (o.op == HJ_P_OR) {
+            const uint32_t a = st & 1u;
+            st >>= 1;
+            st |= a;
+            continue;
+        }
+        if (o.op == HJ_P_NOT) {
+            st ^= 1u;
+            continue;
+        }
+        const DCol& c = p.col[o.col];
+        bool b = false;
+        if (c.type == HJ_INT32) {
+            int32_t v = 0;
+            const bool ok = get_int32(c, r, &v);
+            switch (o.op) {
+                case HJ_P_IS_NULL: b = !ok; break;
+                case HJ_P_IS_NOT_NULL: b = ok; break;
+                case HJ_P_EQ: b = ok && v == o.ival; break;
+                case HJ_P_NEQ: b = ok && v != o.ival; break;
+                case HJ_P_LT: b = ok && v < o.ival; break;
+                case HJ_P_GT: b = ok && v > o.ival; break;
+                case HJ_P_LEQ: b = ok && v <= o.ival; break;
+                default: b = ok && v >= o.ival; break;
+            }
+        } else {
+            StrCursor cur;
+            const bool ok = open_string(c, r, &cur);
+            if (o.op == HJ_P_IS_NULL) b = !ok;
+            else if (o.op == HJ_P_IS_NOT_NULL) b = ok;
+            else if (!ok) b = false;
+            else if (o.op == HJ_P_LIKE || o.op == HJ_P_NOT_LIKE) {
+                const bool mt = like_match(c, cur, (const HJ_GLOBAL uint32_t*)gp(p.lit + o.loff),
+                                           o.llen, o.pct, o.any);
+                b = o.op == HJ_P_LIKE ? mt : !mt;
+            } else {
+                const int cmp = compare_string(c, cur, gp(p.lit + o.loff), o.llen);
+                switch (o.op) {
+                    case HJ_P_EQ: b = cmp == 0; break;
+                    case HJ_P_NEQ: b = cmp != 0; break;
+                    case HJ_P_LT: b = cmp < 0; break;
+                    case HJ_P_GT: b = cmp > 0; break;
+                    case HJ_P_LEQ: b = cmp <= 0; break;
+                    default: b = cmp >= 0; break;
+                }
+            }
+        }
+        st = (st << 1) | (b ? 1u : 0u);
+    }
+    gp(p.out)[row] = (int32_t)(st & 1u);
+}
+
+}  // namespace
+
+void eval_predicate(const PredProg& p, hipStream_t s) {
+    if (!p.n) return;
+    hipLaunchKernelGGL(k_predicate, dim3((uint32_t)((p.n + 255) / 256)), dim3(256), 0, s, p);
+}
+
+}  // namespace hj

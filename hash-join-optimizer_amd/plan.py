@@ -45,11 +45,16 @@ class ColumnarTable:
 
 
 class ScanNode:
-    """include/plan.h:78-80, plus engine-side range filters (lo <= col <= hi, NULL fails)."""
+    """include/plan.h:78-80, plus engine-side range filters (lo <= col <= hi, NULL fails) and a
+    predicate program: the reference's Comparison / LogicalOperation tree (include/statement.h)
+    in postfix order, entries ("eq"|"neq"|"lt"|"gt"|"leq"|"geq", col, literal),
+    ("like"|"not_like", col, pattern), ("is_null"|"is_not_null", col), ("and",), ("or",),
+    ("not",)."""
 
-    def __init__(self, base_table_id, filters=()):
+    def __init__(self, base_table_id, filters=(), preds=()):
         self.base_table_id = int(base_table_id)
         self.filters = [(int(c), int(lo), int(hi)) for c, lo, hi in filters]
+        self.preds = [tuple(p) for p in preds]
 
 
 class JoinNode:
@@ -84,8 +89,8 @@ class Plan:
                                    output_attrs))
         return len(self.nodes) - 1
 
-    def new_scan_node(self, base_table_id, output_attrs, filters=()):
-        self.nodes.append(PlanNode(ScanNode(base_table_id, filters), output_attrs))
+    def new_scan_node(self, base_table_id, output_attrs, filters=(), preds=()):
+        self.nodes.append(PlanNode(ScanNode(base_table_id, filters, preds), output_attrs))
         return len(self.nodes) - 1
 
     def new_input(self, table):
@@ -115,6 +120,24 @@ class Plan:
                 keep.append(fl)
                 n.num_filters = len(nd.data.filters)
                 n.filters = fl
+                pr = (abi.hj_pred * max(1, len(nd.data.preds)))()
+                for j, e in enumerate(nd.data.preds):
+                    pr[j].op = abi.PRED_OPS[e[0]]
+                    if len(e) > 1:
+                        pr[j].column = int(e[1])
+                    if len(e) > 2:
+                        lit = e[2]
+                        if isinstance(lit, (str, bytes)):
+                            b = lit.encode() if isinstance(lit, str) else bytes(lit)
+                            buf = C.create_string_buffer(b, len(b) + 1)
+                            keep.append(buf)
+                            pr[j].sval = C.cast(buf, C.c_char_p)
+                            pr[j].slen = len(b)
+                        else:
+                            pr[j].ival = int(lit)
+                keep.append(pr)
+                n.num_preds = len(nd.data.preds)
+                n.preds = pr
             else:
                 d = nd.data
                 n.kind = abi.NODE_JOIN

@@ -80,6 +80,29 @@ typedef struct hj_filter {
     int32_t reserved;
 } hj_filter;
 
+/* Engine extension with no counterpart in Plan: a load-time predicate pushed into a scan, the
+ * reference's Comparison / LogicalOperation trees (include/statement.h:47-270) written in postfix
+ * order and evaluated per row as the reference's column-level eval does (src/statement.cpp:46-200,
+ * include/inner_column.h:250-560): a comparison of a NULL row is false, NOT negates whatever it
+ * gets (NOT (x = 1) keeps NULL rows), AND / OR take the two results below them. INT32 columns
+ * compare with (int32_t)ival; VARCHAR columns compare bytes lexicographically with sval[0,slen)
+ * and LIKE matches the whole string, '%' any run and '_' one character (UTF-8, neither crossing a
+ * newline), as the reference's RE2 translation does (include/statement.h:118-161). A scan keeps a
+ * row iff its filters pass and the program leaves 1. At most 32 entries, stack depth 32, LIKE
+ * patterns of at most 63 characters. */
+enum {
+    HJ_P_EQ = 0, HJ_P_NEQ = 1, HJ_P_LT = 2, HJ_P_GT = 3, HJ_P_LEQ = 4, HJ_P_GEQ = 5,
+    HJ_P_LIKE = 6, HJ_P_NOT_LIKE = 7, HJ_P_IS_NULL = 8, HJ_P_IS_NOT_NULL = 9,
+    HJ_P_AND = 16, HJ_P_OR = 17, HJ_P_NOT = 18
+};
+typedef struct hj_pred {
+    int32_t op;
+    uint32_t column;   /* comparisons: column of the scanned table */
+    int64_t ival;      /* INT32 column: the literal */
+    const char* sval;  /* VARCHAR column: the literal or LIKE pattern, slen bytes */
+    uint64_t slen;
+} hj_pred;
+
 /* PlanNode (include/plan.h:78-98). kind selects ScanNode or JoinNode fields. */
 typedef struct hj_node {
     int32_t kind;
@@ -91,6 +114,8 @@ typedef struct hj_node {
     const hj_attr* output_attrs;
     uint64_t num_filters;        /* scan only; 0 for a plain reference plan */
     const hj_filter* filters;
+    uint64_t num_preds;          /* scan only: a predicate program (hj_pred), 0 for none */
+    const hj_pred* preds;
 } hj_node;
 
 /* Plan (include/plan.h:173-210) */

@@ -14,6 +14,8 @@
  *   execute / execute_impl ............ src/execute.cpp:584-608
  *   Table::to_columnar / from_columnar  src/build_table.cpp:513-882
  *   load-time INT32 range filter ...... src/build_table.cpp:319-371, include/inner_column.h:250-300
+ *   scan predicates (Comparison, LIKE,  src/statement.cpp:46-230, include/statement.h:47-270,
+ *     AND/OR/NOT) ..................... include/inner_column.h:330-560
  *   UnchainedHash ..................... include/unchained.h:14-247
  *   DumpTable::dump / Table::from_cache include/table.h:30-37,128-211, src/build_table.cpp:147-203
  * Not restated: the slab allocator's chunk order (include/slab_allocator.h), which only fixes the
@@ -1052,16 +1054,181 @@ static vcol* execute_impl(exec_ctx* x, uint64_t node_idx) {
     return res;
 }
 
+/* ------------------------------------------------------------------ scan predicates */
+/* one character of s[i, n) in UTF-8: its code point and byte length; a byte that does not start
+ * a complete sequence (lead 0xC2-0xF4 followed by its continuation bytes) is a character of its
+ * own, 0x110000 + the byte */
+static uint32_t utf8_char(const unsigned char* s, uint64_t n, uint64_t i, uint32_t* len) {
+    const unsigned b = s[i];
+    uint32_t need = 0, cp = b;
+    if (b >= 0xC2 && b <= 0xDF) { need = 1; cp = b & 0x1F; }
+    else if (b >= 0xE0 && b <= 0xEF) { need = 2; cp = b & 0x0F; }
+    else if (b >= 0xF0 && b <= 0xF4) { need = 3; cp = b & 0x07; }
+    else if (b >= 0x80) { *len = 1; return 0x110000u + b; }
+    for (uint32_t k = 1; k <= need; ++k) {
+        if (i + k >= n || (s[i + k] & 0xC0) != 0x80) { *len = 1; return 0x110000u + b; }
+        cp = (cp << 6) | (s[i + k] & 0x3F);
+    }
+    *len = need + 1;
+    return cp;
+}
+
+#define LIKE_ANY_RUN 0xFFFFFFFFu /* '%' */
+#define LIKE_ANY_ONE 0xFFFFFFFEu /* '_' */
+
+/* Comparison::like_match (include/statement.h:118-161): '%' -> ".*", '_' -> ".", other pattern
+ * characters literal, the whole string matched by RE2, whose '.' is one UTF-8 character other
+ * than '\n'. Restated as a dynamic program over (text characters, pattern characters). */
+int oracle_like(const char* str, uint64_t slen, const char* pat, uint64_t plen) {
+    const unsigned char* p = (const unsigned char*)pat;
+    const unsigned char* t = (const unsigned char*)str;
+    uint32_t* tok = (uint32_t*)xmalloc((plen + 1) * 4);
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < plen;) {
+        uint32_t l;
+        const uint32_t cp = utf8_char(p, plen, i, &l);
+        tok[m++] = p[i] == '%' ? LIKE_ANY_RUN : p[i] == '_' ? LIKE_ANY_ONE : cp;
+        i += l;
+    }
+    uint8_t* cur = (uint8_t*)xcalloc(m + 1, 1);
+    uint8_t* nxt = (uint8_t*)xcalloc(m + 1, 1);
+    cur[0] = 1;
+    for (uint64_t j = 1; j <= m; ++j) cur[j] = cur[j - 1] && tok[j - 1] == LIKE_ANY_RUN;
+    for (uint64_t i = 0; i < slen;) {
+        uint32_t l;
+        const uint32_t c = utf8_char(t, slen, i, &l);
+        i += l;
+        nxt[0] = 0;
+        for (uint64_t j = 1; j <= m; ++j) {
+            const uint32_t k = tok[j - 1];
+            if (k == LIKE_ANY_RUN)
+                nxt[j] = nxt[j - 1] || (cur[j] && c != '\n');
+            else if (k == LIKE_ANY_ONE)
+                nxt[j] = cur[j - 1] && c != '\n';
+            else
+                nxt[j] = cur[j - 1] && c == k;
+        }
+        uint8_t* sw = cur;
+        cur = nxt;
+        nxt = sw;
+    }
+    const int r = cur[m];
+    free(tok);
+    free(cur);
+    free(nxt);
+    return r;
+}
+
+typedef struct {
+    int32_t* v;      /* INT32 */
+    uint64_t* off;   /* VARCHAR */
+    char* ch;
+    uint8_t* ok;
+} pred_col;
+
+/* keep[r] &= the program's result for row r (src/statement.cpp:46-230: a comparison of a NULL
+ * row is 0, IS_NULL / IS_NOT_NULL read the validity, AND / OR / NOT combine the row's bits) */
+static int apply_preds(const hj_table* in, const hj_pred* pr, uint64_t np, uint8_t* keep) {
+    const uint64_t n = in->num_rows;
+    if (np > 32) return HJ_E_INVALID;
+    pred_col cols[32];
+    memset(cols, 0, sizeof cols);
+    int depth = 0, rc = 0;
+    for (uint64_t k = 0; k < np && rc == 0; ++k) {
+        const int op = pr[k].op;
+        if (op == HJ_P_AND || op == HJ_P_OR) {
+            if (depth < 2) rc = HJ_E_INVALID;
+            --depth;
+            continue;
+        }
+        if (op == HJ_P_NOT) {
+            if (depth < 1) rc = HJ_E_INVALID;
+            continue;
+        }
+        if (op < HJ_P_EQ || op > HJ_P_IS_NOT_NULL || pr[k].column >= in->num_columns) {
+            rc = HJ_E_INVALID;
+            break;
+        }
+        const hj_column* col = &in->columns[pr[k].column];
+        if (col->type != HJ_INT32 && col->type != HJ_VARCHAR) rc = HJ_E_TYPE;
+        if ((op == HJ_P_LIKE || op == HJ_P_NOT_LIKE) && col->type != HJ_VARCHAR) rc = HJ_E_TYPE;
+        if (col->type == HJ_VARCHAR && op < HJ_P_IS_NULL && !pr[k].sval && pr[k].slen) rc = HJ_E_INVALID;
+        if (++depth > 32) rc = HJ_E_INVALID;
+        if (rc) break;
+        pred_col* pc = &cols[k];
+        pc->ok = (uint8_t*)xmalloc(n + 1);
+        if (col->type == HJ_INT32) {
+            pc->v = (int32_t*)xmalloc(n * 4 + 4);
+            oracle_decode_int32(col, pc->v, pc->ok);
+        } else {
+            pc->off = (uint64_t*)xmalloc((n + 1) * 8);
+            pc->ch = (char*)xmalloc(oracle_column_chars(col) + 1);
+            oracle_decode_varchar(col, pc->off, pc->ch, pc->ok);
+        }
+    }
+    if (rc == 0 && depth != 1 && np) rc = HJ_E_INVALID;
+    for (uint64_t r = 0; rc == 0 && np && r < n; ++r) {
+        uint8_t st[32];
+        int sp = 0;
+        for (uint64_t k = 0; k < np; ++k) {
+            const int op = pr[k].op;
+            if (op == HJ_P_AND) { --sp; st[sp - 1] = st[sp - 1] && st[sp]; continue; }
+            if (op == HJ_P_OR) { --sp; st[sp - 1] = st[sp - 1] || st[sp]; continue; }
+            if (op == HJ_P_NOT) { st[sp - 1] = !st[sp - 1]; continue; }
+            const pred_col* pc = &cols[k];
+            const int nn = pc->ok[r] != 0;
+            int b = 0;
+            if (op == HJ_P_IS_NULL) b = !nn;
+            else if (op == HJ_P_IS_NOT_NULL) b = nn;
+            else if (!nn) b = 0;
+            else if (pc->v) {
+                const int32_t x = pc->v[r], y = (int32_t)pr[k].ival;
+                b = op == HJ_P_EQ ? x == y : op == HJ_P_NEQ ? x != y : op == HJ_P_LT ? x < y
+                  : op == HJ_P_GT ? x > y : op == HJ_P_LEQ ? x <= y : x >= y;
+            } else {
+                const char* x = pc->ch + pc->off[r];
+                const uint64_t xl = pc->off[r + 1] - pc->off[r], yl = pr[k].slen;
+                if (op == HJ_P_LIKE || op == HJ_P_NOT_LIKE) {
+                    const int mt = oracle_like(x, xl, pr[k].sval ? pr[k].sval : "", yl);
+                    b = op == HJ_P_LIKE ? mt : !mt;
+                } else {
+                    int c = memcmp(x, pr[k].sval ? pr[k].sval : "", xl < yl ? xl : yl);
+                    if (c == 0) c = xl < yl ? -1 : xl > yl ? 1 : 0;
+                    b = op == HJ_P_EQ ? c == 0 : op == HJ_P_NEQ ? c != 0 : op == HJ_P_LT ? c < 0
+                      : op == HJ_P_GT ? c > 0 : op == HJ_P_LEQ ? c <= 0 : c >= 0;
+                }
+            }
+            st[sp++] = (uint8_t)b;
+        }
+        keep[r] &= st[0];
+    }
+    for (uint64_t k = 0; k < np && k < 32; ++k) {
+        free(cols[k].v);
+        free(cols[k].off);
+        free(cols[k].ch);
+        free(cols[k].ok);
+    }
+    return rc;
+}
+
 /* ------------------------------------------------------------------ load-time filter */
 /* Rows pass iff every filter column is non-NULL and lo <= v <= hi (include/inner_column.h:
  * 258-259 ANDs the validity bitmap into every comparison; BETWEEN = GEQ AND LEQ,
  * tests/read_sql.cpp:585-591). Passing rows are re-encoded with the writer rules, keeping row
  * order and NULLs of the other columns (src/build_table.cpp:104-129, 319-371). */
-static int filter_table(const hj_table* in, const hj_filter* f, uint64_t nf, hj_table* out,
-                        hj_column** out_cols, const uint8_t*** out_ptrs, uint8_t** out_blocks) {
+static int filter_table(const hj_table* in, const hj_filter* f, uint64_t nf, const hj_pred* pr,
+                        uint64_t np, hj_table* out, hj_column** out_cols,
+                        const uint8_t*** out_ptrs, uint8_t** out_blocks) {
     uint64_t n = in->num_rows;
     uint8_t* keep = (uint8_t*)xmalloc(n ? n : 1);
     memset(keep, 1, n);
+    {
+        const int prc = apply_preds(in, pr, np, keep);
+        if (prc) {
+            free(keep);
+            return prc;
+        }
+    }
     for (uint64_t k = 0; k < nf; ++k) {
         if (f[k].column >= in->num_columns || in->columns[f[k].column].type != HJ_INT32) {
             free(keep);
@@ -1178,7 +1345,7 @@ int oracle_execute(const hj_plan* plan, int threads, hj_result* out, uint64_t* e
         const hj_node* nd = &plan->nodes[i];
         if (nd->kind != HJ_NODE_SCAN) continue;
         scan_table[i] = nd->base_table_id;
-        if (nd->num_filters == 0) continue;
+        if (nd->num_filters == 0 && nd->num_preds == 0) continue;
         if (nd->base_table_id >= nin) {
             rc = HJ_E_INVALID;
             break;
@@ -1187,6 +1354,7 @@ int oracle_execute(const hj_plan* plan, int threads, hj_result* out, uint64_t* e
         const uint8_t** ptrs;
         uint8_t* blocks;
         rc = filter_table(&plan->inputs[nd->base_table_id], nd->filters, nd->num_filters,
+                          nd->preds, nd->num_preds,
                           &inputs[nin], &cols, &ptrs, &blocks);
         if (rc) break;
         owned_ncols[nowned / 3] = plan->inputs[nd->base_table_id].num_columns;
@@ -1280,7 +1448,7 @@ int oracle_nested_loop_execute(const hj_plan* plan, hj_result* out, char* err, s
         const hj_node* nd = &plan->nodes[i];
         if (nd->kind != HJ_NODE_SCAN) continue;
         scan_table[i] = nd->base_table_id;
-        if (!nd->num_filters) continue;
+        if (!nd->num_filters && !nd->num_preds) continue;
         if (nowned >= 3 * 64) {
             rc = HJ_E_INVALID;
             break;
@@ -1289,6 +1457,7 @@ int oracle_nested_loop_execute(const hj_plan* plan, hj_result* out, char* err, s
         const uint8_t** ptrs;
         uint8_t* blocks;
         rc = filter_table(&plan->inputs[nd->base_table_id], nd->filters, nd->num_filters,
+                          nd->preds, nd->num_preds,
                           &inputs[nin], &cols, &ptrs, &blocks);
         if (rc) break;
         owned_nc[nowned / 3] = plan->inputs[nd->base_table_id].num_columns;

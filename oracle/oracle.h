@@ -52,6 +52,9 @@ uint64_t oracle_column_chars(const hj_column* col);
 int oracle_decode_int32(const hj_column* col, int32_t* values, uint8_t* validity);
 int oracle_decode_varchar(const hj_column* col, uint64_t* offsets, char* chars, uint8_t* validity);
 
+/* Comparison::like_match (include/statement.h:118-161): 1 iff str matches the LIKE pattern */
+int oracle_like(const char* str, uint64_t slen, const char* pat, uint64_t plen);
+
 /* .tbl cache files: DumpTable::dump (include/table.h:128-211) and Table::from_cache
  * (src/build_table.cpp:147-203). The header page is TableMeta at byte 0 - num_rows (u64) at 0,
  * num_cols (u64) at 8, types (16 x int32) at 16, num_pages (16 x u64) at 80, 208 bytes - and

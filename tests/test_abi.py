@@ -37,7 +37,8 @@ def test_struct_layouts(hjo):
     assert C.sizeof(a.hj_table) == 24
     assert C.sizeof(a.hj_attr) == 16
     assert C.sizeof(a.hj_filter) == 16
-    assert C.sizeof(a.hj_node) == 80
+    assert C.sizeof(a.hj_node) == 96
+    assert C.sizeof(a.hj_pred) == 32
     assert C.sizeof(a.hj_plan) == 40
     assert C.sizeof(a.hj_out_column) == 16
     assert C.sizeof(a.hj_result) == 24
