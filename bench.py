@@ -165,6 +165,32 @@ def result_pages(ctx, plan, dts, reps=3):
     return out
 
 
+# ------------------------------------------------------------------ scan predicates
+def scan_predicate(ctx, dts, d, reps=3):
+    """a LIKE '%ab%' scan predicate over the 4M-row dimension's VARCHAR column (star4 only):
+    the predicate kernel's event time and its algorithmic bytes (8-byte string reference + the
+    string's bytes per row)"""
+    dm = d["dims"][1]
+    n = len(dm["key"])
+    plan = hjo.Plan()
+    plan.inputs = [None, None, None, None]
+    plan.new_scan_node(2, [(0, hjo.DataType.INT32)], preds=[("like", 2, "%ab%")])
+    plan.root = 0
+    ctx.profile("predicate")
+    rows = 0
+    for _ in range(reps):
+        r = ctx.execute_device(plan, dts)
+        rows = r.num_rows
+        r.free()
+    prof = ctx.profile_report()
+    ctx.profile(None)
+    launches, ms = prof.get("predicate", (0, 0.0))
+    byts = 8 * n + int(dm["soff"][-1])
+    us = ms / max(1, launches) * 1e3
+    return {"rows": n, "pattern": "%ab%", "kept": rows, "kernel_us": us,
+            "alg_bytes": byts, "GB/s": byts / (us * 1e-6) / 1e9 if us else None}
+
+
 # ------------------------------------------------------------------ CPU baseline (oracle)
 def cpu_baseline(d, workload_name, tuples, threads):
     """the oracle (CPU restatement of the reference's execute()), used only as the baseline"""
@@ -274,6 +300,9 @@ def main():
 
     pages = result_pages(ctx, plan, dts)
     log(f"result pages: {pages}")
+    pred = scan_predicate(ctx, dts, d) if args.workload == "star4" else None
+    if pred:
+        log(f"scan predicate: {pred}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -289,6 +318,7 @@ def main():
         "alg_bytes_per_step": info["b_alg"], "output_rows_per_s": info["out_rows"] / step_s,
         "kernel_ms_per_step": {k: v[1] for k, v in per_step.items()},
         "roofline": roofline, "cpu_baseline": cpu, "result_pages": pages,
+        "scan_predicate": pred,
     }
     for t_ in dts:
         t_.free()

@@ -16,29 +16,43 @@ namespace hj {
 namespace {
 
 // sequential reader of one VARCHAR value: a span of a page, or a long string's 0xffff page and
-// its 0xfffe continuation pages
+// its 0xfffe continuation pages. Bytes come 8 at a time from aligned 8-byte loads (the page
+// buffer is 8-byte aligned with slack after its end, so such a load stays inside it).
 struct StrCursor {
-    const HJ_GLOBAL uint8_t* p;
-    uint32_t seg;   // bytes left in the current span
+    const HJ_GLOBAL uint8_t* p;  // next byte not yet in buf
+    uint32_t seg;   // bytes of the current span not yet in buf
     uint32_t page;  // current page (long strings)
     uint64_t rem;   // bytes left in the value
+    uint64_t buf;   // buffered bytes, next in the low byte
+    uint32_t nbuf;
     __device__ inline uint32_t next(const DCol& c) {
-        if (!seg) {
-            ++page;
-            const HJ_GLOBAL uint8_t* pg = page_ptr(c, page);
-            const uint32_t h = *(const HJ_GLOBAL uint32_t*)pg;
-            p = pg + 4;
-            seg = h >> 16;
+        if (!nbuf) {
+            if (!seg) {
+                ++page;
+                const HJ_GLOBAL uint8_t* pg = page_ptr(c, page);
+                const uint32_t h = *(const HJ_GLOBAL uint32_t*)pg;
+                p = pg + 4;
+                seg = h >> 16;
+            }
+            const uintptr_t a = (uintptr_t)p;
+            const uint32_t sh = (uint32_t)(a & 7u);
+            buf = *(const HJ_GLOBAL uint64_t*)(a - sh) >> (8u * sh);
+            nbuf = 8u - sh < seg ? 8u - sh : seg;
+            p += nbuf;
+            seg -= nbuf;
         }
-        --seg;
+        const uint32_t b = (uint32_t)(buf & 0xffu);
+        buf >>= 8;
+        --nbuf;
         --rem;
-        return *p++;
+        return b;
     }
 };
 
 __device__ inline bool open_string(const DCol& c, uint32_t row, StrCursor* cur) {
     const uint64_t ref = gp(c.sref)[row];
     if (ref == kNullRef) return false;
+    cur->nbuf = 0;
     if (ref & kLongRef) {
         const uint32_t q = (uint32_t)ref;
         const HJ_GLOBAL uint8_t* pg = page_ptr(c, q);
@@ -82,23 +96,26 @@ __device__ inline uint32_t next_char(const DCol& c, StrCursor* cur) {
     return cp;
 }
 
-__device__ inline uint64_t like_closure(uint64_t d, uint64_t pct) {
-    for (;;) {
-        const uint64_t e = d | ((d << 1) & pct);
-        if (e == d) return d;
-        d = e;
-    }
+// '%' tokens match the empty run: pattern prefixes ending in them extend for free; runs = the
+// longest run of consecutive '%' tokens
+__device__ inline uint64_t like_closure(uint64_t d, uint64_t pct, uint32_t runs) {
+    for (uint32_t i = 0; i < runs; ++i) d |= (d << 1) & pct;
+    return d;
 }
 
+// ascii: for characters below 128, the pattern positions they match (from LDS)
 __device__ bool like_match(const DCol& c, StrCursor cur, const HJ_GLOBAL uint32_t* tok, uint32_t m,
-                           uint64_t pct, uint64_t any) {
-    uint64_t d = like_closure(1ull, pct);
+                           uint64_t pct, uint64_t any, uint32_t runs, const uint64_t* ascii) {
+    uint64_t d = like_closure(1ull, pct, runs);
     while (cur.rem && d) {
         const uint32_t ch = next_char(c, &cur);
         uint64_t lit = 0;
-        for (uint32_t j = 0; j < m; ++j) lit |= (uint64_t)(tok[j] == ch) << (j + 1);
+        if (ascii && ch < 128)
+            lit = ascii[ch];
+        else
+            for (uint32_t j = 0; j < m; ++j) lit |= (uint64_t)(tok[j] == ch) << (j + 1);
         const uint64_t nl = ch == '\n' ? 0ull : ~0ull;
-        d = like_closure(((d << 1) & (lit | (any & nl))) | (d & pct & nl), pct);
+        d = like_closure(((d << 1) & (lit | (any & nl))) | (d & pct & nl), pct, runs);
     }
     return (d >> m) & 1ull;
 }
@@ -114,9 +131,30 @@ __device__ int compare_string(const DCol& c, StrCursor cur, const HJ_GLOBAL uint
     return i < ll ? -1 : 0;
 }
 
+constexpr int kLikeTables = 4;  // LIKE entries with an ASCII table in LDS
+
 __global__ __launch_bounds__(256) void k_predicate(PredProg p) {
+    // per LIKE entry (the first kLikeTables): the pattern positions each ASCII character matches
+    __shared__ uint64_t ascii[kLikeTables][128];
+    {
+        int nt = 0;  // LIKE entries seen so far: the i-th one uses table i
+        for (int k = 0; k < p.nops && nt < kLikeTables; ++k) {
+            const int op = p.op[k].op;
+            if (op != HJ_P_LIKE && op != HJ_P_NOT_LIKE) continue;
+            if (threadIdx.x < 128) {
+                const HJ_GLOBAL uint32_t* tok = (const HJ_GLOBAL uint32_t*)gp(p.lit + p.op[k].loff);
+                uint64_t lit = 0;
+                for (uint32_t j = 0; j < p.op[k].llen; ++j)
+                    lit |= (uint64_t)(tok[j] == threadIdx.x) << (j + 1);
+                ascii[nt][threadIdx.x] = lit;
+            }
+            ++nt;
+        }
+    }
+    __syncthreads();
     const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= p.n) return;
+    int nl = 0;  // LIKE entries evaluated so far
     const uint32_t r = (uint32_t)row;
     uint32_t st = 0;  // bit stack, top = bit 0
     for (int k = 0; k < p.nops; ++k) {
@@ -138,6 +176,9 @@ __global__ __launch_bounds__(256) void k_predicate(PredProg p) {
             continue;
         }
         const DCol& c = p.col[o.col];
+        const bool like = o.op == HJ_P_LIKE || o.op == HJ_P_NOT_LIKE;
+        const int li = nl;  // this entry's ASCII table, when it has one
+        nl += like ? 1 : 0;
         bool b = false;
         if (c.type == HJ_INT32) {
             int32_t v = 0;
@@ -158,9 +199,10 @@ __global__ __launch_bounds__(256) void k_predicate(PredProg p) {
             if (o.op == HJ_P_IS_NULL) b = !ok;
             else if (o.op == HJ_P_IS_NOT_NULL) b = ok;
             else if (!ok) b = false;
-            else if (o.op == HJ_P_LIKE || o.op == HJ_P_NOT_LIKE) {
+            else if (like) {
                 const bool mt = like_match(c, cur, (const HJ_GLOBAL uint32_t*)gp(p.lit + o.loff),
-                                           o.llen, o.pct, o.any);
+                                           o.llen, o.pct, o.any, o.pad,
+                                           li < kLikeTables ? ascii[li] : nullptr);
                 b = o.op == HJ_P_LIKE ? mt : !mt;
             } else {
                 const int cmp = compare_string(c, cur, gp(p.lit + o.loff), o.llen);

@@ -25,7 +25,7 @@ struct PredOp {
     int32_t ival;    // INT32 literal
     uint32_t loff;   // VARCHAR: literal bytes at lit + loff, llen of them; LIKE: tokens at
     uint32_t llen;   //   (const uint32_t*)(lit + loff), llen of them
-    uint32_t pad;
+    uint32_t pad;    // LIKE: the longest run of consecutive '%' tokens
     uint64_t pct;    // LIKE: bit j set iff token j - 1 is '%'
     uint64_t any;    // LIKE: bit j set iff token j - 1 is '_'
 };
