@@ -93,6 +93,98 @@ def two_way_workload(codec, scale):
     return d, plan.inputs, plan, info, workload
 
 
+def serve_workload(codec, args):
+    """Serving: star joins (1-3 of the resident dimensions, 1-20 % range filters each) over a
+    fact table drawn log-uniformly in [100K, 36M] rows (the nearest of the resident facts:
+    100K, 1M, 4M, 16M, 36M rows), all tables resident. Measures queries/s of a batch submitted
+    at once and p50 / p99 latency of Poisson arrivals, for the plain one-query-at-a-time path
+    (one slot) and the served path (several slots)."""
+    rng = np.random.default_rng(47)
+    dims = dg.STAR_DIMS
+    fact_sizes = [100_000, 1_000_000, 4_000_000, 16_000_000, 36_000_000]
+    t = time.time()
+    d = dg.star4(seed=47, n_fact=max(fact_sizes), dims=dims)
+    dim_tables = dg.star4_tables(d, codec)[1:]
+    facts = []
+    for n in fact_sizes:  # prefixes of the largest fact table
+        facts.append(hjo.ColumnarTable(n, [codec.encode_int32(fk[:n], v[:n])
+                                           for fk, v in zip(d["fact"], d["fact_valid"])]))
+    log(f"serve tables generated+encoded in {time.time() - t:.1f}s")
+    I32, STR = hjo.DataType.INT32, hjo.DataType.VARCHAR
+    queries = []
+    for _ in range(args.queries):
+        n = float(np.exp(rng.uniform(np.log(100_000), np.log(36_000_000))))
+        fi = int(np.argmin([abs(np.log(n / f)) for f in fact_sizes]))
+        k = int(rng.integers(1, 4))
+        plan = hjo.Plan()
+        plan.inputs = [None] * (1 + len(fact_sizes) + len(dims))
+        cur = plan.new_scan_node(fi, [(0, I32), (1, I32), (2, I32)])
+        types = [I32, I32, I32]
+        for j in range(k):
+            w = int(rng.integers(100, 2001))
+            lo = int(rng.integers(0, 10_000 - w))
+            ds = plan.new_scan_node(len(fact_sizes) + j, [(0, I32), (2, STR)],
+                                    [(1, lo, lo + w - 1)])
+            cur = plan.new_join_node(False, cur, ds, j, 0,
+                                     [(i, ty) for i, ty in enumerate(types)] + [(len(types) + 1, STR)])
+            types.append(STR)
+        plan.root = cur
+        queries.append(plan)
+    return facts + dim_tables, queries
+
+
+def run_served(srv, dts, queries, rate=None, seed=0):
+    """submit every query (at once, or at Poisson arrival times of `rate` per second), wait for
+    all; returns (wall seconds, latencies in ms)"""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(queries)) if rate else np.zeros(len(queries))
+    t0 = time.perf_counter()
+    due = t0
+    tickets = []
+    for q, g in zip(queries, gaps):
+        due += g
+        while rate and time.perf_counter() < due:
+            pass
+        tickets.append(srv.submit(q, dts))
+    lat = []
+    for tk in tickets:
+        _, q_ms, r_ms = srv.wait(tk)
+        lat.append(q_ms + r_ms)
+    return time.perf_counter() - t0, lat
+
+
+def serve_main(args):
+    codec = hjo.Codec()
+    tables, queries = serve_workload(codec, args)
+    ctx = hjo.Context()
+    dts = [ctx.upload(t) for t in tables]
+    out = {"metric": "batched queries/s", "unit": "queries/s", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic, seeded",
+           "config": {"workload": "served star joins: 1-3 resident dimensions (2.5M/4M/100K rows, "
+                                  "1-20 % range filters), fact 100K-36M rows log-uniform",
+                      "queries": len(queries), "slots": args.slots, "poisson_rate": args.rate}}
+    for name, slots in (("plain", 1), ("served", args.slots)):
+        with hjo.Server(slots=slots, arena_bytes=8 << 30) as srv:
+            run_served(srv, dts, queries[:8])  # warm-up
+            wall, lat = run_served(srv, dts, queries)
+            out[f"{name}_qps"] = len(queries) / wall
+            out[f"{name}_batch_p50_ms"] = float(np.percentile(lat, 50))
+            out[f"{name}_batch_p99_ms"] = float(np.percentile(lat, 99))
+            wall, lat = run_served(srv, dts, queries, rate=args.rate, seed=1)
+            out[f"{name}_poisson_p50_ms"] = float(np.percentile(lat, 50))
+            out[f"{name}_poisson_p99_ms"] = float(np.percentile(lat, 99))
+            out[f"{name}_poisson_qps"] = len(queries) / wall
+        log(f"{name} ({slots} slots): {out[name + '_qps']:.1f} q/s batched; Poisson {args.rate}/s "
+            f"p50 {out[name + '_poisson_p50_ms']:.2f} ms p99 {out[name + '_poisson_p99_ms']:.2f} ms")
+    out["value"] = out["served_qps"]
+    out["speedup_vs_plain"] = out["served_qps"] / out["plain_qps"]
+    for t_ in dts:
+        t_.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 # ------------------------------------------------------------------ kernel byte model
 def kernel_bytes(name, info, launches):
     """algorithmic HBM bytes of all launches of one step of kernel `name` (None: no model)"""
@@ -220,7 +312,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="star4", choices=["star4", "two_way"])
+    ap.add_argument("--workload", default="star4", choices=["star4", "two_way", "serve"])
+    ap.add_argument("--queries", type=int, default=256, help="serve: queries per run")
+    ap.add_argument("--slots", type=int, default=4, help="serve: contexts of the served path")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -242,6 +337,9 @@ def main():
               f"multi-GPU path in this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serve":
+        serve_main(args)
+        return
     codec = hjo.Codec()
     if args.workload == "star4":
         d, tables, plan, info, workload = star4_workload(codec, args.scale)

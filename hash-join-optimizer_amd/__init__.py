@@ -7,6 +7,7 @@ from . import abi  # noqa: F401
 from .abi import EngineMissing  # noqa: F401
 from .plan import (CacheError, Codec, Column, ColumnarTable, DataType, JoinNode,  # noqa: F401
                    Plan, PlanNode, ScanNode, Table, read_cache, sorted_rows, write_cache)
-from .engine import Context, DeviceResult, DeviceTable, Directory, EngineError  # noqa: F401
+from .engine import (Context, DeviceResult, DeviceTable, Directory, EngineError,  # noqa: F401
+                     Server)
 
 HASH_ALGO = "hip"  # the hash-table implementation this package provides (GPU unchained directory)

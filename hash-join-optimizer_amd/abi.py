@@ -111,6 +111,13 @@ _SIGS = {
     "hj_cache_close": (None, [C.c_void_p]),
     "hj_cache_write": (C.c_int, [P(hj_table), C.c_char_p]),
     "hj_table_upload_cache": (C.c_int, [C.c_void_p, C.c_char_p, P(C.c_void_p)]),
+    "hj_server_create": (C.c_int, [C.c_uint32, C.c_uint64, P(C.c_void_p)]),
+    "hj_server_destroy": (C.c_int, [C.c_void_p]),
+    "hj_server_submit": (C.c_int, [C.c_void_p, P(hj_plan), P(C.c_void_p), P(C.c_uint64)]),
+    "hj_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, P(hj_result), P(C.c_uint64),
+                                 P(C.c_uint64)]),
+    "hj_server_slots": (C.c_uint32, [C.c_void_p]),
+    "hj_server_last_error": (C.c_char_p, [C.c_void_p]),
     "hj_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "hj_profile_enable": (C.c_int, [C.c_void_p, C.c_char_p]),
     "hj_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64]),

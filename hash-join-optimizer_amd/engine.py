@@ -188,3 +188,57 @@ class Directory:
         if self.handle:
             self.ctx.lib.hj_directory_free(self.ctx.handle, self.handle)
             self.handle = None
+
+
+class Server:
+    """A pool of contexts serving queries over resident tables (hj_server_*): submit() queues a
+    query and returns a ticket, wait(ticket) returns (ColumnarTable, queued_ms, run_ms). Tables
+    are uploaded with any Context and shared by every query."""
+
+    def __init__(self, slots=4, arena_bytes=0):
+        self.lib = abi.load()
+        h = C.c_void_p()
+        rc = self.lib.hj_server_create(int(slots), int(arena_bytes), C.byref(h))
+        if rc:
+            raise EngineError(rc, self.lib.hj_global_error().decode())
+        self.handle = h
+        self._pending = {}
+
+    @property
+    def slots(self):
+        return int(self.lib.hj_server_slots(self.handle))
+
+    def submit(self, plan, dtables):
+        sig = (len(plan.nodes), plan.root)
+        cached = getattr(plan, "_c_device_plan", None)
+        if cached is None or cached[0] != sig:
+            cached = (sig,) + plan.to_c(with_inputs=False)
+            plan._c_device_plan = cached
+        _, p, keep = cached
+        arr = (C.c_void_p * max(1, len(dtables)))(*[t.handle for t in dtables])
+        t = C.c_uint64()
+        rc = self.lib.hj_server_submit(self.handle, C.byref(p), arr, C.byref(t))
+        if rc:
+            raise EngineError(rc, "query refused" if rc == -6 else "invalid query")
+        self._pending[t.value] = (p, keep, arr)  # alive until the query is done
+        return t.value
+
+    def wait(self, ticket):
+        res = abi.hj_result()
+        q, r = C.c_uint64(), C.c_uint64()
+        rc = self.lib.hj_server_wait(self.handle, ticket, C.byref(res), C.byref(q), C.byref(r))
+        self._pending.pop(ticket, None)
+        if rc:
+            raise EngineError(rc, self.lib.hj_server_last_error(self.handle).decode())
+        return result_from_c(res, self.lib.hj_result_free), q.value / 1e6, r.value / 1e6
+
+    def close(self):
+        if self.handle:
+            self.lib.hj_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

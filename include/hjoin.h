@@ -219,6 +219,27 @@ int hj_cache_write(const hj_table* table, const char* path);
  * double buffer, then as hj_table_upload. */
 int hj_table_upload_cache(hj_context* ctx, const char* path, hj_dtable** out);
 
+/* ---- serving (no counterpart in the reference, whose execute() runs one query at a time): a
+ * pool of `slots` contexts on the device, each with its own stream, arena (arena_bytes, 0 = 4 GiB)
+ * and host thread. Submitted queries wait in one FIFO queue; a free slot takes the next, so
+ * queries arriving while others run are admitted as slots free up and overlap on the GPU. A
+ * query whose reserved memory (its scans' staging and build tables, and its output at the widest
+ * scan's rows) exceeds a slot's arena is refused with HJ_E_BUSY. Tables are uploaded once with
+ * any context of the device (hj_table_upload) and shared read-only by every query. ---- */
+typedef struct hj_server hj_server;
+int hj_server_create(uint32_t slots, uint64_t arena_bytes, hj_server** out);
+int hj_server_destroy(hj_server* server);
+/* queue a query over resident tables (tables[i] stands for plan->inputs[i]); plan and tables
+ * must stay valid until hj_server_wait has returned for the ticket */
+int hj_server_submit(hj_server* server, const hj_plan* plan, const hj_dtable* const* tables,
+                     uint64_t* ticket);
+/* wait for a query: its result (free with hj_result_free) and its time in the queue and running
+ * (nanoseconds); a failed query returns its error code (message: hj_server_last_error) */
+int hj_server_wait(hj_server* server, uint64_t ticket, hj_result* result, uint64_t* queued_ns,
+                   uint64_t* run_ns);
+uint32_t hj_server_slots(const hj_server* server);
+const char* hj_server_last_error(hj_server* server);
+
 /* ---- engine options (no counterpart in the reference):
  *   "pipeline" 1 (default) / 0: run left-deep chains whose build sides are base scans as one
  *              fused probe pipeline; 0 forces the join-at-a-time path for every plan.
