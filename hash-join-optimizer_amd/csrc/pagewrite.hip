@@ -101,58 +101,107 @@ __device__ inline void int32_window(const PageCol& c, PlanState& st, uint64_t b,
     st.g += (int64_t)lane_u32(incl, 63);
 }
 
-// VARCHAR window of 64 rows: lane i holds row b + i. A short string adds 2 + len bytes, a NULL
-// row only its bit; a string longer than 8185 bytes closes the open page and takes pages of its
-// own.
+__device__ inline uint32_t pick8(const uint32_t (&v)[8], uint32_t k) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x = (uint32_t)j == k ? v[j] : x;
+    return x;
+}
+
+// VARCHAR window of 512 rows: lane i holds rows b + 8i .. b + 8i + 7 (their validity byte vbyte
+// and offsets o[0..8]). A short string adds 2 + len bytes, a NULL row only its bit; a string
+// longer than 8185 bytes closes the open page and takes pages of its own. Per lane, p[k] = bytes
+// of its rows 0..k; over the wave, the lanes' totals are scanned. A page that overflows in lane
+// L is cut at the first of L's rows whose cost exceeds 8192 (lanes 0..7 test one row each).
 __device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t b, uint32_t vbyte,
-                                      uint64_t o0, uint64_t o1) {
+                                      const uint64_t (&o)[9]) {
     const uint32_t lane = threadIdx.x;
     const uint64_t n = c.n;
-    const uint64_t r = b + lane;
-    const bool has = r < n;
-    const bool valid = has && ((vbyte >> (r & 7)) & 1);
-    const uint64_t len = valid ? o1 - o0 : 0;
-    const bool lng = valid && len > kLongStr;
-    const uint32_t a = (valid && !lng) ? 2u + (uint32_t)len : 0u;
-    const uint32_t A = wave_scan(a);
+    const uint64_t r0 = b + 8ull * lane;
+    uint32_t p[8], lens[8], lng = 0, acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const bool valid = r0 + k < n && ((vbyte >> k) & 1u);
+        const uint64_t len = valid ? o[k + 1] - o[k] : 0;
+        const bool lg = valid && len > kLongStr;
+        acc += (valid && !lg) ? 2u + (uint32_t)len : 0u;
+        p[k] = acc;
+        lens[k] = (uint32_t)len;
+        lng |= (lg ? 1u : 0u) << k;
+    }
+    const uint32_t ain = wave_scan(acc), aex = ain - acc;
+    const uint64_t wend = b + 512;
+    const uint64_t last = n - 1;  // the window holds row b < n
     for (;;) {
-        const uint64_t lm = __ballot(lng && r >= st.s);
-        const uint32_t lim = lm ? first_lane(lm) : 64u;
-        const bool cand = has && r >= st.s && lane < lim;
-        const int64_t cost = 5 + st.g + (int64_t)A + (int64_t)((r - st.s) >> 3);
+        // the first long string at or after s
+        const uint32_t ge = st.s <= r0 ? 0xffu : st.s >= r0 + 8 ? 0u : (0xffu << (uint32_t)(st.s - r0)) & 0xffu;
+        const uint32_t lm = lng & ge;
+        const uint64_t lb = __ballot(lm != 0);
+        uint64_t lim = wend;
+        uint32_t LL = 0, kl = 0;
+        if (lb) {
+            LL = first_lane(lb);
+            kl = (uint32_t)__builtin_ctz(lane_u32(lm, LL));
+            lim = b + 8ull * LL + kl;
+        }
+        // the open page's cost if it ended with this lane's last row before lim
+        uint64_t e = r0 + 7;
+        e = e < lim - 1 ? e : lim - 1;
+        e = e < last ? e : last;
+        const uint64_t lo = st.s > r0 ? st.s : r0;
+        const bool cand = r0 < lim && r0 <= last && e >= lo;
+        const int64_t cost =
+            5 + st.g + (int64_t)aex + (int64_t)pick8(p, (uint32_t)(e - r0)) + (int64_t)((e - st.s) >> 3);
         const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
         if (m) {
             const uint32_t L = first_lane(m);
-            const uint64_t xb = b + L;
+            const uint64_t rl = b + 8ull * L, eL = lane_u64(e, L);
+            const uint32_t aexL = lane_u32(aex, L);
+            uint32_t pj = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t v = lane_u32(p[j], L);
+                pj = lane == (uint32_t)j ? v : pj;
+            }
+            const uint64_t row = rl + lane;
+            const int64_t cj = 5 + st.g + (int64_t)aexL + (int64_t)pj + (int64_t)((row - st.s) >> 3);
+            const uint64_t m2 = __ballot(lane < 8 && row >= st.s && row <= eL && cj > (int64_t)kPageBytes);
+            const uint32_t t = first_lane(m2);
+            const uint64_t xb = rl + t;
             if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(xb - st.s), 0);
             ++st.np;
-            st.g = -(int64_t)(lane_u32(A, L) - lane_u32(a, L));
+            uint32_t before = 0;  // bytes of L's rows before row t
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t v = lane_u32(p[j], L);
+                before = (uint32_t)j + 1 == t ? v : before;
+            }
+            st.g = -(int64_t)(aexL + before);
             st.s = xb;
             continue;
         }
-        if (lim == 64) break;
-        const uint64_t rl = b + lim;
-        if (rl > st.s) {
-            if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(rl - st.s), 0);
+        if (!lb) break;
+        // a long string at row lim: close the open page, then one page per piece of 8188 bytes
+        if (lim > st.s) {
+            if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(lim - st.s), 0);
             ++st.np;
         }
-        const uint64_t ll = lane_u64(len, lim);
+        const uint64_t ll = lane_u32(pick8(lens, kl), LL);
         const uint64_t pieces = (ll + kLongPiece - 1) / kLongPiece;
         for (uint64_t p0 = 0; p0 < pieces; p0 += 64) {
-            const uint64_t p = p0 + lane;
-            if (p < pieces) put_ref(c, st.np + p, rl, (uint32_t)p, p == 0 ? 1u : 2u);
+            const uint64_t q = p0 + lane;
+            if (q < pieces) put_ref(c, st.np + q, lim, (uint32_t)q, q == 0 ? 1u : 2u);
         }
         st.np += pieces;
-        st.g = -(int64_t)lane_u32(A, lim);
-        st.s = rl + 1;
+        st.g = -(int64_t)(lane_u32(aex, LL) + lane_u32(pick8(p, kl), LL));
+        st.s = lim + 1;
     }
-    st.g += (int64_t)lane_u32(A, 63);
+    st.g += (int64_t)lane_u32(ain, 63);
 }
 
 // The cut itself is a short chain of wave operations per window, so the loads run a group of
 // windows ahead: the next group's data is requested before the current group is cut.
 constexpr int kIntGroup = 16;  // INT32 windows per group (32768 rows)
-constexpr int kStrGroup = 8;   // VARCHAR windows per group (512 rows)
 
 __device__ uint64_t plan_int32(const PageCol& c) {
     const uint32_t lane = threadIdx.x;
@@ -190,40 +239,24 @@ __device__ uint64_t plan_varchar(const PageCol& c) {
     const uint64_t n = c.n;
     const HJ_GLOBAL uint8_t* vb = gp(c.validity);
     const HJ_GLOBAL uint64_t* off = gp(c.offsets);
-    constexpr uint64_t kRows = 64ull * kStrGroup;
-    uint32_t cb[kStrGroup], nb[kStrGroup];
-    uint64_t c0[kStrGroup], c1[kStrGroup], n0[kStrGroup], n1[kStrGroup];
+    // window data one window ahead: validity byte and offsets of the lane's 8 rows
+    uint32_t nb = 0;
+    uint64_t no[9];
+    auto load = [&](uint64_t b) {
+        const uint64_t r0 = b + 8ull * lane;
+        nb = r0 < n ? vb[r0 >> 3] : 0u;
 #pragma unroll
-    for (int j = 0; j < kStrGroup; ++j) {
-        const uint64_t r = 64ull * j + lane;
-        nb[j] = 0;
-        n0[j] = n1[j] = 0;
-        if (r < n) {
-            nb[j] = vb[r >> 3];
-            n0[j] = off[r];
-            n1[j] = off[r + 1];
-        }
-    }
+        for (int k = 0; k < 9; ++k) no[k] = r0 + k <= n ? off[r0 + k] : 0ull;
+    };
+    load(0);
     PlanState st;
-    for (uint64_t B = 0; B < n; B += kRows) {
+    for (uint64_t b = 0; b < n; b += 512) {
+        const uint32_t cb = nb;
+        uint64_t co[9];
 #pragma unroll
-        for (int j = 0; j < kStrGroup; ++j) {
-            cb[j] = nb[j];
-            c0[j] = n0[j];
-            c1[j] = n1[j];
-        }
-#pragma unroll
-        for (int j = 0; j < kStrGroup; ++j) {
-            const uint64_t r = B + kRows + 64ull * j + lane;
-            if (r < n) {
-                nb[j] = vb[r >> 3];
-                n0[j] = off[r];
-                n1[j] = off[r + 1];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kStrGroup; ++j)
-            if (B + 64ull * j < n) varchar_window(c, st, B + 64ull * j, cb[j], c0[j], c1[j]);
+        for (int k = 0; k < 9; ++k) co[k] = no[k];
+        if (b + 512 < n) load(b + 512);
+        varchar_window(c, st, b, cb, co);
     }
     if (n > st.s) {
         if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(n - st.s), 0);

@@ -9,7 +9,7 @@
 //                 writer would. A page's byte count is monotone in the rows it holds, so a wave
 //                 finds the first row that does not fit among 64 candidates with one ballot:
 //                 INT32 columns step 2048 rows per iteration (one validity word per lane),
-//                 VARCHAR columns 64 rows (one row per lane); strings longer than 8185 bytes
+//                 VARCHAR columns 512 rows (8 per lane); strings longer than 8185 bytes
 //                 close the open page and take their own 0xffff / 0xfffe pages.
 //   k_page_write  one block per page assembles the page in LDS (zeroed first, so every byte
 //                 matches the host writer's zero-filled page) and stores it whole.
