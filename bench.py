@@ -93,6 +93,25 @@ def two_way_workload(codec, scale):
     return d, plan.inputs, plan, info, workload
 
 
+def m2m_workload(codec, scale):
+    """many-to-many skew stress: 50M build keys ~ Zipf(1.3) over [0, 5M) joined with 200M probe
+    keys ~ Zipf(1.3) under an independent key permutation (10 % NULL), INT32 payloads out"""
+    t = time.time()
+    d = dg.many_to_many(seed=44, n_build=int(50_000_000 * scale), n_probe=int(200_000_000 * scale))
+    plan = dg.two_way_plan(d, codec)
+    out_rows = dg.many_to_many_expected_count(d)
+    log(f"m2m generated+encoded in {time.time() - t:.1f}s, {out_rows} output rows")
+    nb, np_ = len(d["bk"]), len(d["pk"])
+    b_alg = 4 * nb + 4 * int(d["pv"].sum()) + (np_ + 7) // 8 + 2 * out_rows * 8
+    info = dict(builds=[nb], probes=[np_], outs=[out_rows], out_rows=out_rows, b_alg=b_alg,
+                fk_bytes=[4 * int(d["pv"].sum()) + (np_ + 7) // 8], out_slots=2)
+    workload = {"workload": "many-to-many INT32 join 50M x 200M, keys Zipf(1.3) over [0, 5M) with "
+                            "independent permutations, 10 % NULL probe keys",
+                "build_rows": nb, "probe_rows": np_, "output_rows": out_rows, "seed": 44,
+                "engine": "join-at-a-time (option pipeline = 0: the build keys repeat)"}
+    return d, plan.inputs, plan, info, workload
+
+
 def serve_workload(codec, args):
     """Serving: star joins (1-3 of the resident dimensions, 1-20 % range filters each) over a
     fact table drawn log-uniformly in [100K, 36M] rows (the nearest of the resident facts:
@@ -312,7 +331,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="star4", choices=["star4", "two_way", "serve"])
+    ap.add_argument("--workload", default="star4", choices=["star4", "two_way", "m2m", "serve"])
     ap.add_argument("--queries", type=int, default=256, help="serve: queries per run")
     ap.add_argument("--slots", type=int, default=4, help="serve: contexts of the served path")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
@@ -343,11 +362,15 @@ def main():
     codec = hjo.Codec()
     if args.workload == "star4":
         d, tables, plan, info, workload = star4_workload(codec, args.scale)
+    elif args.workload == "m2m":
+        d, tables, plan, info, workload = m2m_workload(codec, args.scale)
     else:
         d, tables, plan, info, workload = two_way_workload(codec, args.scale)
     tuples = sum(info["builds"]) + sum(info["probes"])
 
-    ctx = hjo.Context()
+    ctx = hjo.Context(arena_bytes=(64 << 30) if args.workload == "m2m" else 0)
+    if args.workload == "m2m":
+        ctx.set_option("pipeline", 0)
     t = time.time()
     dts = [ctx.upload(tb) for tb in tables]
     log(f"uploaded {len(dts)} tables in {time.time() - t:.1f}s")
@@ -396,14 +419,15 @@ def main():
                     "traffic": measured_traffic(dom, args.workload, args.scale),
                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_s * 1e3}
 
-    pages = result_pages(ctx, plan, dts)
+    # the many-to-many result is ~600M rows: its pages are not copied out here
+    pages = result_pages(ctx, plan, dts) if args.workload != "m2m" else None
     log(f"result pages: {pages}")
     pred = scan_predicate(ctx, dts, d) if args.workload == "star4" else None
     if pred:
         log(f"scan predicate: {pred}")
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload != "m2m":  # the CPU path takes minutes there
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(d, args.workload, tuples, threads)
 
