@@ -579,6 +579,43 @@ __device__ inline void emit(const OutSlots& o, const DirView& d, uint32_t prow, 
     }
 }
 
+// The light matches of a wave's 64 probe rows, written by the whole wave: match e of the wave
+// (in row order) goes to lane e mod 64, which finds its row by a binary search over the rows'
+// inclusive match counts in LDS. Output positions of one row are consecutive and rows follow
+// each other, so the writes are coalesced whatever the rows' match counts; heavy rows go to the
+// heavy list as before. Called by every thread of the block (it holds block barriers).
+struct WaveRows {
+    uint32_t incl[HJ_WAVE];
+    uint32_t lo[HJ_WAVE];
+    uint32_t prow[HJ_WAVE];
+    uint64_t off[HJ_WAVE];
+};
+
+__device__ inline void emit_wave(const OutSlots& o, const DirView& d, uint32_t prow, uint32_t lo,
+                                 uint32_t cnt, uint64_t off, HeavyEntry* heavy,
+                                 uint32_t* heavy_count, WaveRows* wr) {
+    const uint32_t lane = lane_id();
+    if (cnt > kLightMax) emit(o, d, prow, lo, cnt, off, heavy, heavy_count);
+    const uint32_t cl = cnt <= kLightMax ? cnt : 0u;
+    const uint32_t incl = wave_incl_scan(cl);
+    const uint32_t total = __shfl(incl, HJ_WAVE - 1, HJ_WAVE);
+    wr->incl[lane] = incl;
+    wr->lo[lane] = lo;
+    wr->prow[lane] = prow;
+    wr->off[lane] = off;
+    __syncthreads();
+    for (uint32_t e = lane; e < total; e += HJ_WAVE) {
+        uint32_t i = 0;  // rows whose matches all come before match e
+#pragma unroll
+        for (uint32_t step = HJ_WAVE / 2; step; step >>= 1)
+            if (wr->incl[i + step - 1] <= e) i += step;
+        const uint32_t before = i ? wr->incl[i - 1] : 0u;
+        const uint32_t j = e - before;
+        write_out(o, wr->off[i] + j, wr->prow[i], d.srow[wr->lo[i] + j]);
+    }
+    __syncthreads();
+}
+
 // =====================================================================================
 // Build-side key extraction: one block per key page (base scan) or per 1024 rows (relation).
 // Count pass writes per-block survivors; write pass compacts (h, row) with wave ballots.
@@ -724,6 +761,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, u
         if (threadIdx.x == 0) counts[p] = tot;
         return;
     }
+    __shared__ WaveRows wr[kBlock / HJ_WAVE];
     uint64_t run = offsets[p];
     for (uint32_t l0 = 0; l0 < nr; l0 += kBlock) {
         const uint32_t l = l0 + threadIdx.x;
@@ -731,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, u
         if (l < nr && scan_row_key(src, p, pg, hdr, l, r0 + l, &key)) cnt = dir_lookup(d, key, &lo);
         uint64_t tot;
         const uint64_t off = run + block_excl_op<OpSum>((uint64_t)cnt, lds, &tot);
-        if (cnt) emit(o, d, r0 + l, lo, cnt, off, heavy, heavy_count);
+        emit_wave(o, d, r0 + l, lo, cnt, off, heavy, heavy_count, &wr[threadIdx.x / HJ_WAVE]);
         run += tot;
     }
 }
@@ -753,6 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_rel(RelSrc src, DirView d, uin
         if (threadIdx.x == 0) counts[blockIdx.x] = tot;
         return;
     }
+    __shared__ WaveRows wr[kBlock / HJ_WAVE];
     uint64_t run = offsets[blockIdx.x];
     for (uint32_t l0 = 0; l0 < kRelTile; l0 += kBlock) {
         const uint64_t i = base + l0 + threadIdx.x;
@@ -760,7 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_rel(RelSrc src, DirView d, uin
         if (i < src.n && rel_row_key(src, i, &key)) cnt = dir_lookup(d, key, &lo);
         uint64_t tot;
         const uint64_t off = run + block_excl_op<OpSum>((uint64_t)cnt, lds, &tot);
-        if (cnt) emit(o, d, (uint32_t)i, lo, cnt, off, heavy, heavy_count);
+        emit_wave(o, d, (uint32_t)i, lo, cnt, off, heavy, heavy_count, &wr[threadIdx.x / HJ_WAVE]);
         run += tot;
     }
 }
