@@ -147,6 +147,7 @@ class Executor {
             }
         }
         Rel r = node(plan_->root);
+        if (direct_) return direct_;  // the root join wrote its output columns itself
         return materialize(r, plan_->nodes[plan_->root]);
     }
 
@@ -157,6 +158,7 @@ class Executor {
     hipStream_t s_;
     Arena& A_;
     int depth_ = 0;
+    hj_dresult* direct_ = nullptr;  // set when the root join wrote the result columns
 
     template <class T>
     T* scratch(uint64_t n) {
@@ -477,15 +479,52 @@ class Executor {
         }
         OutSlots o;
         std::memset(&o, 0, sizeof o);
-        o.n = (int32_t)used.size();
-        for (size_t i = 0; i < used.size(); ++i) {
-            const Rel& src = used[i].first == 0 ? L : R;
-            const bool from_build = (used[i].first == 0) == bl;
-            uint32_t* dst = result<uint32_t>(total);
-            o.dst[i] = dst;
-            o.src[i] = src.slots[used[i].second].rid;
-            o.side[i] = from_build ? 1 : 0;
-            out.slots.push_back({src.slots[used[i].second].table, dst});
+        // the root join writes INT32 output columns straight from the probe (no row ids, no
+        // gather pass) when each has a row-space mirror and its declared type matches
+        const hj_node& root = plan_->nodes[plan_->root];
+        bool direct = &nd == &root && nd.num_output_attrs <= (uint64_t)kMaxSlots &&
+                      nd.num_output_attrs > 0;
+        for (uint64_t i = 0; direct && i < nd.num_output_attrs; ++i) {
+            const AttrRef& a = out.attrs[i];
+            const Rel& src = used[a.slot].first == 0 ? L : R;
+            const DevColumn& dc = column(src.slots[used[a.slot].second].table, a.col);
+            direct = dc.type == HJ_INT32 && dc.flat && root.output_attrs[i].type == HJ_INT32;
+        }
+        std::unique_ptr<hj_dresult> dres;
+        if (direct) {
+            dres.reset(new hj_dresult());
+            dres->num_rows = total;
+            dres->live = true;
+            const uint64_t vbytes = ((total + 63) / 64) * 8;
+            o.values = 1;
+            o.n = (int32_t)nd.num_output_attrs;
+            for (uint64_t i = 0; i < nd.num_output_attrs; ++i) {
+                const AttrRef& a = out.attrs[i];
+                const Rel& src = used[a.slot].first == 0 ? L : R;
+                const DevColumn& dc = column(src.slots[used[a.slot].second].table, a.col);
+                hj_dresult::Col c;
+                c.type = HJ_INT32;
+                c.values = result<int32_t>(total);
+                c.validity = result<uint8_t>(vbytes);
+                o.dst[i] = (uint32_t*)c.values;
+                o.src[i] = src.slots[used[a.slot].second].rid;
+                o.side[i] = (used[a.slot].first == 0) == bl ? 1 : 0;
+                o.flat[i] = dc.flat;
+                o.vbits[i] = dc.vbits;
+                o.valid8[i] = dc.vbits ? scratch<uint8_t>(total + 8) : nullptr;
+                dres->cols.push_back(c);
+            }
+        } else {
+            o.n = (int32_t)used.size();
+            for (size_t i = 0; i < used.size(); ++i) {
+                const Rel& src = used[i].first == 0 ? L : R;
+                const bool from_build = (used[i].first == 0) == bl;
+                uint32_t* dst = result<uint32_t>(total);
+                o.dst[i] = dst;
+                o.src[i] = src.slots[used[i].second].rid;
+                o.side[i] = from_build ? 1 : 0;
+                out.slots.push_back({src.slots[used[i].second].table, dst});
+            }
         }
         out.n = total;
         HeavyEntry* heavy = scratch<HeavyEntry>(total / kLightMax + 1);
@@ -496,6 +535,16 @@ class Executor {
         else
             { Prof p_(ctx_, "probe_rel_write"); probe_rel_write(prs, dv, poffs, o, heavy, hcount, s_); }
         { Prof p_(ctx_, "expand_heavy"); expand_heavy(dv, o, heavy, hcount, s_); }
+        if (dres) {
+            for (int i = 0; i < o.n; ++i) {
+                Prof p_(ctx_, "pack_validity");
+                pack_valid8(o.valid8[i], total, dres->cols[i].validity, s_);
+            }
+            direct_ = dres.release();
+            out.slots.clear();
+            for (size_t i = 0; i < used.size(); ++i)
+                out.slots.push_back({(used[i].first == 0 ? L : R).slots[used[i].second].table, nullptr});
+        }
         // the scratch (build side, directory, counts) is reused by the next operator; the
         // stream orders the reuse after these kernels
         A_.reset(mark);

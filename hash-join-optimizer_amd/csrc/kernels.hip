@@ -4,6 +4,8 @@
 // give the bytes a launch must move (the roofline the kernel is held to).
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace hj {
 
 namespace {
@@ -556,7 +558,14 @@ __device__ inline void write_out(const OutSlots& o, uint64_t pos, uint32_t prow,
     for (int s = 0; s < o.n; ++s) {
         const uint32_t src_row = o.side[s] ? brow : prow;
         const uint32_t* m = o.src[s];
-        o.dst[s][pos] = m ? m[src_row] : src_row;
+        const uint32_t row = m ? m[src_row] : src_row;
+        if (!o.values) {
+            o.dst[s][pos] = row;
+            continue;
+        }
+        const bool ok = !o.vbits[s] || ((o.vbits[s][row >> 6] >> (row & 63u)) & 1ull);
+        o.dst[s][pos] = ok ? (uint32_t)o.flat[s][row] : 0u;
+        if (o.valid8[s]) o.valid8[s][pos] = ok ? 1 : 0;  // none for a column without NULL
     }
 }
 
@@ -840,6 +849,40 @@ void probe_rel_write(const RelSrc& src, const DirView& d, const uint64_t* offset
     hipLaunchKernelGGL(k_probe_rel<true>, dim3(rel_blocks(src.n)), dim3(kBlock), 0, s, src, d,
                        nullptr, offsets, o, heavy, heavy_count);
 }
+namespace {
+// thread t: rows 8t .. 8t + 7, read as one 8-byte word, written as bitmap byte t; without
+// validity bytes (a column without NULL) every row below n is set
+__global__ __launch_bounds__(kBlock) void k_pack_valid8(const uint8_t* v8, uint64_t n,
+                                                        uint8_t* bits) {
+    const uint64_t nbytes = (n + 63) / 64 * 8;
+    for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < nbytes;
+         t += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t r = 8 * t;
+        uint32_t byte = 0;
+        if (r < n) {
+            if (!v8) {
+                byte = 0xffu;
+            } else if (r + 8 <= n) {
+                const uint64_t w = *(const uint64_t*)(v8 + r);  // 8-byte aligned: r = 8t
+#pragma unroll
+                for (int k = 0; k < 8; ++k) byte |= (uint32_t)((w >> (8 * k)) & 0xffu ? 1u : 0u) << k;
+            } else {
+                for (uint64_t k = 0; r + k < n; ++k) byte |= (v8[r + k] ? 1u : 0u) << k;
+            }
+            if (n - r < 8) byte &= (1u << (n - r)) - 1u;
+        }
+        bits[t] = (uint8_t)byte;
+    }
+}
+}  // namespace
+
+void pack_valid8(const uint8_t* valid8, uint64_t n, uint8_t* bits, hipStream_t s) {
+    if (!n) return;
+    const uint64_t nbytes = (n + 63) / 64 * 8;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(8192, (nbytes + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_pack_valid8, dim3(grid), dim3(kBlock), 0, s, valid8, n, bits);
+}
+
 void expand_heavy(const DirView& d, const OutSlots& o, const HeavyEntry* heavy,
                   const uint32_t* heavy_count, hipStream_t s) {
     hipLaunchKernelGGL(k_expand_heavy, dim3(2048), dim3(kBlock), 0, s, d, o, heavy, heavy_count);

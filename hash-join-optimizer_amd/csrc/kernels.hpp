@@ -49,6 +49,13 @@ struct OutSlots {
     const uint32_t* src[kMaxSlots];
     int32_t side[kMaxSlots];
     int32_t n;
+    // value mode (a root join whose outputs are INT32 columns): entry c is output column c,
+    // dst[c] receives the value flat[c][row] (0 for NULL) and valid8[c] its validity byte, row
+    // being the build or probe row (side[c]) mapped through src[c]
+    int32_t values;
+    const int32_t* flat[kMaxSlots];
+    const uint64_t* vbits[kMaxSlots];
+    uint8_t* valid8[kMaxSlots];
 };
 
 struct HeavyEntry {
@@ -108,6 +115,8 @@ void probe_scan_write(const ScanSrc& src, const DirView& d, const uint64_t* offs
 void probe_rel_count(const RelSrc& src, const DirView& d, uint64_t* counts, hipStream_t s);
 void probe_rel_write(const RelSrc& src, const DirView& d, const uint64_t* offsets,
                      const OutSlots& o, HeavyEntry* heavy, uint32_t* heavy_count, hipStream_t s);
+// validity bitmap (whole 64-bit words) of n rows from their validity bytes (nullptr: all set)
+void pack_valid8(const uint8_t* valid8, uint64_t n, uint8_t* bits, hipStream_t s);
 void expand_heavy(const DirView& d, const OutSlots& o, const HeavyEntry* heavy,
                   const uint32_t* heavy_count, hipStream_t s);
 
