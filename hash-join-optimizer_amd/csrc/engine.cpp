@@ -560,6 +560,7 @@ class Executor {
         int slot;
         uint32_t col;
     };
+    std::vector<AttrSlot> pipe_keys_;  // the pipeline's probe keys (slot, column)
 
     bool chain(uint64_t idx, uint64_t& source, std::vector<uint64_t>& joins, int depth) {
         if (idx >= plan_->num_nodes || depth > kMaxStages + 1) return false;
@@ -589,6 +590,7 @@ class Executor {
     enum class Pipe { kNo, kRel, kResult, kRetry };
 
     Pipe try_pipeline(Rel& out, hj_dresult** dres, bool allow_emit) {
+        pipe_keys_.clear();
         uint64_t source = 0;
         std::vector<uint64_t> joins;
         if (!chain(plan_->root, source, joins, 0) || joins.empty() ||
@@ -628,6 +630,7 @@ class Executor {
             if (battr >= bat.size() || pattr >= cur.size())
                 throw Error(HJ_E_INVALID, "join attribute out of range");
             const AttrSlot bk = bat[battr], pk = cur[pattr];
+            pipe_keys_.push_back(pk);  // never NULL in the output: NULL keys do not match
             const DevColumn& bcol = column(slot_table[bk.slot], bk.col);
             const DevColumn& pcol = column(slot_table[pk.slot], pk.col);
             if (bcol.type != HJ_INT32 || pcol.type != HJ_INT32)
@@ -857,6 +860,10 @@ class Executor {
                 throw Error(HJ_E_TYPE, "unsupported output column type (only INT32 and VARCHAR)");
             EmitCol& ec = ed.c[i];
             ec.col = dc.view();
+            // an output column with no NULL row: a column without NULL, or a probe key
+            ec.nonull = dc.all_dense ? 1 : 0;
+            for (const AttrSlot& k : pipe_keys_)
+                if (k.slot == cur[i].slot && k.col == cur[i].col) ec.nonull = 1;
             ec.slot = ps ? 1 + hp.qpos[ps - 1] : 0;
             ec.type = dc.type;
             if (dc.type == HJ_VARCHAR) ec.sidx = nstr++;

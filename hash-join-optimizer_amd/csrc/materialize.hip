@@ -199,6 +199,13 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
     if (voided) return;
     const uint64_t off = s_sum[0];
     const uint32_t R = s_rb[G];
+    if (ec.nonull && blockIdx.x + 1 == gridDim.x) {
+        // the last group knows the row count: the whole bitmap of a column without NULL
+        const uint64_t n = off + R < d.cap_rows ? off + R : d.cap_rows;
+        HJ_GLOBAL uint64_t* bits = (HJ_GLOBAL uint64_t*)gp(ec.validity);
+        for (uint64_t w = t; w < (n + 63) / 64; w += kB)
+            bits[w] = n - 64 * w >= 64 ? ~0ull : (1ull << (n - 64 * w)) - 1ull;
+    }
     if (blockIdx.x + 1 == gridDim.x && t == 0) {
         if (blockIdx.y == 0) gp(d.out)[0] = off + R;
         if (str) {
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
                 int32_t v = 0;
                 const bool ok = get_int32(ec.col, rid[u], &v);
                 gp(ec.values)[off + f] = ok ? v : 0;
-                gp(ec.valid8)[off + f] = ok ? 1 : 0;
+                if (!ec.nonull) gp(ec.valid8)[off + f] = ok ? 1 : 0;
             }
         }
         return;
@@ -263,7 +270,7 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
         run += tot;
         if (!have) continue;
         offs[o + 1] = start + len;
-        gp(ec.valid8)[o] = ref != kNullRef ? 1 : 0;
+        if (!ec.nonull) gp(ec.valid8)[o] = ref != kNullRef ? 1 : 0;
         if (!len || start + len > ec.chars_cap) continue;
         HJ_GLOBAL char* dst = gp(ec.chars) + start;
         if (!(ref & kLongRef)) {
@@ -298,6 +305,7 @@ __global__ __launch_bounds__(kB) void k_emit_bits(EmitDesc d) {
 #pragma unroll
         for (int c = 0; c < kEmitCols; ++c) {
             if (c >= d.ncol) break;
+            if (d.c[c].nonull) continue;  // set whole by k_emit
             const uint64_t bits = __ballot(r < n && gp(d.c[c].valid8)[r < n ? r : 0] != 0);
             if (lane == 0) ((HJ_GLOBAL uint64_t*)gp(d.c[c].validity))[wd] = bits;
         }
@@ -309,7 +317,9 @@ __global__ __launch_bounds__(kB) void k_emit_bits(EmitDesc d) {
 void emit_rows(const EmitDesc& d, hipStream_t s) {
     if (!d.blocks || !d.ncol || !d.group || d.group > kMaxGroup) return;
     hipLaunchKernelGGL(k_emit, dim3((d.blocks + d.group - 1) / d.group, d.ncol), dim3(kB), 0, s, d);
-    hipLaunchKernelGGL(k_emit_bits, dim3(512), dim3(kB), 0, s, d);
+    bool bytes = false;
+    for (int c = 0; c < d.ncol; ++c) bytes = bytes || !d.c[c].nonull;
+    if (bytes) hipLaunchKernelGGL(k_emit_bits, dim3(512), dim3(kB), 0, s, d);
 }
 
 void mat_gather(const MatBatch& b, hipStream_t s) {

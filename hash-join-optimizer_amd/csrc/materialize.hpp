@@ -61,8 +61,10 @@ struct EmitCol {
     uint64_t* offsets;    // VARCHAR, cap_rows + 1
     char* chars;          // VARCHAR, chars_cap bytes
     uint64_t chars_cap;
-    uint8_t* valid8;      // scratch: validity byte per output row
+    uint8_t* valid8;      // scratch: validity byte per output row (unused when nonull)
     uint8_t* validity;    // bitmap, 64-bit words
+    int32_t nonull;       // no output row is NULL: the bitmap is set whole, no validity bytes
+    int32_t pad;
 };
 
 struct EmitDesc {
