@@ -168,9 +168,12 @@ def run_served(srv, dts, queries, rate=None, seed=0):
             pass
         tickets.append(srv.submit(q, dts))
     lat = []
+    nbytes = 0
     for tk in tickets:
-        _, q_ms, r_ms = srv.wait(tk)
+        b, q_ms, r_ms = srv.wait(tk, keep=False)  # the result's pages are released unread
         lat.append(q_ms + r_ms)
+        nbytes += b
+    run_served.result_bytes = nbytes
     return time.perf_counter() - t0, lat
 
 
@@ -190,6 +193,7 @@ def serve_main(args):
             run_served(srv, dts, queries[:8])  # warm-up
             wall, lat = run_served(srv, dts, queries)
             out[f"{name}_qps"] = len(queries) / wall
+            out["result_bytes"] = run_served.result_bytes
             out[f"{name}_batch_p50_ms"] = float(np.percentile(lat, 50))
             out[f"{name}_batch_p99_ms"] = float(np.percentile(lat, 99))
             wall, lat = run_served(srv, dts, queries, rate=args.rate, seed=1)

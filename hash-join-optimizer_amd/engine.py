@@ -223,13 +223,19 @@ class Server:
         self._pending[t.value] = (p, keep, arr)  # alive until the query is done
         return t.value
 
-    def wait(self, ticket):
+    def wait(self, ticket, keep=True):
+        """(ColumnarTable, queued_ms, run_ms); with keep=False the result is released at once
+        and its page bytes come back in place of the table"""
         res = abi.hj_result()
         q, r = C.c_uint64(), C.c_uint64()
         rc = self.lib.hj_server_wait(self.handle, ticket, C.byref(res), C.byref(q), C.byref(r))
         self._pending.pop(ticket, None)
         if rc:
             raise EngineError(rc, self.lib.hj_server_last_error(self.handle).decode())
+        if not keep:
+            nbytes = sum(res.columns[j].num_pages for j in range(res.num_columns)) * 8192
+            self.lib.hj_result_free(C.byref(res))
+            return nbytes, q.value / 1e6, r.value / 1e6
         return result_from_c(res, self.lib.hj_result_free), q.value / 1e6, r.value / 1e6
 
     def close(self):
