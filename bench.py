@@ -154,19 +154,103 @@ def serve_workload(codec, args):
     return facts + dim_tables, queries
 
 
+def batch_workload(codec, args):
+    """256 independent 2- and 3-way INT32 joins: build sides of unique keys, sizes log-uniform
+    in [1K, 4M] x scale; probe sides log-uniform in [10K, 40M] x scale with FKs uniform or Zipf
+    (s = 0.8 or 1.2) over the build keys, match rate ~ U[0.1, 1.0] (other FKs miss every build),
+    5 % NULL FKs; output the payloads of every side"""
+    rng = np.random.default_rng(48)
+    I32 = hjo.DataType.INT32
+    sc = args.scale
+    plans = []
+    t = time.time()
+    for q in range(args.queries):
+        ways = int(rng.integers(2, 4))
+        nb = [int(np.exp(rng.uniform(np.log(1_000 * sc + 1), np.log(4_000_000 * sc + 1))))
+              for _ in range(ways - 1)]
+        npr = int(np.exp(rng.uniform(np.log(10_000 * sc + 1), np.log(40_000_000 * sc + 1))))
+        plan = hjo.Plan()
+        fk_cols, fk_valid = [], []
+        for n in nb:
+            m = rng.uniform(0.1, 1.0)
+            dist = rng.choice(["uniform", "zipf0.8", "zipf1.2"])
+            if dist == "uniform":
+                fk = rng.integers(0, n, npr)
+            else:
+                fk, _ = dg.zipf_keys(rng, n, 0.8 if dist == "zipf0.8" else 1.2, npr)
+            miss = rng.random(npr) >= m
+            fk = np.where(miss, n + rng.integers(0, n + 1, npr), fk).astype(np.int32)
+            fk_cols.append(fk)
+            fk_valid.append((rng.random(npr) >= 0.05).astype(np.uint8))
+        probe_cols = [codec.encode_int32(fk, v) for fk, v in zip(fk_cols, fk_valid)]
+        probe_cols.append(codec.encode_int32(rng.integers(0, 2**31, npr).astype(np.int32)))
+        plan.inputs.append(hjo.ColumnarTable(npr, probe_cols))
+        for n in nb:
+            plan.inputs.append(hjo.ColumnarTable(n, [
+                codec.encode_int32(rng.permutation(n).astype(np.int32)),
+                codec.encode_int32(rng.integers(0, 2**31, n).astype(np.int32))]))
+        nk = len(nb)
+        cur = plan.new_scan_node(0, [(j, I32) for j in range(nk + 1)])
+        width = nk + 1
+        for j in range(nk):
+            b = plan.new_scan_node(1 + j, [(0, I32), (1, I32)])
+            # the probe side's attributes: the j build payloads prepended so far, the FKs, its
+            # payload; FK j is attribute 2j
+            cur = plan.new_join_node(True, b, cur, 0, 2 * j,
+                                     [(1, I32)] + [(2 + i, I32) for i in range(width)])
+            width += 1
+        plan.root = cur
+        plans.append(plan)
+    log(f"batch of {len(plans)} joins generated+encoded in {time.time() - t:.1f}s")
+    return plans
+
+
+def batch_main(args):
+    codec = hjo.Codec()
+    plans = batch_workload(codec, args)
+    ctx = hjo.Context()
+    dts = [[ctx.upload(t) for t in p.inputs] for p in plans]
+    out = {"metric": "batched queries/s", "unit": "queries/s", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic, seeded",
+           "config": {"workload": "batch of independent 2- and 3-way INT32 joins, builds "
+                                  "log-uniform [1K, 4M] x scale, probes log-uniform [10K, 40M] x "
+                                  "scale, uniform / Zipf(0.8, 1.2) FKs, match rate U[0.1, 1.0]",
+                      "queries": len(plans), "scale": args.scale, "slots": args.slots,
+                      "probe_rows": int(sum(p.inputs[0].num_rows for p in plans))}}
+    for name, slots in (("plain", 1), ("served", args.slots)):
+        with hjo.Server(slots=slots, arena_bytes=8 << 30) as srv:
+            run_served(srv, dts[:8], plans[:8])  # warm-up
+            wall, lat = run_served(srv, dts, plans)
+            out[f"{name}_qps"] = len(plans) / wall
+            out[f"{name}_p50_ms"] = float(np.percentile(lat, 50))
+            out[f"{name}_p99_ms"] = float(np.percentile(lat, 99))
+            out["result_bytes"] = run_served.result_bytes
+        log(f"{name} ({slots} slots): {out[name + '_qps']:.1f} joins/s, p50 "
+            f"{out[name + '_p50_ms']:.2f} ms p99 {out[name + '_p99_ms']:.2f} ms")
+    out["value"] = out["served_qps"]
+    out["speedup_vs_plain"] = out["served_qps"] / out["plain_qps"]
+    for d in dts:
+        for t_ in d:
+            t_.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def run_served(srv, dts, queries, rate=None, seed=0):
     """submit every query (at once, or at Poisson arrival times of `rate` per second), wait for
     all; returns (wall seconds, latencies in ms)"""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(queries)) if rate else np.zeros(len(queries))
+    per_query = bool(dts) and isinstance(dts[0], list)  # each query's own tables
     t0 = time.perf_counter()
     due = t0
     tickets = []
-    for q, g in zip(queries, gaps):
+    for i, (q, g) in enumerate(zip(queries, gaps)):
         due += g
         while rate and time.perf_counter() < due:
             pass
-        tickets.append(srv.submit(q, dts))
+        tickets.append(srv.submit(q, dts[i] if per_query else dts))
     lat = []
     nbytes = 0
     for tk in tickets:
@@ -337,7 +421,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="star4", choices=["star4", "two_way", "m2m", "serve"])
+    ap.add_argument("--workload", default="star4",
+                    choices=["star4", "two_way", "m2m", "serve", "batch"])
     ap.add_argument("--queries", type=int, default=256, help="serve: queries per run")
     ap.add_argument("--slots", type=int, default=4, help="serve: contexts of the served path")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
@@ -364,6 +449,9 @@ def main():
 
     if args.workload == "serve":
         serve_main(args)
+        return
+    if args.workload == "batch":
+        batch_main(args)
         return
     codec = hjo.Codec()
     if args.workload == "star4":
