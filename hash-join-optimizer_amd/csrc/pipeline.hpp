@@ -23,17 +23,18 @@ constexpr int kMaxOutStr = 16;                // output VARCHAR columns counted 
 // word index of slot s's row in a table
 __host__ __device__ inline uint32_t row_word(uint32_t s) { return 8u * (s >> 2) + 4u + (s & 3u); }
 
-// key slots for n build tuples: 2^bits >= 4n (load factor <= 1/4, at most one key per 4-key
-// bucket on average, so a lookup nearly always ends in its home bucket)
+// key slots for n build tuples: 2^bits >= 2n (load factor <= 1/2, at most two keys per 4-key
+// bucket on average; a full bucket continues in the next one, which shares its 64-byte line half
+// the time)
 __host__ __device__ inline uint32_t table_bits(uint32_t n) {
     uint32_t k = 0;
     while ((1ull << k) < (uint64_t)n) ++k;
-    return k + 2 < kMinTableBits ? kMinTableBits : k + 2;
+    return k + 1 < kMinTableBits ? kMinTableBits : k + 1;
 }
 
 // Linear probing runs over buckets within a segment of at most 2^kSegBits buckets (the next
 // bucket after a segment's last is its first), so that one workgroup can build a segment in LDS.
-constexpr uint32_t kSegBits = 10;
+constexpr uint32_t kSegBits = 9;
 __host__ __device__ inline uint32_t seg_mask(uint32_t k) {
     return (k - 2 < kSegBits ? (1u << (k - 2)) : (1u << kSegBits)) - 1u;
 }
