@@ -87,7 +87,7 @@ __device__ inline u32x4 load_bucket(const HJ_GLOBAL uint32_t* K, uint32_t b) {
 
 // slot of `key` (!= kEmptyKey), kDead when absent, given the keys v of its home bucket b;
 // a key leaves its home bucket only when that bucket is full, so a bucket with a free slot ends
-// the search, and at load factor <= 1/4 the home bucket nearly always does
+// the search, and at load factor <= 1/2 the home bucket nearly always does
 __device__ inline uint32_t bucket_resolve(const HJ_GLOBAL uint32_t* K, uint32_t smask, uint32_t b,
                                           uint32_t key, u32x4 v) {
     for (uint32_t m = 0; m <= smask; ++m) {  // at most the whole segment
