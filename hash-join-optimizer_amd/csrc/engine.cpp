@@ -612,7 +612,7 @@ class Executor {
         const size_t bb_bytes = sizeof(BuildBatch);
         const size_t pd_off = (bb_bytes + 255) & ~(size_t)255;
         const size_t meta_off = (pd_off + sizeof(PipeDesc) + 255) & ~(size_t)255;
-        const size_t desc_bytes = meta_off + (kMetaWords + 2 * kEmitOut) * 4;
+        const size_t desc_bytes = meta_off + (kMetaWords + 2 * kEmitOut + kCountWays * kMaxStages) * 4;
         if (desc_bytes > hj_context::kPinnedBytes - hj_context::kDescOffset)
             throw Error(HJ_E_INVALID, "pipeline descriptors exceed the staging buffer");
         uint8_t* ddesc = (uint8_t*)A_.scratch(desc_bytes);
@@ -698,7 +698,7 @@ class Executor {
                 jb.poff = scratch<uint16_t>((uint64_t)jb.nblk * (P + 1));
             }
         hb.sgrid = (uint32_t)ctx_->num_cus * 4;  // segment pass: a grid-stride loop
-        hb.jmeta = metas + 4;
+        hb.pcnt = metas + kMetaWords + 2 * kEmitOut;
 
         hp.nf = (int32_t)srel.filters.size();
         for (size_t f = 0; f < srel.filters.size(); ++f) {

@@ -249,7 +249,8 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
     __shared__ uint32_t lds[kNW];
     const uint32_t t = threadIdx.x, lane = lane_id();
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
-    const BuildJob& jb = bb->job[build_job_of(bb, blockIdx.x)];
+    const int job = build_job_of(bb, blockIdx.x);
+    const BuildJob& jb = bb->job[job];
     const uint32_t x = blockIdx.x - jb.blk0;
     const uint32_t rows = jb.rows, G = (rows + 63u) >> 6;
     const uint32_t r0 = x * kBuildRows;
@@ -381,7 +382,8 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
         }
     if (t == 0) {
         gp(jb.blk)[x] = hist[P];
-        atomicAdd((uint32_t*)gp(jb.meta), hist[P]);  // the job's tuple count
+        // the job's tuple count, in one of its kCountWays partial counters
+        if (hist[P]) atomicAdd((uint32_t*)&gp(bb->pcnt)[job * kCountWays + x % kCountWays], hist[P]);
     }
 }
 
@@ -401,9 +403,19 @@ __global__ __launch_bounds__(kB) void k_sb_segbuild(const BuildBatch* __restrict
     const uint32_t t = threadIdx.x;
     const int nj = bb->njobs;
     uint32_t first[kMaxStages + 1];  // first flat segment of each job
-    uint32_t nj_t[kMaxStages];       // tuples per job (all kMaxStages counters: independent loads)
+    uint32_t nj_t[kMaxStages];       // tuples per job (all partial counters: independent loads)
 #pragma unroll
-    for (int j = 0; j < kMaxStages; ++j) nj_t[j] = gp(bb->jmeta)[4 * j];
+    for (int j = 0; j < kMaxStages; ++j) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < (int)kCountWays; ++i) c += gp(bb->pcnt)[j * kCountWays + i];
+        nj_t[j] = c;
+    }
+    // the count for the kernels after this one
+    if (blockIdx.x == 0 && t == 0)
+#pragma unroll
+        for (int j = 0; j < kMaxStages; ++j)
+            if (j < nj) gp(bb->job[j].meta)[0] = nj_t[j];
     first[0] = 0;
 #pragma unroll
     for (int j = 0; j < kMaxStages; ++j)

@@ -52,6 +52,9 @@ constexpr uint32_t kMaxPartBlocks = 512;  // ... and 512 filter blocks per job
 // bits (kmax from the unfiltered rows). A segment of a table with more segments than partitions
 // reads its partition's tuples and keeps its own (at most 4 segments per partition).
 constexpr uint32_t kPartBits = 10;
+// Device-scope atomics on one address serialize across the XCDs: the filter pass adds its
+// blocks' tuple counts into kCountWays counters per job (block x into x % kCountWays)
+constexpr uint32_t kCountWays = 8;
 
 // rows per block of the build's filter and insert passes
 constexpr uint32_t kBuildRows = 8192;
@@ -88,7 +91,8 @@ struct BuildBatch {
     int32_t part;               // partitioned build (filter + segment passes) instead of
                                 // filter + clear + insert
     uint32_t sgrid;             // blocks of the segment pass (a grid-stride loop)
-    const uint32_t* jmeta;      // the counters of kMaxStages jobs, 4 words each (job[j].meta)
+    uint32_t* pcnt;             // partitioned build: tuples per job, kCountWays partial counts
+                                // each (zeroed), summed by the segment pass into job[j].meta[0]
     int32_t njobs;
     uint32_t grid;      // blocks of the filter and insert passes (every job's nblk)
     uint32_t cgrid;     // blocks per job of the clear pass (grid-stride)
