@@ -113,15 +113,19 @@ __device__ inline uint32_t pick8(const uint32_t (&v)[8], uint32_t k) {
 // longer than 8185 bytes closes the open page and takes pages of its own. Per lane, p[k] = bytes
 // of its rows 0..k; over the wave, the lanes' totals are scanned. A page that overflows in lane
 // L is cut at the first of L's rows whose cost exceeds 8192 (lanes 0..7 test one row each).
+// Row positions inside the window are 32-bit and relative to b (the open page may have started
+// before the window: a negative start, at least -65536 since a page holds at most 65535 rows),
+// and byte counts are 32-bit (a window holds at most 512 * 8187 bytes of short strings).
 __device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t b, uint32_t vbyte,
                                       const uint64_t (&o)[9]) {
     const uint32_t lane = threadIdx.x;
-    const uint64_t n = c.n;
-    const uint64_t r0 = b + 8ull * lane;
+    const uint64_t rem = c.n - b;
+    const int32_t nw = rem < 512 ? (int32_t)rem : 512;  // rows of the window (>= 1)
+    const int32_t r0 = 8 * (int32_t)lane;
     uint32_t p[8], lens[8], lng = 0, acc = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const bool valid = r0 + k < n && ((vbyte >> k) & 1u);
+        const bool valid = r0 + k < nw && ((vbyte >> k) & 1u);
         const uint64_t len = valid ? o[k + 1] - o[k] : 0;
         const bool lg = valid && len > kLongStr;
         acc += (valid && !lg) ? 2u + (uint32_t)len : 0u;
@@ -130,73 +134,70 @@ __device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t 
         lng |= (lg ? 1u : 0u) << k;
     }
     const uint32_t ain = wave_scan(acc), aex = ain - acc;
-    const uint64_t wend = b + 512;
-    const uint64_t last = n - 1;  // the window holds row b < n
+    int32_t s = (int32_t)((int64_t)st.s - (int64_t)b);  // open page start
+    int32_t g = (int32_t)st.g;
     for (;;) {
         // the first long string at or after s
-        const uint32_t ge = st.s <= r0 ? 0xffu : st.s >= r0 + 8 ? 0u : (0xffu << (uint32_t)(st.s - r0)) & 0xffu;
+        const uint32_t ge = s <= r0 ? 0xffu : s >= r0 + 8 ? 0u : (0xffu << (uint32_t)(s - r0)) & 0xffu;
         const uint32_t lm = lng & ge;
         const uint64_t lb = __ballot(lm != 0);
-        uint64_t lim = wend;
+        int32_t lim = 512;
         uint32_t LL = 0, kl = 0;
         if (lb) {
             LL = first_lane(lb);
             kl = (uint32_t)__builtin_ctz(lane_u32(lm, LL));
-            lim = b + 8ull * LL + kl;
+            lim = 8 * (int32_t)LL + (int32_t)kl;
         }
         // the open page's cost if it ended with this lane's last row before lim
-        uint64_t e = r0 + 7;
+        int32_t e = r0 + 7;
         e = e < lim - 1 ? e : lim - 1;
-        e = e < last ? e : last;
-        const uint64_t lo = st.s > r0 ? st.s : r0;
-        const bool cand = r0 < lim && r0 <= last && e >= lo;
-        const int64_t cost =
-            5 + st.g + (int64_t)aex + (int64_t)pick8(p, (uint32_t)(e - r0)) + (int64_t)((e - st.s) >> 3);
-        const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
+        e = e < nw - 1 ? e : nw - 1;
+        const int32_t lo = s > r0 ? s : r0;
+        const bool cand = r0 < lim && r0 < nw && e >= lo;
+        const int32_t cost = 5 + g + (int32_t)aex + (int32_t)pick8(p, (uint32_t)(e - r0) & 7u) + ((e - s) >> 3);
+        const uint64_t m = __ballot(cand && cost > (int32_t)kPageBytes);
         if (m) {
             const uint32_t L = first_lane(m);
-            const uint64_t rl = b + 8ull * L, eL = lane_u64(e, L);
-            const uint32_t aexL = lane_u32(aex, L);
+            const int32_t rl = 8 * (int32_t)L, eL = (int32_t)lane_u32((uint32_t)e, L);
+            const int32_t aexL = (int32_t)lane_u32(aex, L);
             uint32_t pj = 0;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint32_t v = lane_u32(p[j], L);
                 pj = lane == (uint32_t)j ? v : pj;
             }
-            const uint64_t row = rl + lane;
-            const int64_t cj = 5 + st.g + (int64_t)aexL + (int64_t)pj + (int64_t)((row - st.s) >> 3);
-            const uint64_t m2 = __ballot(lane < 8 && row >= st.s && row <= eL && cj > (int64_t)kPageBytes);
+            const int32_t row = rl + (int32_t)lane;
+            const int32_t cj = 5 + g + aexL + (int32_t)pj + ((row - s) >> 3);
+            const uint64_t m2 = __ballot(lane < 8 && row >= s && row <= eL && cj > (int32_t)kPageBytes);
             const uint32_t t = first_lane(m2);
-            const uint64_t xb = rl + t;
-            if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(xb - st.s), 0);
+            const int32_t xb = rl + (int32_t)t;
+            if (lane == 0) put_ref(c, st.np, (uint64_t)((int64_t)b + s), (uint32_t)(xb - s), 0);
             ++st.np;
-            uint32_t before = 0;  // bytes of L's rows before row t
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t v = lane_u32(p[j], L);
-                before = (uint32_t)j + 1 == t ? v : before;
-            }
-            st.g = -(int64_t)(aexL + before);
-            st.s = xb;
+            // bytes of L's rows before row xb: p[t - 1] of lane L, which lane t - 1 holds in pj
+            const uint32_t before = t ? lane_u32(pj, t - 1) : 0u;
+            g = -(aexL + (int32_t)before);
+            s = xb;
             continue;
         }
         if (!lb) break;
         // a long string at row lim: close the open page, then one page per piece of 8188 bytes
-        if (lim > st.s) {
-            if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(lim - st.s), 0);
+        if (lim > s) {
+            if (lane == 0) put_ref(c, st.np, (uint64_t)((int64_t)b + s), (uint32_t)(lim - s), 0);
             ++st.np;
         }
         const uint64_t ll = lane_u32(pick8(lens, kl), LL);
         const uint64_t pieces = (ll + kLongPiece - 1) / kLongPiece;
+        const uint64_t rowl = b + (uint64_t)lim;
         for (uint64_t p0 = 0; p0 < pieces; p0 += 64) {
             const uint64_t q = p0 + lane;
-            if (q < pieces) put_ref(c, st.np + q, lim, (uint32_t)q, q == 0 ? 1u : 2u);
+            if (q < pieces) put_ref(c, st.np + q, rowl, (uint32_t)q, q == 0 ? 1u : 2u);
         }
         st.np += pieces;
-        st.g = -(int64_t)(lane_u32(aex, LL) + lane_u32(pick8(p, kl), LL));
-        st.s = lim + 1;
+        g = -(int32_t)(lane_u32(aex, LL) + lane_u32(pick8(p, kl), LL));
+        s = lim + 1;
     }
-    st.g += (int64_t)lane_u32(ain, 63);
+    st.s = (uint64_t)((int64_t)b + s);
+    st.g = (int64_t)g + (int64_t)lane_u32(ain, 63);
 }
 
 // The cut itself is a short chain of wave operations per window, so the loads run a group of
