@@ -3,7 +3,7 @@
 // Build, all build sides in the same three launches:
 //   filter  rows passing the scan's filters with a non-NULL key -> staged (key, row) at the
 //           start of the block's row range                          reads key + filter columns
-//   clear   the job's tuple count, then its 2^k key slots (2^k >= 4n) and Bloom words
+//   clear   the job's tuple count, then its 2^k key slots (2^k >= 2n) and Bloom words
 //   insert  a 32-bit compare-and-swap of the key into a slot of its home bucket (then the other
 //           slots, then the next buckets), the row next to it; a key found twice flags the
 //           build (the caller then falls back to the general join path)
