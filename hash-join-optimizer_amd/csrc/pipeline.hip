@@ -881,7 +881,7 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
             return;
         }
         // joins in order, each on the lanes the previous ones kept
-        uint32_t brow[NS][U];  // key slot of the match, until the output
+        uint32_t brow[NS][U];  // key slot of the match, then its build row
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
             uint64_t any = 0;
@@ -899,6 +899,10 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
                     brow[q][u] = sl;
                     live[u] &= __ballot(sl != kDead);
                 }
+                // the build rows of the matches, read while their buckets are still cached
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (in_mask(live[u])) brow[q][u] = tkey[q][row_word(brow[q][u])];
             }
         }
         uint64_t any = 0;
@@ -912,7 +916,7 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
             if (in_mask(live[u])) {
                 uint32_t br[NS];
 #pragma unroll
-                for (int q = 0; q < NS; ++q) br[q] = tkey[q][row_word(brow[q][u])];
+                for (int q = 0; q < NS; ++q) br[q] = brow[q][u];
                 gp(pd.stage[0])[st0 + pos[u]] = r[u];
 #pragma unroll
                 for (int q = 0; q < NS; ++q) gp(pd.stage[1 + q])[st0 + pos[u]] = br[q];
