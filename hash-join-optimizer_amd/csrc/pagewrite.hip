@@ -62,43 +62,47 @@ struct PlanState {
     uint64_t np = 0;
 };
 
-// INT32 window of 2048 rows: lane i holds validity word i (rows b + 32i .. b + 32i + 31)
+// INT32 window of 2048 rows: lane i holds validity word i (rows b + 32i .. b + 32i + 31).
+// Row positions are 32-bit and relative to b (the open page starts at most 65535 rows before
+// the window), valid-row counts 32-bit.
 __device__ inline void int32_window(const PageCol& c, PlanState& st, uint64_t b, uint32_t vword) {
     const uint32_t lane = threadIdx.x;
-    const uint64_t n = c.n;
-    const uint64_t r0 = b + 32ull * lane;
+    const uint64_t rem = c.n - b;
+    const int32_t nw = rem < 2048 ? (int32_t)rem : 2048;  // rows of the window (>= 1)
+    const int32_t r0 = 32 * (int32_t)lane;
     uint32_t word = 0, nrow = 0;
-    if (r0 < n) {
-        nrow = (uint32_t)(n - r0 < 32 ? n - r0 : 32);
+    if (r0 < nw) {
+        nrow = (uint32_t)(nw - r0 < 32 ? nw - r0 : 32);
         word = nrow < 32 ? vword & ((1u << nrow) - 1u) : vword;
     }
     const uint32_t cnt = (uint32_t)__popc(word);
     const uint32_t incl = wave_scan(cnt);
     const uint32_t excl = incl - cnt;
-    const uint64_t e = r0 + nrow;
+    const int32_t e = r0 + (int32_t)nrow;
+    int32_t s = (int32_t)((int64_t)st.s - (int64_t)b);  // open page start
+    int32_t g = (int32_t)st.g;
     for (;;) {
         // cost of the open page if it ended with this lane's last row
-        const bool cand = nrow && e > st.s;
-        const int64_t cost = 4 * (st.g + (int64_t)incl) + (int64_t)((e - 1 - st.s) >> 3) + 5;
-        const uint64_t m = __ballot(cand && cost > (int64_t)kPageBytes);
+        const bool cand = nrow && e > s;
+        const int32_t cost = 4 * (g + (int32_t)incl) + ((e - 1 - s) >> 3) + 5;
+        const uint64_t m = __ballot(cand && cost > (int32_t)kPageBytes);
         if (!m) break;
         // the first row that does not fit lies in word L: lanes 0..31 test one row each
         const uint32_t L = first_lane(m);
         const uint32_t wl = lane_u32(word, L), xl = lane_u32(excl, L);
-        const uint64_t rl = b + 32ull * L;
-        const uint64_t x = rl + lane;
+        const int32_t x = 32 * (int32_t)L + (int32_t)lane;
         const uint32_t upto = lane < 31 ? (2u << lane) - 1u : 0xffffffffu;
-        const int64_t cx =
-            4 * (st.g + (int64_t)(xl + __popc(wl & upto))) + (int64_t)((x - st.s) >> 3) + 5;
-        const uint64_t m2 = __ballot(lane < 32 && x >= st.s && x < n && cx > (int64_t)kPageBytes);
+        const int32_t cx = 4 * (g + (int32_t)(xl + __popc(wl & upto))) + ((x - s) >> 3) + 5;
+        const uint64_t m2 = __ballot(lane < 32 && x >= s && x < nw && cx > (int32_t)kPageBytes);
         const uint32_t t = first_lane(m2);
-        const uint64_t xb = rl + t;
-        if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(xb - st.s), 0);
+        const int32_t xb = 32 * (int32_t)L + (int32_t)t;
+        if (lane == 0) put_ref(c, st.np, (uint64_t)((int64_t)b + s), (uint32_t)(xb - s), 0);
         ++st.np;
-        st.g = -(int64_t)(xl + __popc(wl & ((1u << t) - 1u)));
-        st.s = xb;
+        g = -(int32_t)(xl + __popc(wl & ((1u << t) - 1u)));
+        s = xb;
     }
-    st.g += (int64_t)lane_u32(incl, 63);
+    st.s = (uint64_t)((int64_t)b + s);
+    st.g = (int64_t)g + (int64_t)lane_u32(incl, 63);
 }
 
 __device__ inline uint32_t pick8(const uint32_t (&v)[8], uint32_t k) {
