@@ -44,8 +44,12 @@ constexpr uint64_t kCharsBound = 1ull << 30;
 constexpr uint32_t kMetaWords = 4 * (kMaxStages + 1);
 // totals of the direct output: rows, string bytes per column, overflow
 constexpr uint32_t kEmitOut = 2 + kEmitCols;
-// blocks of the fused pass per block of the direct output
+// blocks of the fused pass per block of the direct output. Each output block first sums the
+// counts of every pass block before its own, so fewer, larger groups suit a small output; when no
+// scan of the pipeline is filtered, nearly every source row of a key join finds its match and
+// the output is as large as the source, which wants more blocks.
 constexpr uint32_t kEmitGroup = 8;
+constexpr uint32_t kEmitGroupUnfiltered = 4;
 static_assert(kMaxOutStr >= kEmitCols, "every output column may be a string column");
 
 namespace {
@@ -603,6 +607,7 @@ class Executor {
 
         BuildBatch hb;
         PipeDesc hp;
+        bool any_filter = false;  // a build scan with filters or predicates
         std::memset(&hb, 0, sizeof hb);
         std::memset(&hp, 0, sizeof hp);
         hb.njobs = (int32_t)joins.size();
@@ -641,6 +646,7 @@ class Executor {
             BuildJob& jb = hb.job[i];
             jb.key = bcol.view();
             jb.nf = (int32_t)brel.filters.size();
+            if (!brel.filters.empty()) any_filter = true;
             for (size_t f = 0; f < brel.filters.size(); ++f) {
                 jb.fcol[f] = filter_view(brel, brel.filters[f].column);
                 jb.flo[f] = brel.filters[f].lo;
@@ -817,7 +823,7 @@ class Executor {
             Prof p_(ctx_, "probe_fused", true);
             pipe_probe_fused(hp, dpd, s_, p_.a, p_.b);
         }
-        if (emit) return emit_result(hp, oslots, slot_table, cur, metas, dres);
+        if (emit) return emit_result(hp, oslots, slot_table, cur, metas, any_filter, dres);
         { Prof p_(ctx_, "probe_tail"); pipe_probe_tail(hp, dpd, s_); }
         struct F4 {
             uint32_t v[4];
@@ -842,7 +848,7 @@ class Executor {
     // counters: fallback flag, rows, string totals, overflow
     Pipe emit_result(const PipeDesc& hp, const std::vector<int>& oslots,
                      const std::vector<uint32_t>& slot_table, const std::vector<AttrSlot>& cur,
-                     uint32_t* metas, hj_dresult** dres) {
+                     uint32_t* metas, bool any_filter, hj_dresult** dres) {
         const hj_node& root = plan_->nodes[plan_->root];
         const uint64_t cap = hp.src_rows ? hp.src_rows : 1;
         const uint64_t vbytes = ((cap + 63) / 64) * 8;
@@ -892,7 +898,7 @@ class Executor {
         for (int q = 0; q <= hp.nstream; ++q) ed.stage[q] = hp.stage[q];
         ed.meta = hp.meta;
         ed.cap_rows = cap;
-        ed.group = kEmitGroup;
+        ed.group = any_filter || hp.nf ? kEmitGroup : kEmitGroupUnfiltered;
         ed.blk_bytes = hp.blk_bytes;
         ed.bytes_stride = kMaxOutStr;
         ed.out = (uint64_t*)(metas + kMetaWords);
