@@ -10,4 +10,10 @@ from .plan import (CacheError, Codec, Column, ColumnarTable, DataType, JoinNode,
 from .engine import (Context, DeviceResult, DeviceTable, Directory, EngineError,  # noqa: F401
                      Server)
 
-HASH_ALGO = "hip"  # the hash-table implementation this package provides (GPU unchained directory)
+from .build import hash_algo as _hash_algo
+
+# the reference's CMake HASH_ALGO value this library is (CMakeLists.txt:173-187): "hip", the GPU
+# tables (bucketized open addressing for left-deep chains whose build keys are unique, the
+# unchained directory for every other join); the built library reports the same through
+# hj_hash_algo()
+HASH_ALGO = _hash_algo("hip")

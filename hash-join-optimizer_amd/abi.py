@@ -77,6 +77,7 @@ _SIGS = {
     "hj_context_destroy": (C.c_int, [C.c_void_p]),
     "hj_last_error": (C.c_char_p, [C.c_void_p]),
     "hj_global_error": (C.c_char_p, []),
+    "hj_hash_algo": (C.c_char_p, []),
     "hj_execute": (C.c_int, [C.c_void_p, P(hj_plan), P(hj_result)]),
     "hj_result_free": (None, [P(hj_result)]),
     "hj_table_upload": (C.c_int, [C.c_void_p, P(hj_table), P(C.c_void_p)]),

@@ -77,6 +77,13 @@ uint32_t log2_pow2_ceil(uint64_t n) {
     return k;
 }
 
+// field widths of the reference's StrRef (include/value_t.h:14-17): the reference's copy_scan
+// encodes every string of a scanned VARCHAR column as (table, column, page, index) and throws
+// std::out_of_range when a field does not fit (include/value_t.h:36-47)
+constexpr uint64_t kStrRefTableMask = (1ull << 10) - 1;
+constexpr uint64_t kStrRefColumnMask = (1ull << 10) - 1;
+constexpr uint64_t kStrRefPageMask = (1ull << 18) - 1;
+
 // ------------------------------------------------------------------ relations
 struct Slot {
     uint32_t table;
@@ -139,6 +146,8 @@ class Executor {
         if (!plan_->nodes || plan_->root >= plan_->num_nodes)
             throw Error(HJ_E_INVALID, "plan root out of range");
         depth_ = 0;
+        check_strrefs(plan_->root);
+        depth_ = 0;
         if (ctx_->use_pipeline) {
             for (int attempt = 0; attempt < 2; ++attempt) {
                 Rel pr;
@@ -182,6 +191,38 @@ class Executor {
         const hj_dtable* tb = table(t);
         if (c >= tb->cols.size()) throw Error(HJ_E_INVALID, "column index out of range");
         return tb->cols[c];
+    }
+
+    // The reference's scans encode each string of an output VARCHAR column as a StrRef and
+    // throw where a field overflows (src/execute.cpp:323-358, include/value_t.h:36-47); the
+    // engine keeps no StrRefs but reports the same error, with the same message, for the first
+    // scan in the reference's post-order (left subtree, right subtree) whose output overflows.
+    void check_strrefs(uint64_t idx) {
+        if (idx >= plan_->num_nodes || ++depth_ > 4096) return;  // reported by the plan walk
+        const hj_node& nd = plan_->nodes[idx];
+        if (nd.kind == HJ_NODE_JOIN) {
+            check_strrefs(nd.left);
+            check_strrefs(nd.right);
+        } else if (nd.kind == HJ_NODE_SCAN && nd.base_table_id < plan_->num_inputs &&
+                   tables_[nd.base_table_id]) {
+            const hj_dtable* tb = tables_[nd.base_table_id];
+            for (uint64_t i = 0; i < nd.num_output_attrs && nd.output_attrs; ++i) {
+                const uint64_t c = nd.output_attrs[i].index;
+                if (c >= tb->cols.size() || tb->cols[c].type != HJ_VARCHAR) continue;
+                const auto over = [](const char* field, uint64_t v, uint64_t mask) {
+                    return Error(HJ_E_RANGE, std::string("StrRef ") + field + " overflow: " +
+                                                 std::to_string(v) + " > " + std::to_string(mask));
+                };
+                const DevColumn& dc = tb->cols[c];
+                if (!dc.has_strings) continue;
+                if (nd.base_table_id > kStrRefTableMask)
+                    throw over("table", nd.base_table_id, kStrRefTableMask);
+                if (c > kStrRefColumnMask) throw over("column", c, kStrRefColumnMask);
+                if (dc.strref_page_over != ~0ull)
+                    throw over("page", dc.strref_page_over, kStrRefPageMask);
+            }
+        }
+        --depth_;
     }
 
     Rel node(uint64_t idx) {
@@ -778,7 +819,10 @@ class Executor {
             // fused pass: contiguous ranges of 64-row groups per block
             const int U = probe_unroll(hp.nstream);
             const uint64_t groups = (nsrc + 63) / 64;
-            const uint64_t maxb = probe_resident_blocks(hp.nstream, (uint32_t)ctx_->num_cus);
+            // at most the blocks resident at once, and at most what the direct output's
+            // prefix sums cover
+            const uint64_t maxb = std::min<uint64_t>(
+                probe_resident_blocks(hp.nstream, (uint32_t)ctx_->num_cus), kEmitMaxBlocks);
             const uint64_t per = (uint64_t)U * 4;  // groups per block and iteration
             uint64_t gpb = std::max<uint64_t>(per, (groups + maxb - 1) / maxb);
             gpb = (gpb + per - 1) / per * per;
@@ -902,7 +946,13 @@ class Executor {
         ed.blk_bytes = hp.blk_bytes;
         ed.bytes_stride = kMaxOutStr;
         ed.out = (uint64_t*)(metas + kMetaWords);
-        { Prof p_(ctx_, "emit"); emit_rows(ed, s_); }
+        {
+            Prof p_(ctx_, "emit");
+            if (!emit_rows(ed, s_))
+                throw Error(HJ_E_INVALID, "direct output: " + std::to_string(ed.blocks) +
+                                              " fused-pass blocks exceed " +
+                                              std::to_string(kEmitMaxBlocks));
+        }
         // one round trip: pipeline counters and the direct output's totals
         uint8_t* pin = (uint8_t*)ctx_->pinned;
         const size_t bytes = kMetaWords * 4 + kEmitOut * 8;
@@ -1118,6 +1168,11 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
         dc.h_row_start[p] = (uint32_t)rows;
         group_start[p] = (uint32_t)groups;
         uint32_t prow = nr;
+        if (hc.type == HJ_VARCHAR && p > kStrRefPageMask && dc.strref_page_over == ~0ull &&
+            (nr == 0xffffu || (nr != 0xfffeu && nn > 0)))
+            dc.strref_page_over = p;
+        if (hc.type == HJ_VARCHAR && (nr == 0xffffu || (nr != 0xfffeu && nn > 0)))
+            dc.has_strings = true;
         if (hc.type == HJ_VARCHAR && (nr == 0xffffu || nr == 0xfffeu)) {
             if (nr == 0xfffeu && (p == 0)) throw Error(HJ_E_INVALID, "0xfffe page without a long string");
             if (nn > HJ_PAGE_SIZE - 4) throw Error(HJ_E_INVALID, "corrupt long-string page");
@@ -1409,6 +1464,11 @@ const char* hj_global_error(void) {
     copy = g_err;
     return copy.c_str();
 }
+
+#if !defined(HJ_HASH_ALGO_HIP)
+#error "the engine is the HASH_ALGO=hip table; build it through build.py (-DHJ_HASH_ALGO_HIP)"
+#endif
+const char* hj_hash_algo(void) { return "hip"; }
 
 const char* hj_last_error(const hj_context* ctx) {
     return ctx ? ctx->last_error.c_str() : hj_global_error();

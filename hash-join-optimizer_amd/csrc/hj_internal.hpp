@@ -90,6 +90,10 @@ struct DevColumn {
     uint64_t* sref = nullptr;      // device, VARCHAR only: string reference per row
     uint32_t uniform_rows = 0;
     uint32_t all_dense = 1;
+    // VARCHAR: the first page past the reference's 18-bit StrRef page field that holds a string
+    // (a non-NULL row or a long string's 0xffff page), ~0 when there is none
+    uint64_t strref_page_over = ~0ull;
+    bool has_strings = false;  // VARCHAR: some row is non-NULL (the reference makes a StrRef)
     std::vector<uint32_t> h_row_start;
     DCol view() const {
         DCol d;

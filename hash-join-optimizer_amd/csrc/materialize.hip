@@ -132,6 +132,7 @@ __global__ __launch_bounds__(kB) void k_mat_c(MatBatch mb) {
 // taken in output order (pass block, then staged row) 256 at a time.
 constexpr uint32_t kMaxGroup = 16;
 constexpr int kPU = 8;  // blocks of the fused pass <= kPU * 256
+static_assert(kPU * kB == kEmitMaxBlocks, "k_emit's loads cover every block of the fused pass");
 
 // block-wide sums of K values per thread in one pass: the K wave reductions are interleaved
 // (one wait per shuffle step for all of them), then two barriers; the result is in out[k]
@@ -314,12 +315,14 @@ __global__ __launch_bounds__(kB) void k_emit_bits(EmitDesc d) {
 
 }  // namespace
 
-void emit_rows(const EmitDesc& d, hipStream_t s) {
-    if (!d.blocks || !d.ncol || !d.group || d.group > kMaxGroup) return;
+bool emit_rows(const EmitDesc& d, hipStream_t s) {
+    if (d.blocks > kEmitMaxBlocks || d.group > kMaxGroup) return false;
+    if (!d.blocks || !d.ncol || !d.group) return true;
     hipLaunchKernelGGL(k_emit, dim3((d.blocks + d.group - 1) / d.group, d.ncol), dim3(kB), 0, s, d);
     bool bytes = false;
     for (int c = 0; c < d.ncol; ++c) bytes = bytes || !d.c[c].nonull;
     if (bytes) hipLaunchKernelGGL(k_emit_bits, dim3(512), dim3(kB), 0, s, d);
+    return true;
 }
 
 void mat_gather(const MatBatch& b, hipStream_t s) {

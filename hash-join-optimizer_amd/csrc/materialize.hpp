@@ -51,6 +51,9 @@ struct MatBatch {
 // Output buffers hold cap_rows rows (an upper bound of the row count, known on the host), each
 // chars buffer chars_cap bytes; a larger string total sets the overflow word instead of writing.
 constexpr int kEmitCols = 16;
+// k_emit sums the counts of every fused-pass block before its group with a fixed number of
+// loads per thread: the pass may have at most this many blocks (emit_rows refuses more)
+constexpr uint32_t kEmitMaxBlocks = 2048;
 
 struct EmitCol {
     DCol col;
@@ -84,7 +87,9 @@ struct EmitDesc {
                                  // overflow of a chars buffer (zeroed)
 };
 
-void emit_rows(const EmitDesc& d, hipStream_t s);
+// false (nothing launched) when the pass has more than kEmitMaxBlocks blocks or a group is
+// larger than 16
+bool emit_rows(const EmitDesc& d, hipStream_t s);
 
 // passes a and b; pass c separately, once chars buffers exist
 void mat_gather(const MatBatch& b, hipStream_t s);

@@ -1107,9 +1107,10 @@ void launch_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s, hipEven
 
 template <int NS>
 uint32_t resident_blocks(uint32_t num_cus) {
-    static uint32_t per_cu = 0;
-    if (!per_cu) {
-        // the smaller occupancy of the two instantiations (with and without source filters)
+    // the smaller occupancy of the two instantiations (with and without source filters),
+    // queried once per process (a function-local static's initializer runs exactly once, also
+    // when several contexts' threads arrive together)
+    static const uint32_t per_cu = [] {
         int n0 = 0, n1 = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, k_sp_probe<NS, probe_unroll(NS), false>,
                                                          kB, 0) != hipSuccess)
@@ -1117,8 +1118,8 @@ uint32_t resident_blocks(uint32_t num_cus) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, k_sp_probe<NS, probe_unroll(NS), true>,
                                                          kB, 0) != hipSuccess)
             n1 = 1;
-        per_cu = (uint32_t)std::max(1, std::min(n0, n1));
-    }
+        return (uint32_t)std::max(1, std::min(n0, n1));
+    }();
     return per_cu * num_cus;
 }
 

@@ -173,10 +173,10 @@ int hj_server_wait(hj_server* s, uint64_t ticket, hj_result* result, uint64_t* q
     {
         std::unique_lock<std::mutex> lk(s->mu);
         auto it = s->jobs.find(ticket);
-        if (it == s->jobs.end()) return HJ_E_INVALID;
+        if (it == s->jobs.end()) return HJ_E_INVALID;  // unknown, or claimed by another waiter
         j = it->second;
+        s->jobs.erase(it);  // claimed: this waiter alone owns the job from here on
         s->cv_done.wait(lk, [j] { return j->done; });
-        s->jobs.erase(it);
     }
     *result = j->result;
     if (queued_ns) *queued_ns = j->start_ns - j->submit_ns;

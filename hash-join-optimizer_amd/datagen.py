@@ -14,6 +14,9 @@ Zipf(s) over a finite domain [0, N) is sampled by inverse CDF over ranks 1..N (w
 followed by a seeded Fisher-Yates rank->key permutation per column; NULLs are Bernoulli per row.
 Every generator returns plain numpy arrays; build_*_plan turns them into a Plan.
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from .plan import ColumnarTable, Codec, DataType, Plan
@@ -23,13 +26,25 @@ STR = DataType.VARCHAR
 
 
 def zipf_ranks(rng, n_domain, s, size, chunk=1 << 22):
+    """ranks 0..n_domain-1 with weights (rank+1)^-s by inverse CDF; the uniforms are drawn in
+    chunk order from rng, and the chunks' CDF searches (which release the GIL) run on a pool of
+    host threads, so the draw is the same for any thread count"""
     w = np.arange(1, n_domain + 1, dtype=np.float64) ** (-s)
     cdf = np.cumsum(w)
     cdf /= cdf[-1]
     out = np.empty(size, dtype=np.int64)
-    for i in range(0, size, chunk):
-        u = rng.random(min(chunk, size - i))
+
+    def search(i, u):
         out[i:i + len(u)] = np.searchsorted(cdf, u, side="right")
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+        pending = []
+        for i in range(0, size, chunk):
+            pending.append(pool.submit(search, i, rng.random(min(chunk, size - i))))
+            if len(pending) >= 32:  # bounded: at most 32 chunks of uniforms in memory
+                pending.pop(0).result()
+        for f in pending:
+            f.result()
     np.minimum(out, n_domain - 1, out=out)
     return out
 

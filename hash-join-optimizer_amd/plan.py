@@ -25,15 +25,18 @@ class DataType(IntEnum):
 class Column:
     """include/plan.h:106-161: a typed sequence of 8192-byte pages."""
 
-    def __init__(self, type, pages=None):
+    def __init__(self, type, pages=None, page_index=None):
         self.type = DataType(type)
         if pages is None:
             pages = np.zeros((0, PAGE_SIZE), dtype=np.uint8)
         self.pages = np.ascontiguousarray(pages, dtype=np.uint8).reshape(-1, PAGE_SIZE)
+        # Column::pages holds page pointers, and one page may stand at several positions:
+        # page i of the column is pages[page_index[i]] (None: pages in order)
+        self.page_index = None if page_index is None else np.asarray(page_index, dtype=np.int64)
 
     @property
     def num_pages(self):
-        return self.pages.shape[0]
+        return self.pages.shape[0] if self.page_index is None else len(self.page_index)
 
 
 class ColumnarTable:
@@ -160,8 +163,13 @@ class Plan:
 def column_to_c(col, keep):
     ptrs = (C.c_void_p * max(1, col.num_pages))()
     base = col.pages.ctypes.data
-    for i in range(col.num_pages):
-        ptrs[i] = base + i * PAGE_SIZE
+    if col.page_index is None:
+        for i in range(col.num_pages):
+            ptrs[i] = base + i * PAGE_SIZE
+    elif col.num_pages:
+        addr = (base + col.page_index * PAGE_SIZE).astype(np.uint64)
+        C.memmove(ptrs, addr.ctypes.data, 8 * col.num_pages)
+        keep.append(addr)
     keep.append(ptrs)
     keep.append(col.pages)
     c = abi.hj_column()

@@ -145,6 +145,11 @@ typedef struct hj_context hj_context;     /* HIP stream + HBM arena + resident t
 typedef struct hj_dtable hj_dtable;       /* a ColumnarTable resident in HBM */
 typedef struct hj_dresult hj_dresult;     /* a join result resident in HBM (flat columns) */
 
+/* ---- hash-table selection: the reference's CMake option HASH_ALGO (CMakeLists.txt:173-187,
+ * include/hash_config.h:3-22) picks its table at configure time. This library is built with
+ * HASH_ALGO=hip (every other value is refused by its build) and returns "hip". ---- */
+const char* hj_hash_algo(void);
+
 /* ---- context: replaces Contest::build_context / destroy_context (src/execute.cpp:610-614) ---- */
 /* arena_bytes = 0 picks the default (env HJ_ARENA_BYTES, else 16 GiB). */
 int hj_context_create(uint64_t arena_bytes, hj_context** out);
@@ -234,7 +239,9 @@ int hj_server_destroy(hj_server* server);
 int hj_server_submit(hj_server* server, const hj_plan* plan, const hj_dtable* const* tables,
                      uint64_t* ticket);
 /* wait for a query: its result (free with hj_result_free) and its time in the queue and running
- * (nanoseconds); a failed query returns its error code (message: hj_server_last_error) */
+ * (nanoseconds); a failed query returns its error code (message: hj_server_last_error). One
+ * waiter per ticket: the first call claims it, and any other call for the same ticket returns
+ * HJ_E_INVALID at once. */
 int hj_server_wait(hj_server* server, uint64_t ticket, hj_result* result, uint64_t* queued_ns,
                    uint64_t* run_ns);
 uint32_t hj_server_slots(const hj_server* server);

@@ -37,6 +37,13 @@
 
 #include "hjoin.h"
 
+// The adapter stands for the reference's execute.cpp under HASH_ALGO=hip; a translation unit
+// configured for one of the reference's CPU tables (include/hash_config.h) is a different build.
+#if defined(USE_ROBINHOOD_HASH) || defined(USE_HOPSCOTCH_HASH) || defined(USE_CUCKOO_HASH) || \
+    defined(USE_UNCHAINED_HASH) || defined(USE_STD_HASH)
+#error "hjoin_contest.hpp replaces execute.cpp only under HASH_ALGO=hip"
+#endif
+
 namespace hj_contest {
 
 namespace detail {

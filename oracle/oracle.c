@@ -118,6 +118,22 @@ int oracle_strref_encode(int is_long, uint64_t table, uint64_t col, uint64_t pag
     *out = r;
     return 0;
 }
+/* the message the reference's constructor throws for the first field that overflows
+ * (include/value_t.h:36-47: table, column, page, offset in that order) */
+static void strref_overflow(errstate* e, uint64_t table, uint64_t col, uint64_t page, uint64_t off) {
+    if (table > SR_TABLE_MASK)
+        err_set(e, HJ_E_RANGE, "StrRef table overflow: %llu > %llu", (unsigned long long)table,
+                (unsigned long long)SR_TABLE_MASK);
+    else if (col > SR_COLUMN_MASK)
+        err_set(e, HJ_E_RANGE, "StrRef column overflow: %llu > %llu", (unsigned long long)col,
+                (unsigned long long)SR_COLUMN_MASK);
+    else if (page > SR_PAGE_MASK)
+        err_set(e, HJ_E_RANGE, "StrRef page overflow: %llu > %llu", (unsigned long long)page,
+                (unsigned long long)SR_PAGE_MASK);
+    else
+        err_set(e, HJ_E_RANGE, "StrRef offset overflow: %llu > %llu", (unsigned long long)off,
+                (unsigned long long)SR_OFFSET_MASK);
+}
 static inline uint64_t sr_table(uint64_t r) { return (r >> SR_TABLE_SHIFT) & SR_TABLE_MASK; }
 static inline uint64_t sr_column(uint64_t r) { return (r >> SR_COLUMN_SHIFT) & SR_COLUMN_MASK; }
 static inline uint64_t sr_page(uint64_t r) { return (r >> SR_PAGE_SHIFT) & SR_PAGE_MASK; }
@@ -258,7 +274,7 @@ static void* scan_worker(void* arg) {
                 if (nr == 0xffff) {
                     uint64_t ref;
                     if (oracle_strref_encode(1, t->table_id, in_idx, p, 0, &ref)) {
-                        err_set(t->err, HJ_E_RANGE, "StrRef overflow");
+                        strref_overflow(t->err, t->table_id, in_idx, p, 0);
                         return NULL;
                     }
                     vcol_append(res, oracle_value_strref(ref));
@@ -270,10 +286,11 @@ static void* scan_worker(void* arg) {
                     for (uint32_t i = 0; i < nr; ++i) {
                         if (bit_get(bm, i)) {
                             uint64_t ref;
-                            if (oracle_strref_encode(0, t->table_id, in_idx, p, di++, &ref)) {
-                                err_set(t->err, HJ_E_RANGE, "StrRef overflow");
+                            if (oracle_strref_encode(0, t->table_id, in_idx, p, di, &ref)) {
+                                strref_overflow(t->err, t->table_id, in_idx, p, di);
                                 return NULL;
                             }
+                            ++di;
                             vcol_append(res, oracle_value_strref(ref));
                         } else {
                             vcol_append(res, 0);

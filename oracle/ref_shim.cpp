@@ -17,6 +17,8 @@
 #include "unchained.h"
 #include "value_t.h"
 
+#include <cstdio>
+
 extern "C" {
 
 uint64_t ref_splitmix64(uint64_t x) { return Contest::splitmix64(x); }
@@ -31,6 +33,19 @@ uint64_t ref_strref(int is_long, uint64_t table, uint64_t col, uint64_t page, ui
     } catch (const std::out_of_range&) {
         *err = -1;
         return 0;
+    }
+}
+
+// the message StrRef's constructor throws for these fields ("" when it does not throw)
+void ref_strref_message(int is_long, uint64_t table, uint64_t col, uint64_t page, uint64_t off,
+                        char* buf, uint64_t n) {
+    if (!n) return;
+    buf[0] = 0;
+    try {
+        StrRef s(is_long != 0, table, col, page, off);
+        (void)s;
+    } catch (const std::out_of_range& e) {
+        std::snprintf(buf, n, "%s", e.what());
     }
 }
 

@@ -125,6 +125,8 @@ def load_ref():
             ("ref_splitmix64", C.c_uint64, [C.c_uint64]),
             ("ref_strref", C.c_uint64, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                         P(C.c_int)]),
+            ("ref_strref_message", None, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64,
+                                          C.c_uint64, C.c_char_p, C.c_uint64]),
             ("ref_value_int32", C.c_uint64, [C.c_int32]),
             ("ref_value_strref", C.c_uint64, [C.c_uint64]),
             ("ref_value_tag", C.c_int, [C.c_uint64]),
@@ -237,3 +239,88 @@ def random_plan(rng, codec, ntables=None, max_rows=60, key_domain=8, null_p=0.15
         frontier.append((node, out))
     plan.root = frontier[0][0]
     return plan
+
+
+# ------------------------------------------------------------------ StrRef field limits
+STRREF_PAGE_LIMIT = 1 << 18  # pages 0 .. 2^18-1 fit the reference's 18-bit page field
+
+
+def strref_plan(codec, case):
+    """A root scan of one VARCHAR column placed where the reference's StrRef fields overflow
+    (include/value_t.h:14-17): case "table" scans input 1024, "column" outputs column 1024,
+    "page" / "page_null" a column of 2^18 + 1 pages (one shared all-NULL page, then a last page
+    holding the string "a" / only NULL), "none" the same column as "page" with 2^18 pages.
+    Returns (plan, the field args that overflow or None)."""
+    I, V = DataType.INT32, DataType.VARCHAR
+    one = Table([[1, "a"]], [I, V]).to_columnar(codec)
+    plan = Plan()
+    if case == "table":
+        plan.inputs = [one] * 1025
+        plan.new_scan_node(1024, [(1, V)])
+        bad = (0, 1024, 1, 0, 0)
+    elif case == "column":
+        cols = [codec.encode_int32([7])] * 1024 + [codec.encode_strings(["a"])]
+        plan.inputs = [ColumnarTable(1, cols)]
+        plan.new_scan_node(0, [(0, I), (1024, V)])
+        bad = (0, 0, 1024, 0, 0)
+    else:
+        npages = STRREF_PAGE_LIMIT + (0 if case == "none" else 1)
+        null_page = codec.encode_strings([None]).pages[0]
+        last = codec.encode_strings([None] if case == "page_null" else ["a"]).pages[0]
+        idx = np.zeros(npages, dtype=np.int64)
+        idx[-1] = 1
+        col = _plan.Column(V, np.stack([null_page, last]), page_index=idx)
+        plan.inputs = [ColumnarTable(npages, [col])]
+        plan.new_scan_node(0, [(0, V)])
+        bad = (0, 0, 0, npages - 1, 0) if case == "page" else None
+    plan.root = 0
+    return plan, bad
+
+
+# ------------------------------------------------------------------ numpy views of results
+def int_rows(codec, ct):
+    """an all-INT32 result as a sorted (rows x 2*cols) int64 array: (valid, value) per column,
+    rows in lexicographic order; equal arrays <=> equal sorted multisets of rows"""
+    parts = []
+    for col in ct.columns:
+        v, ok = codec.decode(col)
+        parts.append(ok.astype(np.int64))
+        parts.append(np.where(ok, v.astype(np.int64), 0))
+    a = np.stack(parts, axis=1) if parts else np.zeros((ct.num_rows, 0), np.int64)
+    if len(a):
+        a = a[np.lexsort(a.T[::-1])]
+    return a
+
+
+_hip = None
+
+
+def device_to_numpy(ptr, n, dtype):
+    """n elements at device address ptr, copied into a new host array (hipMemcpy, D2H): test
+    infrastructure for reading an engine result that stays in HBM"""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = C.c_int
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    out = np.empty(n, dtype=dtype)
+    if n:
+        rc = _hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
+        if rc:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out
+
+
+def dresult_int32(dres, c):
+    """column c of a device result (INT32): values and the validity bitmap as host arrays"""
+    lib = dres.ctx.lib
+    ty, vals, chars, valid = C.c_int32(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    rc = lib.hj_dresult_column(dres.handle, c, C.byref(ty), C.byref(vals), C.byref(chars),
+                               C.byref(valid))
+    if rc or ty.value != int(DataType.INT32):
+        raise RuntimeError("not an INT32 column")
+    n = dres.num_rows
+    v = device_to_numpy(vals.value, n, np.int32)
+    bits = device_to_numpy(valid.value, (n + 7) // 8, np.uint8)
+    ok = np.unpackbits(bits, bitorder="little")[:n].astype(bool)
+    return v, ok

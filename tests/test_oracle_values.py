@@ -5,6 +5,8 @@ import ctypes as C
 import numpy as np
 import pytest
 
+import hjtest
+
 
 def strref(oracle, *a):
     out = C.c_uint64()
@@ -86,3 +88,30 @@ def test_against_reference_splitmix(oracle, ref):
     rng = np.random.default_rng(9)
     for x in rng.integers(0, 2**63, size=1000, dtype=np.uint64).tolist() + [0, 1, 2**64 - 1]:
         assert oracle.lib.oracle_splitmix64(x) == ref.ref_splitmix64(x)
+
+
+def ref_message(ref, args):
+    buf = C.create_string_buffer(256)
+    ref.ref_strref_message(*args, buf, 256)
+    return buf.value.decode()
+
+
+@pytest.mark.parametrize("case", ["table", "column", "page"])
+def test_oracle_execute_strref_overflow_message(oracle, ref, case):
+    """the oracle's execute() fails where the reference's copy_scan throws, with the message the
+    reference's StrRef constructor throws (compiled from include/value_t.h)"""
+    plan, bad = hjtest.strref_plan(oracle.codec, case)
+    want = ref_message(ref, bad)
+    assert want.startswith("StrRef " + case + " overflow")
+    with pytest.raises(RuntimeError) as e:
+        oracle.execute(plan)
+    assert e.value.args[0].endswith(": " + want)
+
+
+@pytest.mark.parametrize("case", ["page_null", "none"])
+def test_oracle_execute_strref_within_limits(oracle, case):
+    """no StrRef is made for NULL rows, and page 2^18 - 1 still fits"""
+    plan, _ = hjtest.strref_plan(oracle.codec, case)
+    rows = oracle.rows(oracle.execute(plan))
+    assert len(rows) == hjtest.STRREF_PAGE_LIMIT + (1 if case == "page_null" else 0)
+    assert sum(1 for r in rows if r == ("a",)) == (1 if case == "none" else 0)
