@@ -300,7 +300,9 @@ def device_to_numpy(ptr, n, dtype):
     infrastructure for reading an engine result that stays in HBM"""
     global _hip
     if _hip is None:
-        _hip = C.CDLL("libamdhip64.so")
+        # the HIP runtime the engine library itself is linked against (dlsym on its handle
+        # searches its dependencies), not another copy found by name
+        _hip = abi.load()
         _hip.hipMemcpy.restype = C.c_int
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     out = np.empty(n, dtype=dtype)
