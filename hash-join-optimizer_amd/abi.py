@@ -18,6 +18,9 @@ HJ_E_RANGE = -5
 HJ_E_BUSY = -6
 HJ_E_NODEV = -7
 
+NOALLOC_SCRATCH_BYTES = 64 << 20  # include/hjoin.h HJ_NOALLOC_SCRATCH_BYTES
+NOALLOC_MAX_PROBE_KEYS = 3_300_000  # include/hjoin.h HJ_NOALLOC_MAX_PROBE_KEYS
+
 
 class hj_column(C.Structure):
     _fields_ = [("type", C.c_int32), ("num_pages", C.c_uint32),
@@ -102,6 +105,8 @@ _SIGS = {
     "hj_directory_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, P(C.c_void_p)]),
     "hj_directory_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                      C.c_void_p, C.c_void_p]),
+    "hj_directory_probe_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                             C.c_void_p, C.c_void_p]),
     "hj_directory_probe_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                            P(C.c_void_p), P(C.c_uint64)]),
     "hj_directory_info": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),

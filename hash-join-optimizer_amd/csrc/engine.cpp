@@ -1446,6 +1446,35 @@ void to_host(hj_context* ctx, const hj_dresult* r, hj_result* out) {
 }  // namespace hj
 
 // ====================================================================== directory object
+// ---------------------------------------------------------------- fixed device staging
+// hj_directory_probe_noalloc stages its probe keys, counts and first rows in this array instead
+// of allocating device memory: every device the library runs on holds its own copy until the
+// process ends. Host code never uses the array's name as an address (on the host it is only a
+// stand-in); the address each device gives for it is fetched once per device, with that device
+// current, when the first context is created there, and kept in g_scratch_ptr.
+constexpr uint64_t kNoAllocScratchBytes = HJ_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of `scratch` (the caller made the device current)
+void init_noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        throw hj::Error(HJ_E_NODEV, "device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw hj::Error(HJ_E_HIP,
+                        std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (uint8_t*)ptr;
+}
+}  // namespace
+
 struct hj_directory {
     uint64_t n = 0;
     uint32_t k = 0;
@@ -1498,6 +1527,7 @@ int hj_context_create(uint64_t arena_bytes, hj_context** out) {
         ctx->arena.init(ctx->arena_mem, bytes);
         HJ_HIP(hipHostMalloc(&ctx->pinned, hj_context::kPinnedBytes, hipHostMallocDefault));
         if (const char* e = std::getenv("HJ_PIPELINE")) ctx->use_pipeline = std::atoi(e) != 0;
+        init_noalloc_scratch(ctx->device);
     });
     if (rc) {
         if (ctx->arena_mem) (void)hipFree(ctx->arena_mem);
@@ -1832,6 +1862,37 @@ int hj_directory_probe(hj_context* ctx, const hj_directory* dir, const uint32_t*
         (void)hipFree(dk);
         (void)hipFree(dc);
         (void)hipFree(df);
+    });
+}
+
+int hj_directory_probe_noalloc(hj_context* ctx, const hj_directory* dir,
+                               const uint32_t* probe_keys, uint64_t m, uint64_t* counts,
+                               uint64_t* first_row) {
+    if (!ctx || !dir || (m && (!probe_keys || !counts))) return HJ_E_INVALID;
+    return guarded(ctx, [&] {
+        if (m > HJ_NOALLOC_MAX_PROBE_KEYS)
+            throw Error(HJ_E_INVALID, "hj_directory_probe_noalloc: " + std::to_string(m) +
+                                          " probe keys exceed the limit of " +
+                                          std::to_string((uint64_t)HJ_NOALLOC_MAX_PROBE_KEYS));
+        if (!m) return;
+        uint8_t* base = ctx->device >= 0 && ctx->device < kMaxDevices ? g_scratch_ptr[ctx->device]
+                                                                      : nullptr;
+        if (!base) throw Error(HJ_E_HIP, "no staging address for this device");
+        // [keys: 4 m B][counts: 8 m B][first rows: 8 m B], each from a 256-byte boundary; the
+        // kernel writes every count (and first row) it stages, so nothing needs clearing
+        const uint64_t okeys = 0, ocnt = Arena::align(4 * m), ofirst = ocnt + Arena::align(8 * m);
+        static_assert(HJ_NOALLOC_MAX_PROBE_KEYS * 20ull + 512 <= HJ_NOALLOC_SCRATCH_BYTES,
+                      "the staging holds the largest accepted probe");
+        uint32_t* dk = (uint32_t*)(base + okeys);
+        uint64_t* dc = (uint64_t*)(base + ocnt);
+        uint64_t* df = first_row ? (uint64_t*)(base + ofirst) : nullptr;
+        HJ_HIP(hipMemcpyAsync(dk, probe_keys, m * 4, hipMemcpyHostToDevice, ctx->stream));
+        dir_probe_keys(dir->view, dk, m, dc, df, ctx->stream);
+        HJ_HIP(hipGetLastError());
+        HJ_HIP(hipMemcpyAsync(counts, dc, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (first_row)
+            HJ_HIP(hipMemcpyAsync(first_row, df, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HJ_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
 

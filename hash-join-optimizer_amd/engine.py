@@ -166,6 +166,17 @@ class Directory:
             cnt.ctypes.data, first.ctypes.data))
         return cnt[:len(k)], first[:len(k)]
 
+    def probe_noalloc(self, keys):
+        """probe() through hj_directory_probe_noalloc: no device allocation, at most
+        abi.NOALLOC_MAX_PROBE_KEYS keys per call, calls on one device must not overlap"""
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        cnt = np.zeros(max(1, len(k)), dtype=np.uint64)
+        first = np.zeros(max(1, len(k)), dtype=np.uint64)
+        self.ctx._check(self.ctx.lib.hj_directory_probe_noalloc(
+            self.ctx.handle, self.handle, k.ctypes.data if len(k) else None, len(k),
+            cnt.ctypes.data, first.ctypes.data))
+        return cnt[:len(k)], first[:len(k)]
+
     def probe_pairs(self, keys):
         k = np.ascontiguousarray(keys, dtype=np.uint32)
         ptr = C.c_void_p()

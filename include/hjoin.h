@@ -274,6 +274,18 @@ int hj_directory_build(hj_context* ctx, const uint32_t* keys, uint64_t n, hj_dir
 int hj_directory_probe(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
                        uint64_t m, uint64_t* counts, uint64_t* first_row);
 /* all matches of probe key i as (probe index, build row id) pairs, sorted by probe index */
+/* No counterpart in the reference: hj_directory_probe for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops or harnesses that forbid hipMalloc). It
+ * allocates nothing: the probe keys, counts and first rows are staged in one fixed device buffer
+ * of HJ_NOALLOC_SCRATCH_BYTES that each device holds from its first context until the process
+ * ends. At most HJ_NOALLOC_MAX_PROBE_KEYS keys per call (more: HJ_E_INVALID, nothing run). Every
+ * call on a device shares that device's buffer: calls that overlap on one device race and must
+ * be serialized by the caller. Results equal hj_directory_probe's. */
+#define HJ_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define HJ_NOALLOC_MAX_PROBE_KEYS 3300000ull
+int hj_directory_probe_noalloc(hj_context* ctx, const hj_directory* dir,
+                               const uint32_t* probe_keys, uint64_t m, uint64_t* counts,
+                               uint64_t* first_row);
 int hj_directory_probe_pairs(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
                              uint64_t m, uint64_t** pairs, uint64_t* num_pairs);
 int hj_directory_info(const hj_directory* dir, uint64_t* num_slots, uint64_t* num_tuples,
