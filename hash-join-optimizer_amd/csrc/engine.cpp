@@ -148,6 +148,7 @@ class Executor {
         depth_ = 0;
         check_strrefs(plan_->root);
         depth_ = 0;
+        prepare();
         if (ctx_->use_pipeline) {
             for (int attempt = 0; attempt < 2; ++attempt) {
                 Rel pr;
@@ -155,7 +156,7 @@ class Executor {
                 const Pipe st = try_pipeline(pr, &dr, attempt == 0);
                 if (st == Pipe::kResult) return dr;
                 if (st == Pipe::kRel) return materialize(pr, plan_->nodes[plan_->root]);
-                A_.clear();
+                A_.clear_to(prep_mark_);  // the query's column index stays
                 if (st != Pipe::kRetry) break;  // not a chain, or a duplicate build key
             }
         }
@@ -172,6 +173,93 @@ class Executor {
     Arena& A_;
     int depth_ = 0;
     hj_dresult* direct_ = nullptr;  // set when the root join wrote the result columns
+    // the query's own row-space access to the columns its scans read (built by prepare(), in
+    // the arena below prep_mark_): a group index, or for an INT32 column without one a decoded
+    // row-space array
+    struct QCol {
+        uint64_t* gidx = nullptr;
+        int32_t* flat = nullptr;
+        uint64_t* vbits = nullptr;
+    };
+    std::map<uint64_t, QCol> qcols_;
+    size_t prep_mark_ = 0;
+
+    static uint64_t qkey(uint32_t t, uint32_t c) { return ((uint64_t)t << 32) | c; }
+
+    // the view of column c of input t that this query's kernels read
+    DCol qview(uint32_t t, uint32_t c) const {
+        DCol d = column(t, c).view();
+        auto it = qcols_.find(qkey(t, c));
+        if (it != qcols_.end()) {
+            d.gidx = it->second.gidx;
+            d.flat = it->second.flat;
+            d.vbits = it->second.vbits;
+        }
+        return d;
+    }
+
+    // Index (or, where a page holds fewer than 64 rows before the last one, decode) every INT32
+    // and VARCHAR column that a scan of the plan reads: outputs, filters and predicates. This is
+    // the query's first device work; the reference's copy_scan does the same job in its execute()
+    // (src/execute.cpp:265-369).
+    void prepare() {
+        qcols_.clear();
+        std::vector<std::pair<uint32_t, uint32_t>> want;
+        for (uint64_t i = 0; i < plan_->num_nodes; ++i) {
+            const hj_node& nd = plan_->nodes[i];
+            if (nd.kind != HJ_NODE_SCAN || nd.base_table_id >= plan_->num_inputs ||
+                !tables_[nd.base_table_id])
+                continue;
+            const uint32_t t = (uint32_t)nd.base_table_id;
+            const size_t nc = tables_[t]->cols.size();
+            auto add = [&](uint64_t c) {
+                if (c < nc) want.push_back({t, (uint32_t)c});
+            };
+            for (uint64_t a = 0; a < nd.num_output_attrs && nd.output_attrs; ++a)
+                add(nd.output_attrs[a].index);
+            for (uint64_t f = 0; f < nd.num_filters && nd.filters; ++f) add(nd.filters[f].column);
+            for (uint64_t e = 0; e < nd.num_preds && nd.preds; ++e)
+                if (nd.preds[e].op < HJ_P_AND) add(nd.preds[e].column);
+        }
+        GroupIndexBatch gb;
+        std::memset(&gb, 0, sizeof gb);
+        auto flush = [&] {
+            if (!gb.n) return;
+            { Prof p_(ctx_, "group_index"); launch_group_index(gb, s_); }
+            std::memset(&gb, 0, sizeof gb);
+        };
+        for (auto& tc : want) {
+            const uint64_t key = qkey(tc.first, tc.second);
+            if (qcols_.count(key)) continue;
+            const DevColumn& dc = column(tc.first, tc.second);
+            if (dc.type != HJ_INT32 && dc.type != HJ_VARCHAR) continue;
+            QCol q;
+            const uint64_t G = (dc.num_rows + 63) / 64;
+            if (dc.view().umagic) continue;  // dense, uniform pages: addressed by arithmetic
+            if (dc.regular) {
+                q.gidx = scratch<uint64_t>(2 * G + 2);
+                if (G) {
+                    gb.col[gb.n] = dc.view();
+                    gb.out[gb.n] = q.gidx;
+                    gb.blk0[gb.n + 1] = gb.blk0[gb.n] + (uint32_t)((G + 255) / 256);
+                    if (++gb.n == kIndexCols) flush();
+                }
+            } else if (dc.type == HJ_INT32) {
+                q.flat = scratch<int32_t>(dc.num_rows + 1);
+                if (!dc.all_dense) {
+                    q.vbits = scratch<uint64_t>(G + 1);
+                    HJ_HIP(hipMemsetAsync(q.vbits, 0, (G + 1) * 8, s_));
+                }
+                Prof p_(ctx_, "decode_int32");
+                launch_decode_int32(dc.view(), q.flat, q.vbits, s_);
+            } else {
+                continue;  // a VARCHAR column with long strings: rows found by page search
+            }
+            qcols_[key] = q;
+        }
+        flush();
+        prep_mark_ = A_.mark();
+    }
 
     template <class T>
     T* scratch(uint64_t n) {
@@ -337,7 +425,7 @@ class Executor {
         if (depth != 1) throw Error(HJ_E_INVALID, "a scan predicate must leave exactly one result");
         pp.nops = (int32_t)nd.num_preds;
         pp.ncol = (int32_t)used.size();
-        for (size_t i = 0; i < used.size(); ++i) pp.col[i] = tb->cols[used[i]].view();
+        for (size_t i = 0; i < used.size(); ++i) pp.col[i] = qview(r.table, used[i]);
         uint8_t* dlit = result<uint8_t>(lit.size() + 4);
         if (!lit.empty()) {
             HJ_HIP(hipMemcpyAsync(dlit, lit.data(), lit.size(), hipMemcpyHostToDevice, s_));
@@ -356,13 +444,13 @@ class Executor {
     }
 
     DCol filter_view(const Rel& r, uint32_t col) const {
-        return col == kPredColumn ? r.pred_view : column(r.table, col).view();
+        return col == kPredColumn ? r.pred_view : qview(r.table, col);
     }
 
     ScanSrc scan_src(const Rel& r, const AttrRef& key) const {
         ScanSrc s;
         std::memset(&s, 0, sizeof s);
-        s.key = column(r.table, key.col).view();
+        s.key = qview(r.table, key.col);
         s.nf = (int32_t)r.filters.size();
         for (size_t f = 0; f < r.filters.size(); ++f) {
             s.fcol[f] = filter_view(r, r.filters[f].column);
@@ -374,7 +462,7 @@ class Executor {
 
     RelSrc rel_src(const Rel& r, const AttrRef& key) const {
         RelSrc s;
-        s.key = column(r.slots[key.slot].table, key.col).view();
+        s.key = qview(r.slots[key.slot].table, key.col);
         s.rid = r.slots[key.slot].rid;
         s.n = r.n;
         return s;
@@ -532,8 +620,8 @@ class Executor {
         for (uint64_t i = 0; direct && i < nd.num_output_attrs; ++i) {
             const AttrRef& a = out.attrs[i];
             const Rel& src = used[a.slot].first == 0 ? L : R;
-            const DevColumn& dc = column(src.slots[used[a.slot].second].table, a.col);
-            direct = dc.type == HJ_INT32 && dc.flat && root.output_attrs[i].type == HJ_INT32;
+            const DCol dc = qview(src.slots[used[a.slot].second].table, a.col);
+            direct = dc.type == HJ_INT32 && root.output_attrs[i].type == HJ_INT32;
         }
         std::unique_ptr<hj_dresult> dres;
         if (direct) {
@@ -546,7 +634,7 @@ class Executor {
             for (uint64_t i = 0; i < nd.num_output_attrs; ++i) {
                 const AttrRef& a = out.attrs[i];
                 const Rel& src = used[a.slot].first == 0 ? L : R;
-                const DevColumn& dc = column(src.slots[used[a.slot].second].table, a.col);
+                const DCol dc = qview(src.slots[used[a.slot].second].table, a.col);
                 hj_dresult::Col c;
                 c.type = HJ_INT32;
                 c.values = result<int32_t>(total);
@@ -554,9 +642,11 @@ class Executor {
                 o.dst[i] = (uint32_t*)c.values;
                 o.src[i] = src.slots[used[a.slot].second].rid;
                 o.side[i] = (used[a.slot].first == 0) == bl ? 1 : 0;
-                o.flat[i] = dc.flat;
-                o.vbits[i] = dc.vbits;
-                o.valid8[i] = dc.vbits ? scratch<uint8_t>(total + 8) : nullptr;
+                o.col[i] = dc;
+                // validity bytes unless the column has no NULL row
+                o.valid8[i] = (dc.flat ? dc.vbits != nullptr : !dc.all_dense)
+                                  ? scratch<uint8_t>(total + 8)
+                                  : nullptr;
                 dres->cols.push_back(c);
             }
         } else {
@@ -623,6 +713,15 @@ class Executor {
         return true;
     }
 
+    // the pipeline's kernels read a column in 64-row groups with 32-bit buffer offsets: its
+    // row-space array, or its pages through the group index
+    static bool wave_ok(const DCol& d) {
+        const uint64_t G = (d.num_rows + 63) / 64;
+        if (d.flat) return d.num_rows * 4 < (1ull << 32);
+        return (d.gidx || d.umagic) && (uint64_t)d.num_pages * HJ_PAGE_SIZE < (1ull << 32) &&
+               G * 16 < (1ull << 32);
+    }
+
     static uint32_t lg2ceil(uint64_t n) {
         uint32_t k = 0;
         while ((1ull << k) < n) ++k;
@@ -685,11 +784,13 @@ class Executor {
             if (brel.n >= (1ull << 28)) return Pipe::kNo;  // key slots of the table
 
             BuildJob& jb = hb.job[i];
-            jb.key = bcol.view();
+            jb.key = qview(slot_table[bk.slot], bk.col);
+            if (!wave_ok(jb.key)) return Pipe::kNo;
             jb.nf = (int32_t)brel.filters.size();
             if (!brel.filters.empty()) any_filter = true;
             for (size_t f = 0; f < brel.filters.size(); ++f) {
                 jb.fcol[f] = filter_view(brel, brel.filters[f].column);
+                if (!wave_ok(jb.fcol[f])) return Pipe::kNo;
                 jb.flo[f] = brel.filters[f].lo;
                 jb.fhi[f] = brel.filters[f].hi;
             }
@@ -714,7 +815,8 @@ class Executor {
             PipeStage& st = hp.st[i];
             st.job = (int32_t)i;
             st.key_slot = pk.slot;
-            st.keycol = pcol.view();
+            st.keycol = qview(slot_table[pk.slot], pk.col);
+            if (pk.slot == 0 && !wave_ok(st.keycol)) return Pipe::kNo;
 
             const std::vector<AttrSlot>& L = bl ? bat : cur;
             const std::vector<AttrSlot>& R = bl ? cur : bat;
@@ -789,7 +891,7 @@ class Executor {
             const int st = hp.order[q];
             const BuildJob& jb = hb.job[hp.st[st].job];
             hp.qpos[st] = q;
-            hp.sj[q] = {hp.st[st].keycol.flat, hp.st[st].keycol.vbits, jb.table, jb.meta, jb.bf};
+            hp.sj[q] = {hp.st[st].keycol, jb.table, jb.meta, jb.bf};
         }
         // output: written directly from the fused pass's staged survivors when every join is
         // fused (at most one output row per source row: cap rows), else compacted row ids
@@ -839,7 +941,7 @@ class Executor {
                 const int ps = cur[i].slot;
                 const DevColumn& dc = column(slot_table[ps], cur[i].col);
                 if (dc.type != HJ_VARCHAR) continue;
-                hp.ostr[hp.nostr] = dc.view();
+                hp.ostr[hp.nostr] = qview(slot_table[ps], cur[i].col);
                 hp.ostr_slot[hp.nostr] = ps ? 1 + hp.qpos[ps - 1] : 0;
                 ++hp.nostr;
             }
@@ -909,7 +1011,7 @@ class Executor {
             if (dc.type != HJ_INT32 && dc.type != HJ_VARCHAR)
                 throw Error(HJ_E_TYPE, "unsupported output column type (only INT32 and VARCHAR)");
             EmitCol& ec = ed.c[i];
-            ec.col = dc.view();
+            ec.col = qview(slot_table[ps], cur[i].col);
             // an output column with no NULL row: a column without NULL, or a probe key
             ec.nonull = dc.all_dense ? 1 : 0;
             for (const AttrSlot& k : pipe_keys_)
@@ -1027,7 +1129,7 @@ class Executor {
             c.validity = result<uint8_t>(vbytes ? vbytes : 8);
             MatCol m;
             std::memset(&m, 0, sizeof m);
-            m.col = dc.view();
+            m.col = qview(sl.table, a.col);
             m.rid = sl.rid;
             m.type = dc.type;
             m.validity = c.validity;
@@ -1211,6 +1313,8 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
         }
         if (p == 0) first_rows = prow;
         if (p + 1 < hc.num_pages && prow != first_rows) uniform = false;
+        if (p + 1 < hc.num_pages && prow < 64) dc.regular = false;
+        if (hc.type == HJ_VARCHAR && (nr == 0xffffu || nr == 0xfffeu)) dc.regular = false;
         rows += prow;
         if (rows >= (1ull << 32)) throw Error(HJ_E_INVALID, "a table must have fewer than 2^32 rows");
     }
@@ -1256,24 +1360,10 @@ void upload_column(hj_context* ctx, const hj_column& hc, uint64_t table_rows, hj
                               ctx->stream));
         HJ_HIP(hipStreamSynchronize(ctx->stream));
     }
+    // page-level index only (the pages themselves stay as the caller gave them): each query
+    // indexes or decodes the rows it reads on its own
     dc.rank = (uint16_t*)dev_alloc(groups * 2);
     if (!dc.all_dense && groups) launch_rank_index(dc.view(), dc.rank, ctx->stream);
-    if (hc.type == HJ_INT32) {
-        dc.flat = (int32_t*)dev_alloc(rows * 4);
-        if (!dc.all_dense) {
-            const size_t vb = ((rows + 63) / 64 + 1) * 8;
-            dc.vbits = (uint64_t*)dev_alloc(vb);
-            HJ_HIP(hipMemsetAsync(dc.vbits, 0, vb, ctx->stream));
-        }
-        // decode through the page path (flat unset in this view)
-        DCol v = dc.view();
-        v.flat = nullptr;
-        v.vbits = nullptr;
-        launch_decode_int32(v, dc.flat, dc.vbits, ctx->stream);
-    } else if (hc.type == HJ_VARCHAR) {
-        dc.sref = (uint64_t*)dev_alloc(rows * 8);
-        launch_decode_varchar(dc.view(), dc.sref, ctx->stream);
-    }
     HJ_HIP(hipStreamSynchronize(ctx->stream));
 }
 

@@ -8,12 +8,17 @@
 //                                    non-NULL rows of the page before the group
 //   group_start u32[num_pages]       index of the page's first group in rank
 //   chunk_page  u32[rows/2048 + 2]   page holding the first row of each 2048-row chunk
-// and, for INT32 columns, a row-space mirror decoded from the pages at upload:
-//   flat        i32[rows]            the value of every row (0 for NULL)
-//   vbits       u64[rows/64 + 1]     validity, bit r & 63 of word r >> 6 (absent: no NULLs)
-// so that row r of an INT32 column is one load (plus one broadcast load of its validity word),
-// and 64 consecutive rows are one coalesced load: the same bytes as the pages, without the
-// page search. VARCHAR columns get sref[rows], one 8-byte reference per row (see DCol).
+// Every query then reads the pages themselves. Before its first kernel, a query indexes the
+// columns it reads in 64-row groups of row space (k_group_index, in the query's arena):
+//   gidx        u64[2 * groups]      [2g] the validity bits of rows 64g .. 64g + 63 (0 past the
+//                                    last row), [2g + 1] where the group's values are: the page
+//                                    of row 64g, the index of its first row among that page's
+//                                    non-NULL values, and how many of its rows are in that page
+//                                    (the rest open the next page)
+// so that 64 consecutive rows are one 16-byte record load and one load of packed values per
+// page they touch. A column with a page of fewer than 64 rows before its last page has no such
+// index: an INT32 column is decoded into a row-space array for the query (flat, vbits), and a
+// VARCHAR row is found by page search.
 // A row r is found by page search (or r / uniform_rows when every page but the last holds the
 // same number of rows, the dense 1984-row case of include/column_t.h:14), then its value index
 // among the page's packed non-NULL values is rank + popcount of the bitmap bits below it.
@@ -42,15 +47,21 @@ struct DCol {
     const uint16_t* rank;
     const uint32_t* group_start;
     const uint32_t* chunk_page;  // page holding row c * 2048, for c = 0 .. ceil(rows/2048)
-    const int32_t* flat;         // INT32: value per row (0 for NULL)
-    const uint64_t* vbits;       // INT32: validity per row; nullptr when the column has no NULL
+    const uint64_t* gidx;        // the query's group index (see above), or nullptr
+    const int32_t* flat;         // INT32 row-space array of the query (a decoded column without
+                                 // a group index, or a predicate's 0 / 1 results), or nullptr
+    const uint64_t* vbits;       // its validity per row; nullptr when it has no NULL
     const uint64_t* sref;        // VARCHAR: per row, kNullRef, kLongRef | first page, or
-                                 // (byte offset into pages) << 16 | length
+                                 // (byte offset into pages) << 16 | length; nullptr: none
     uint32_t num_pages;
     int32_t type;
     uint32_t uniform_rows;
     uint32_t all_dense;
     uint64_t num_rows;
+    // INT32 with every page dense and every page but the last of uniform_rows (>= 64) rows:
+    // ceil(2^64 / uniform_rows), so that the page of row r is umul64hi(r, umagic) and no index
+    // is needed; 0 otherwise
+    uint64_t umagic;
 };
 
 // ---------------------------------------------------------------- hashing
@@ -74,6 +85,29 @@ __host__ __device__ inline uint32_t slot_of(uint32_t h, uint32_t k) {
 
 constexpr uint64_t kNullRef = ~0ull;        // sref of a NULL string
 constexpr uint64_t kLongRef = 1ull << 63;    // sref of a long string: | its 0xffff page
+
+// ---------------------------------------------------------------- row-space group index
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+__host__ __device__ inline uint64_t gloc_pack(uint32_t page, uint32_t rank0, uint32_t split) {
+    return (uint64_t)page | ((uint64_t)rank0 << 32) | ((uint64_t)split << 48);
+}
+__device__ inline uint32_t gloc_page(uint64_t d) { return (uint32_t)d; }
+__device__ inline uint32_t gloc_rank0(uint64_t d) { return (uint32_t)(d >> 32) & 0xffffu; }
+__device__ inline uint32_t gloc_split(uint64_t d) { return (uint32_t)(d >> 48) & 0xffu; }
+
+// value index (among its page's non-NULL values) and page of row l (< 64) of a group with
+// validity bits vw and location d
+__device__ inline uint32_t gloc_value(uint64_t vw, uint64_t d, uint32_t l, uint32_t* page) {
+    const uint32_t split = gloc_split(d);
+    const uint64_t below = vw & ((1ull << l) - 1ull);
+    if (l < split) {
+        *page = gloc_page(d);
+        return gloc_rank0(d) + (uint32_t)__popcll(below);
+    }
+    *page = gloc_page(d) + 1u;
+    return (uint32_t)__popcll(below >> split);  // split <= l < 64
+}
 
 // ---------------------------------------------------------------- page access
 __device__ inline const HJ_GLOBAL uint8_t* page_ptr(const DCol& c, uint32_t p) {
@@ -149,9 +183,27 @@ __device__ inline uint64_t long_length(const DCol& c, uint32_t p) {
     return total;
 }
 
-// validity of row r of an INT32 column's row-space mirror
+// validity of row r of an INT32 column's row-space array
 __device__ inline bool flat_valid(const DCol& c, uint32_t r) {
     return !c.vbits || ((gp(c.vbits)[r >> 6] >> (r & 63u)) & 1ull);
+}
+
+// INT32 value of global row r through the group index (one 16-byte record, then the value)
+__device__ inline bool gidx_int32(const uint8_t* pages, const uint64_t* gidx, uint32_t r,
+                                  int32_t* v) {
+    const u64x2 e = *(const HJ_GLOBAL u64x2*)(gp(gidx) + 2u * (r >> 6));
+    const uint32_t l = r & 63u;
+    if (!((e.x >> l) & 1ull)) return false;
+    uint32_t page;
+    const uint32_t vi = gloc_value(e.x, e.y, l, &page);
+    *v = *(const HJ_GLOBAL int32_t*)(gp(pages) + (size_t)page * HJ_PAGE_BYTES + 4u + 4u * vi);
+    return true;
+}
+
+// page of row r of a dense column of uniform pages (exact for r < 2^32: the error of the
+// rounded-up reciprocal stays below 2^-32 < 1 / uniform_rows)
+__device__ inline uint32_t dense_page(uint64_t umagic, uint32_t r) {
+    return (uint32_t)__umul64hi((uint64_t)r, umagic);
 }
 
 // INT32 value of global row r; false for NULL
@@ -159,6 +211,12 @@ __device__ inline bool get_int32(const DCol& c, uint32_t r, int32_t* v) {
     if (c.flat) {
         *v = gp(c.flat)[r];
         return flat_valid(c, r);
+    }
+    if (c.gidx) return gidx_int32(c.pages, c.gidx, r, v);
+    if (c.umagic) {
+        const uint32_t p = dense_page(c.umagic, r);
+        *v = *(const HJ_GLOBAL int32_t*)(page_ptr(c, p) + 4u + 4u * (r - p * c.uniform_rows));
+        return true;
     }
     uint32_t p = find_page(c, r);
     const HJ_GLOBAL uint8_t* pg = page_ptr(c, p);
@@ -208,6 +266,33 @@ __device__ inline bool get_string(const DCol& c, uint32_t r, const uint8_t** src
     *len = (uint32_t)offs[vi] - start;
     *long_page = 0xffffffffu;
     return true;
+}
+
+// the reference of VARCHAR row r in the encoding of DCol::sref: kNullRef, kLongRef | its 0xffff
+// page, or (byte offset of its bytes in pages) << 16 | length
+__device__ inline uint64_t string_ref(const DCol& c, uint32_t r) {
+    if (c.sref) return gp(c.sref)[r];
+    uint32_t page, vi;
+    const HJ_GLOBAL uint8_t* pg;
+    uint32_t hdr;
+    if (c.gidx) {
+        const u64x2 e = *(const HJ_GLOBAL u64x2*)(gp(c.gidx) + 2u * (r >> 6));
+        const uint32_t l = r & 63u;
+        if (!((e.x >> l) & 1ull)) return kNullRef;
+        vi = gloc_value(e.x, e.y, l, &page);
+        pg = page_ptr(c, page);
+        hdr = *(const HJ_GLOBAL uint32_t*)pg;
+    } else {
+        page = find_page(c, r);
+        pg = page_ptr(c, page);
+        hdr = *(const HJ_GLOBAL uint32_t*)pg;
+        if ((hdr & 0xffffu) == 0xffffu) return kLongRef | page;
+        if (!value_index(c, page, pg, hdr, r - gp(c.row_start)[page], &vi)) return kNullRef;
+    }
+    const HJ_GLOBAL uint16_t* offs = (const HJ_GLOBAL uint16_t*)(pg + 4);
+    const uint32_t start = vi ? offs[vi - 1] : 0u;
+    const uint64_t at = (uint64_t)page * HJ_PAGE_BYTES + 4u + 2u * (hdr >> 16) + start;
+    return (at << 16) | (uint64_t)(offs[vi] - start);
 }
 
 // ---------------------------------------------------------------- wave / block primitives

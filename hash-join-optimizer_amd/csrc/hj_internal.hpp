@@ -71,6 +71,11 @@ struct Arena {
         lo = 0;
         hi = cap;
     }
+    // everything above scratch mark m and every result
+    void clear_to(size_t m) {
+        lo = m;
+        hi = cap;
+    }
     size_t free_bytes() const { return hi - lo; }
 };
 
@@ -85,11 +90,11 @@ struct DevColumn {
     uint16_t* rank = nullptr;      // device
     uint32_t* group_start = nullptr;
     uint32_t* chunk_page = nullptr;
-    int32_t* flat = nullptr;       // device, INT32 only: value per row
-    uint64_t* vbits = nullptr;     // device, INT32 with NULLs only: validity per row
-    uint64_t* sref = nullptr;      // device, VARCHAR only: string reference per row
     uint32_t uniform_rows = 0;
     uint32_t all_dense = 1;
+    // every page but the last holds >= 64 rows (and, VARCHAR, no long string): a query can index
+    // the column in 64-row groups of row space (each group in at most two pages)
+    bool regular = true;
     // VARCHAR: the first page past the reference's 18-bit StrRef page field that holds a string
     // (a non-NULL row or a long string's 0xffff page), ~0 when there is none
     uint64_t strref_page_over = ~0ull;
@@ -102,14 +107,20 @@ struct DevColumn {
         d.rank = rank;
         d.group_start = group_start;
         d.chunk_page = chunk_page;
-        d.flat = flat;
-        d.vbits = vbits;
-        d.sref = sref;
+        d.gidx = nullptr;
+        d.flat = nullptr;
+        d.vbits = nullptr;
+        d.sref = nullptr;
         d.num_pages = num_pages;
         d.type = type;
         d.uniform_rows = uniform_rows;
         d.all_dense = all_dense;
         d.num_rows = num_rows;
+        d.umagic = 0;
+        if (type == HJ_INT32 && all_dense && uniform_rows >= 64 && num_pages) {
+            // ceil(2^64 / uniform_rows) = floor((2^64 - 1) / uniform_rows) + 1
+            d.umagic = ~0ull / uniform_rows + 1ull;
+        }
         return d;
     }
 };

@@ -50,12 +50,11 @@ struct OutSlots {
     int32_t side[kMaxSlots];
     int32_t n;
     // value mode (a root join whose outputs are INT32 columns): entry c is output column c,
-    // dst[c] receives the value flat[c][row] (0 for NULL) and valid8[c] its validity byte, row
-    // being the build or probe row (side[c]) mapped through src[c]
+    // dst[c] receives the value of row `row` of col[c] (0 for NULL) and valid8[c] its validity
+    // byte, row being the build or probe row (side[c]) mapped through src[c]
     int32_t values;
-    const int32_t* flat[kMaxSlots];
-    const uint64_t* vbits[kMaxSlots];
     uint8_t* valid8[kMaxSlots];
+    DCol col[kMaxSlots];
 };
 
 struct HeavyEntry {
@@ -76,6 +75,16 @@ void launch_rank_index(const DCol& c, uint16_t* rank_out, hipStream_t s);
 void launch_decode_int32(const DCol& c, int32_t* flat, uint64_t* vbits, hipStream_t s);
 // VARCHAR pages -> sref[rows] (kNullRef / kLongRef | page / offset << 16 | length)
 void launch_decode_varchar(const DCol& c, uint64_t* sref, hipStream_t s);
+// group index of up to kIndexCols columns in one launch: out[c] receives 2 words per 64-row
+// group of col[c] (see hj_device.hpp); every page of col[c] but the last holds >= 64 rows
+constexpr int kIndexCols = 16;
+struct GroupIndexBatch {
+    DCol col[kIndexCols];
+    uint64_t* out[kIndexCols];
+    uint32_t blk0[kIndexCols + 1];  // first block of each column, blk0[n] = the grid
+    int32_t n;
+};
+void launch_group_index(const GroupIndexBatch& b, hipStream_t s);
 
 // ---- generic exclusive scans (tile = 4096 elements per block)
 size_t scan_partials_needed(uint64_t n);

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kB) void k_mat_a(MatBatch mb) {
                 ok = get_int32(mc.col, r, &v);
                 gp(mc.values)[i] = ok ? v : 0;
             } else {
-                const uint64_t ref = gp(mc.col.sref)[r];
+                const uint64_t ref = string_ref(mc.col, r);
                 ok = ref != kNullRef;
                 uint64_t l = 0;
                 if (ok) l = (ref & kLongRef) ? long_length(mc.col, (uint32_t)ref) : (ref & 0xffffu);
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
         }
         uint64_t ref = kNullRef, len = 0;
         if (have) {
-            ref = gp(ec.col.sref)[rid];
+            ref = string_ref(ec.col, rid);
             if (ref != kNullRef)
                 len = (ref & kLongRef) ? long_length(ec.col, (uint32_t)ref) : (ref & 0xffffu);
         }

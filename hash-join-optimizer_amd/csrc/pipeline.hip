@@ -121,6 +121,88 @@ __device__ inline uint32_t mask_rank(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// ---------------------------------------------------------------- columns read by waves
+// An INT32 column read in 64-row groups, straight from its pages: a wave loads the group
+// records it needs (lane u: one 16-byte record of the group index), then, per group, each lane
+// computes where its row's value sits in the packed page (the page of the group's first row for
+// the group's first `split` rows, the next page for the rest; the index among non-NULL values
+// from the record's first value index and a popcount of the validity bits below the lane) and
+// loads it. A dense column of uniform pages needs no index: its record is computed (every row
+// valid, the page by a multiply with the rounded-up reciprocal of the rows per page). A column
+// the query decoded into a row-space array is read from it: its record is the validity word,
+// the value at row * 4.
+enum : uint32_t { kWaveIndexed = 0, kWaveDense = 1, kWaveFlat = 2 };
+struct WaveCol {
+    __amdgpu_buffer_rsrc_t val;  // the pages (or the row-space array)
+    __amdgpu_buffer_rsrc_t rec;  // the group index (or the row-space validity words)
+    uint32_t mode;
+    uint32_t all_valid;          // ~0: a row-space array without validity words
+    uint32_t upr;                // dense: rows per page
+    uint64_t umagic;             // dense: ceil(2^64 / upr)
+};
+
+__device__ inline WaveCol wave_col(const DCol& c, uint32_t rows) {
+    const uint32_t G = (rows + 63u) >> 6;
+    WaveCol w;
+    w.all_valid = 0;
+    w.upr = c.uniform_rows;
+    w.umagic = c.umagic;
+    if (c.flat) {
+        w.val = make_rsrc(c.flat, rows * 4u);
+        w.rec = make_rsrc(c.vbits ? (const void*)c.vbits : (const void*)c.flat,
+                          c.vbits ? G * 8u : 0u);
+        w.mode = kWaveFlat;
+        w.all_valid = c.vbits ? 0u : ~0u;
+    } else {
+        w.val = make_rsrc(c.pages, c.num_pages * HJ_PAGE_BYTES);
+        w.rec = make_rsrc(c.gidx, c.gidx ? G * 16u : 0u);
+        w.mode = c.gidx ? kWaveIndexed : kWaveDense;
+    }
+    return w;
+}
+
+// the record of group grp (on the lane that loads it): validity bits in .x/.y, location in .z/.w
+__device__ inline u32x4 wave_rec(const WaveCol& w, uint32_t grp) {
+    if (w.mode == kWaveFlat) {
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(w.rec, grp * 8u, 0, 0);
+        return u32x4{v.x | w.all_valid, v.y | w.all_valid, 0u, 0u};
+    }
+    if (w.mode == kWaveDense) {
+        const uint32_t r = grp * 64u;
+        const uint32_t p = dense_page(w.umagic, r);
+        const uint32_t local0 = r - p * w.upr;
+        const uint32_t split = w.upr - local0 < 64u ? w.upr - local0 : 64u;
+        const uint64_t d = gloc_pack(p, local0, split);
+        return u32x4{~0u, ~0u, (uint32_t)d, (uint32_t)(d >> 32)};
+    }
+    return __builtin_amdgcn_raw_buffer_load_b128(w.rec, grp * 16u, 0, 0);
+}
+
+// group u's validity bits and location (wave-uniform) from the records held by lanes 0 .. U-1
+__device__ inline uint64_t rec_valid(const u32x4& r, int u) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)r.y, u) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)r.x, u);
+}
+__device__ inline uint64_t rec_loc(const u32x4& r, int u) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)r.w, u) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)r.z, u);
+}
+
+// this lane's byte offset in w.val for row 64 * grp + lane, given the group's validity bits vw
+// and location d (wave-uniform)
+__device__ inline uint32_t wave_offset(const WaveCol& w, uint32_t grp, uint64_t vw, uint64_t d,
+                                       uint32_t lane) {
+    if (w.mode == kWaveFlat) return (grp * 64u + lane) * 4u;
+    const uint32_t split = (uint32_t)(d >> 48) & 0xffu;
+    const uint32_t below = mask_rank(vw);  // valid rows of the group before this lane
+    const uint32_t in_first = split >= 64u ? (uint32_t)__popcll(vw)
+                                           : (uint32_t)__popcll(vw & ((1ull << split) - 1ull));
+    const bool first = lane < split;
+    const uint32_t vi = first ? ((uint32_t)(d >> 32) & 0xffffu) + below : below - in_first;
+    const uint32_t page = (uint32_t)d + (first ? 0u : 1u);
+    return page * HJ_PAGE_BYTES + 4u + 4u * vi;
+}
+
 // the lanes of m run what the condition guards (m is wave-uniform)
 __device__ inline bool in_mask(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
@@ -175,52 +257,43 @@ __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
     const BuildJob& jb = bb->job[build_job_of(bb, blockIdx.x)];
     const uint32_t x = blockIdx.x - jb.blk0;
-    const uint32_t rows = jb.rows, G = (rows + 63u) >> 6;
+    const uint32_t rows = jb.rows;
     const uint32_t r0 = x * kBuildRows;
     const uint32_t r1 = r0 + kBuildRows < rows ? r0 + kBuildRows : rows;
     if (t == 0) s_cnt = 0;
     __syncthreads();
-    const __amdgpu_buffer_rsrc_t rk = make_rsrc(jb.key.flat, rows * 4u);
-    const __amdgpu_buffer_rsrc_t rkv =
-        make_rsrc(jb.key.vbits ? (const void*)jb.key.vbits : (const void*)jb.key.flat,
-                  jb.key.vbits ? G * 8u : 0u);
-    const uint32_t kv_none = jb.key.vbits ? 0u : ~0u;
+    const WaveCol wk = wave_col(jb.key, rows);
     const int nf = jb.nf;
-    // lane u < kFU reads the validity word of group u of the wave's rows
-    const uint32_t lanev = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 32u;
+    // lane u < kFU reads the record of group u of the wave's rows
+    const uint32_t lu = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 4u;
     for (uint32_t base = r0; base < r1; base += kB * kFU) {
         // wave w takes groups base/64 + 4u + w, u < kFU
-        const uint32_t vo = (base + w * 64u + lane) * 4u;
-        const uint32_t vv = (base / 64u + w) * 8u + lanev;
+        const uint32_t grp0 = base / 64u + w;
+        const u32x4 kr = wave_rec(wk, grp0 + lu);
         int32_t key[kFU];
-#pragma unroll
-        for (int u = 0; u < kFU; ++u)
-            key[u] = __builtin_amdgcn_raw_buffer_load_b32(rk, vo + u * 1024u, 0, 0);
-        const u32x2 kvw = __builtin_amdgcn_raw_buffer_load_b64(rkv, vv, 0, 0);
         uint64_t live[kFU];
 #pragma unroll
         for (int u = 0; u < kFU; ++u) {
+            const uint64_t vw = rec_valid(kr, u);
+            key[u] = __builtin_amdgcn_raw_buffer_load_b32(
+                wk.val, wave_offset(wk, grp0 + 4u * u, vw, rec_loc(kr, u), lane), 0, 0);
             const uint32_t g0 = base + u * 256u + w * 64u;
-            live[u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) &
-                      lane_word(kvw.x | kv_none, kvw.y | kv_none, u);
+            live[u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) & vw;
         }
         for (int f = 0; f < nf; ++f) {
-            const DCol& fc = jb.fcol[f];
-            const __amdgpu_buffer_rsrc_t rf = make_rsrc(fc.flat, rows * 4u);
-            const __amdgpu_buffer_rsrc_t rfv =
-                make_rsrc(fc.vbits ? (const void*)fc.vbits : (const void*)fc.flat,
-                          fc.vbits ? G * 8u : 0u);
-            const uint32_t fv_none = fc.vbits ? 0u : ~0u;
+            const WaveCol wf = wave_col(jb.fcol[f], rows);
             const int32_t lo = jb.flo[f], hi = jb.fhi[f];
+            const u32x4 fr = wave_rec(wf, grp0 + lu);
             int32_t v[kFU];
+            uint64_t fvw[kFU];
 #pragma unroll
-            for (int u = 0; u < kFU; ++u)
-                v[u] = __builtin_amdgcn_raw_buffer_load_b32(rf, vo + u * 1024u, 0, 0);
-            const u32x2 fvw = __builtin_amdgcn_raw_buffer_load_b64(rfv, vv, 0, 0);
+            for (int u = 0; u < kFU; ++u) {
+                fvw[u] = rec_valid(fr, u);
+                v[u] = __builtin_amdgcn_raw_buffer_load_b32(
+                    wf.val, wave_offset(wf, grp0 + 4u * u, fvw[u], rec_loc(fr, u), lane), 0, 0);
+            }
 #pragma unroll
-            for (int u = 0; u < kFU; ++u)
-                live[u] &= __ballot(v[u] >= lo && v[u] <= hi) &
-                           lane_word(fvw.x | fv_none, fvw.y | fv_none, u);
+            for (int u = 0; u < kFU; ++u) live[u] &= __ballot(v[u] >= lo && v[u] <= hi) & fvw[u];
         }
         uint32_t pos[kFU];
         block_append_masks<kFU>(&s_cnt, live, pos);
@@ -252,7 +325,7 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
     const int job = build_job_of(bb, blockIdx.x);
     const BuildJob& jb = bb->job[job];
     const uint32_t x = blockIdx.x - jb.blk0;
-    const uint32_t rows = jb.rows, G = (rows + 63u) >> 6;
+    const uint32_t rows = jb.rows;
     const uint32_t r0 = x * kBuildRows;
     const uint32_t r1 = r0 + kBuildRows < rows ? r0 + kBuildRows : rows;
     const uint32_t fb = jb.pbits, P = 1u << fb;
@@ -260,79 +333,74 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
         for (uint32_t i = t; i < kBloomWords; i += kB) gp(jb.bf)[i] = 0;
     for (uint32_t i = t; i <= P; i += kB) hist[i] = 0;
     __syncthreads();
-    const __amdgpu_buffer_rsrc_t rk = make_rsrc(jb.key.flat, rows * 4u);
-    const __amdgpu_buffer_rsrc_t rkv =
-        make_rsrc(jb.key.vbits ? (const void*)jb.key.vbits : (const void*)jb.key.flat,
-                  jb.key.vbits ? G * 8u : 0u);
-    const uint32_t kv_none = jb.key.vbits ? 0u : ~0u;
+    const WaveCol wk = wave_col(jb.key, rows);
     const int nf = jb.nf;
-    const uint32_t lanev = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 32u;
+    const uint32_t lu = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 4u;
     int32_t key[kFR][kFU];
     uint32_t rank[kFR][kFU];
     uint32_t alive = 0;  // bit r * kFU + u: this lane's row of round r, group u passes
-    // every round's keys and validity words at once, with the first filter's column, then the
-    // other filters' columns the same way
+    // every round's group records at once (key column and first filter column), then every
+    // round's keys and filter values, then the other filters' columns the same way
     uint64_t live[kFR][kFU];
     {
-        u32x2 kvw[kFR], fvw[kFR];
-        int32_t v[kFR][kFU];
-        const DCol& fc = jb.fcol[0];
-        const __amdgpu_buffer_rsrc_t rf = make_rsrc(fc.flat, nf ? rows * 4u : 0u);
-        const __amdgpu_buffer_rsrc_t rfv =
-            make_rsrc(fc.vbits ? (const void*)fc.vbits : (const void*)fc.flat,
-                      nf && fc.vbits ? G * 8u : 0u);
-        const uint32_t fv_none = nf && fc.vbits ? 0u : ~0u;
+        const WaveCol wf = nf ? wave_col(jb.fcol[0], rows) : wk;
         const int32_t lo = nf ? jb.flo[0] : INT32_MIN, hi = nf ? jb.fhi[0] : INT32_MAX;
+        u32x4 kr[kFR], fr[kFR];
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd) {
-            const uint32_t base = r0 + rd * (kB * kFU);
-            const uint32_t vo = (base + w * 64u + lane) * 4u;
+            const uint32_t grp0 = (r0 + rd * (kB * kFU)) / 64u + w;
+            kr[rd] = wave_rec(wk, grp0 + lu);
+            if (nf) fr[rd] = wave_rec(wf, grp0 + lu);
+        }
+        int32_t v[kFR][kFU];
 #pragma unroll
-            for (int u = 0; u < kFU; ++u)
-                key[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(rk, vo + u * 1024u, 0, 0);
-            kvw[rd] = __builtin_amdgcn_raw_buffer_load_b64(rkv, (base / 64u + w) * 8u + lanev, 0, 0);
+        for (int rd = 0; rd < kFR; ++rd) {
+            const uint32_t grp0 = (r0 + rd * (kB * kFU)) / 64u + w;
 #pragma unroll
-            for (int u = 0; u < kFU; ++u)
-                v[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(rf, vo + u * 1024u, 0, 0);
-            fvw[rd] = __builtin_amdgcn_raw_buffer_load_b64(rfv, (base / 64u + w) * 8u + lanev, 0, 0);
+            for (int u = 0; u < kFU; ++u) {
+                const uint32_t g = grp0 + 4u * u;
+                key[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(
+                    wk.val, wave_offset(wk, g, rec_valid(kr[rd], u), rec_loc(kr[rd], u), lane), 0, 0);
+                v[rd][u] = nf ? __builtin_amdgcn_raw_buffer_load_b32(
+                                    wf.val,
+                                    wave_offset(wf, g, rec_valid(fr[rd], u), rec_loc(fr[rd], u), lane),
+                                    0, 0)
+                              : 0;
+            }
         }
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd)
 #pragma unroll
             for (int u = 0; u < kFU; ++u) {
                 const uint32_t g0 = r0 + rd * (kB * kFU) + u * 256u + w * 64u;
-                // without a filter the loads read 0 (empty resource) and the range passes all
-                live[rd][u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) &
-                              lane_word(kvw[rd].x | kv_none, kvw[rd].y | kv_none, u) &
-                              __ballot(v[rd][u] >= lo && v[rd][u] <= hi) &
-                              lane_word(fvw[rd].x | fv_none, fvw[rd].y | fv_none, u);
+                live[rd][u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) & rec_valid(kr[rd], u) &
+                              (nf ? __ballot(v[rd][u] >= lo && v[rd][u] <= hi) &
+                                        rec_valid(fr[rd], u)
+                                  : ~0ull);
             }
     }
     for (int f = 1; f < nf; ++f) {
-        const DCol& fc = jb.fcol[f];
-        const __amdgpu_buffer_rsrc_t rf = make_rsrc(fc.flat, rows * 4u);
-        const __amdgpu_buffer_rsrc_t rfv =
-            make_rsrc(fc.vbits ? (const void*)fc.vbits : (const void*)fc.flat,
-                      fc.vbits ? G * 8u : 0u);
-        const uint32_t fv_none = fc.vbits ? 0u : ~0u;
+        const WaveCol wf = wave_col(jb.fcol[f], rows);
         const int32_t lo = jb.flo[f], hi = jb.fhi[f];
+        u32x4 fr[kFR];
+#pragma unroll
+        for (int rd = 0; rd < kFR; ++rd) fr[rd] = wave_rec(wf, (r0 + rd * (kB * kFU)) / 64u + w + lu);
         int32_t v[kFR][kFU];
-        u32x2 fvw[kFR];
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd) {
-            const uint32_t base = r0 + rd * (kB * kFU);
-            const uint32_t vo = (base + w * 64u + lane) * 4u;
+            const uint32_t grp0 = (r0 + rd * (kB * kFU)) / 64u + w;
 #pragma unroll
             for (int u = 0; u < kFU; ++u)
-                v[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(rf, vo + u * 1024u, 0, 0);
-            fvw[rd] = __builtin_amdgcn_raw_buffer_load_b64(rfv, (base / 64u + w) * 8u + lanev, 0, 0);
+                v[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(
+                    wf.val,
+                    wave_offset(wf, grp0 + 4u * u, rec_valid(fr[rd], u), rec_loc(fr[rd], u), lane),
+                    0, 0);
         }
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd)
 #pragma unroll
             for (int u = 0; u < kFU; ++u)
-                live[rd][u] &= __ballot(v[rd][u] >= lo && v[rd][u] <= hi) &
-                               lane_word(fvw[rd].x | fv_none, fvw[rd].y | fv_none, u);
+                live[rd][u] &= __ballot(v[rd][u] >= lo && v[rd][u] <= hi) & rec_valid(fr[rd], u);
     }
 #pragma unroll
     for (int rd = 0; rd < kFR; ++rd)
@@ -680,7 +748,7 @@ __device__ inline void count_bytes(const PipeDesc& pd, uint64_t* s_bytes, uint32
         uint32_t rid = row;
 #pragma unroll
         for (int q = 0; q < NS; ++q) rid = sl == 1 + q ? brow[q] : rid;
-        const uint64_t ref = gp(pd.ostr[c].sref)[rid];
+        const uint64_t ref = string_ref(pd.ostr[c], rid);
         if (ref == kNullRef) continue;
         const uint64_t len =
             (ref & kLongRef) ? long_length(pd.ostr[c], (uint32_t)ref) : (ref & 0xffffu);
@@ -783,17 +851,13 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     __syncthreads();
     const uint32_t nsrc = pd.src_rows;
     const uint32_t G = (nsrc + 63u) >> 6;
-    __amdgpu_buffer_rsrc_t rk[NS], rv[NS];  // key columns, their validity words (0 bytes: none)
-    uint32_t nonull[NS];                    // ~0: the key column has no NULL
+    WaveCol wc[NS];  // the key columns, read from their pages
     const HJ_GLOBAL uint32_t* tkey[NS];
     uint32_t tk[NS], tsp[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
         const StreamJoin& sj = pd.sj[q];
-        rk[q] = make_rsrc(sj.flat, nsrc * 4u);
-        rv[q] = make_rsrc(sj.vbits ? (const void*)sj.vbits : (const void*)sj.flat,
-                          sj.vbits ? G * 8u : 0u);
-        nonull[q] = sj.vbits ? 0u : ~0u;
+        wc[q] = wave_col(sj.key, nsrc);
         tkey[q] = gp(sj.table);
         tk[q] = table_bits(gp(sj.meta)[0]);
         tsp[q] = gp(sj.meta)[2];
@@ -806,21 +870,28 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     const uint32_t g_hi = g_lo + pd.probe_gpb < G ? g_lo + pd.probe_gpb : G;
     const uint32_t st0 = g_lo * 64u;
     uint32_t qc = 0;  // entries in this wave's queue (uniform)
-    // keys and validity words of every streamed join are fetched one iteration ahead into one
-    // of two register sets (alternating), so that they are in flight while the current groups
-    // are filtered and looked up. Lane u < U holds the validity word of group u.
-    const uint32_t lane4 = lane * 4u;
-    const uint32_t lanev = (lane < (uint32_t)U ? lane : (uint32_t)U - 1u) * 8u;
+    // Every streamed join's group records are fetched two iterations ahead and its keys one
+    // iteration ahead (the keys' page offsets come from the records), each into one of two
+    // register sets (alternating), so that both are in flight while the current groups are
+    // filtered and looked up. Lane u < U holds the record and the validity word of group u.
+    const uint32_t lu = lane < (uint32_t)U ? lane : (uint32_t)U - 1u;
     int32_t ka[NS][U], kb[NS][U];
     u32x2 va[NS], vb[NS];
-    auto fetch = [&](int32_t (&kx)[NS][U], u32x2 (&vx)[NS], uint32_t gx) {
-        const uint32_t vo = gx * (U * 256u) + lane4, vv = gx * (U * 8u) + lanev;
+    u32x4 ra[NS], rb[NS];
+    auto fetch_rec = [&](u32x4 (&rx)[NS], uint32_t gx) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) rx[q] = wave_rec(wc[q], gx * U + lu);
+    };
+    auto fetch = [&](int32_t (&kx)[NS][U], u32x2 (&vx)[NS], const u32x4 (&rx)[NS], uint32_t gx) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
+            vx[q] = u32x2{rx[q].x, rx[q].y};
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                kx[q][u] = __builtin_amdgcn_raw_buffer_load_b32(rk[q], vo + u * 256u, 0, 0);
-            vx[q] = __builtin_amdgcn_raw_buffer_load_b64(rv[q], vv, 0, 0);
+                kx[q][u] = __builtin_amdgcn_raw_buffer_load_b32(
+                    wc[q].val,
+                    wave_offset(wc[q], gx * U + u, rec_valid(rx[q], u), rec_loc(rx[q], u), lane),
+                    0, 0);
         }
     };
     // one iteration: the U groups of wave-iteration gx, whose keys and validity words are in
@@ -829,8 +900,8 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
         uint32_t vl = ~0u, vh = ~0u;
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-            vl &= vw[q].x | nonull[q];
-            vh &= vw[q].y | nonull[q];
+            vl &= vw[q].x;
+            vh &= vw[q].y;
         }
         // live rows of each group: in this block's range, non-NULL in every streamed join
         uint64_t live[U];
@@ -924,14 +995,21 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
             }
         }
     };
+    // Issue order per iteration: the records two iterations ahead, then the keys of the next
+    // iteration (waiting only for that iteration's records, which were issued before the
+    // current keys: vector loads complete in order), then the current groups.
     uint32_t gb = g_lo / U + w;
-    fetch(ka, va, gb);
+    fetch_rec(ra, gb);
+    fetch_rec(rb, gb + kNW);
+    fetch(ka, va, ra, gb);
     while (gb * U < g_hi) {
-        fetch(kb, vb, gb + kNW);
+        fetch_rec(ra, gb + 2u * kNW);
+        fetch(kb, vb, rb, gb + kNW);
         step(ka, va, gb);
         gb += kNW;
         if (gb * U >= g_hi) break;
-        fetch(ka, va, gb + kNW);
+        fetch_rec(rb, gb + 2u * kNW);
+        fetch(ka, va, ra, gb + kNW);
         step(kb, vb, gb);
         gb += kNW;
     }

@@ -107,8 +107,7 @@ struct PipeStage {
 // a join of the fused pass, in probe order (filled by the host, so that the pass reaches every
 // pointer it needs in one load from the descriptor)
 struct StreamJoin {
-    const int32_t* flat;    // key column of the source: values ...
-    const uint64_t* vbits;  // ... and validity words (nullptr: no NULL)
+    DCol key;               // key column of the source (pages and group index) ...
     const uint32_t* table;  // the build's table ...
     const uint32_t* meta;   // ... its counters ([0] tuples, [2] kEmptyKey present) ...
     const uint32_t* bf;     // ... and Bloom words

@@ -50,7 +50,7 @@ struct StrCursor {
 };
 
 __device__ inline bool open_string(const DCol& c, uint32_t row, StrCursor* cur) {
-    const uint64_t ref = gp(c.sref)[row];
+    const uint64_t ref = string_ref(c, row);
     if (ref == kNullRef) return false;
     cur->nbuf = 0;
     if (ref & kLongRef) {

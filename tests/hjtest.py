@@ -326,3 +326,66 @@ def dresult_int32(dres, c):
     bits = device_to_numpy(valid.value, (n + 7) // 8, np.uint8)
     ok = np.unpackbits(bits, bitorder="little")[:n].astype(bool)
     return v, ok
+
+
+# ------------------------------------------------------------------ pages of chosen row counts
+def _bitmap(valid):
+    nb = (len(valid) + 7) // 8
+    bits = np.zeros(nb * 8, dtype=np.uint8)
+    bits[:len(valid)] = valid
+    return np.packbits(bits, bitorder="little")
+
+
+def int32_pages(values, valid, sizes):
+    """an INT32 Column whose page i holds sizes[i] rows (the page format of include/plan.h:
+    u16 rows, u16 non-NULL count, the non-NULL values packed from byte 4, the validity bitmap
+    ending at the page end), whatever the reference's writer would have cut"""
+    values = np.asarray(values, dtype=np.int32)
+    valid = np.asarray(valid, dtype=np.uint8)
+    assert sum(sizes) == len(values)
+    pages, r = [], 0
+    for n in sizes:
+        v, ok = values[r:r + n], valid[r:r + n]
+        r += n
+        nn = int(ok.sum())
+        bm = _bitmap(ok)
+        assert 4 + 4 * nn + len(bm) <= _plan.PAGE_SIZE, "rows do not fit one page"
+        pg = np.zeros(_plan.PAGE_SIZE, dtype=np.uint8)
+        pg[0:4] = np.array([n, nn], dtype=np.uint16).view(np.uint8)
+        pg[4:4 + 4 * nn] = v[ok == 1].view(np.uint8)
+        pg[_plan.PAGE_SIZE - len(bm):] = bm
+        pages.append(pg)
+    return _plan.Column(DataType.INT32, np.stack(pages) if pages else None)
+
+
+def varchar_pages(strings, sizes):
+    """a VARCHAR Column whose page i holds sizes[i] rows (u16 rows, u16 non-NULL count, u16 end
+    offsets of the non-NULL strings, their bytes, the bitmap ending at the page end)"""
+    assert sum(sizes) == len(strings)
+    pages, r = [], 0
+    for n in sizes:
+        part = strings[r:r + n]
+        r += n
+        ok = np.array([s is not None for s in part], dtype=np.uint8)
+        bs = [s.encode() for s in part if s is not None]
+        ends = np.cumsum([len(b) for b in bs]).astype(np.uint16) if bs else np.zeros(0, np.uint16)
+        bm = _bitmap(ok)
+        body = b"".join(bs)
+        nn = len(bs)
+        assert 4 + 2 * nn + len(body) + len(bm) <= _plan.PAGE_SIZE, "rows do not fit one page"
+        pg = np.zeros(_plan.PAGE_SIZE, dtype=np.uint8)
+        pg[0:4] = np.array([n, nn], dtype=np.uint16).view(np.uint8)
+        pg[4:4 + 2 * nn] = ends.view(np.uint8)
+        pg[4 + 2 * nn:4 + 2 * nn + len(body)] = np.frombuffer(body, dtype=np.uint8)
+        pg[_plan.PAGE_SIZE - len(bm):] = bm
+        pages.append(pg)
+    return _plan.Column(DataType.VARCHAR, np.stack(pages) if pages else None)
+
+
+def split_sizes(rng, n, lo, hi):
+    sizes = []
+    while n > 0:
+        k = min(n, int(rng.integers(lo, hi + 1)))
+        sizes.append(k)
+        n -= k
+    return sizes
