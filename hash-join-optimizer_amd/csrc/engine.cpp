@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstddef>
 #include <cstdio>
 #include <cstdlib>
@@ -62,13 +63,60 @@ void set_global_error(const std::string& m) {
     g_err = m;
 }
 
+// Wait for a query's round trip: poll the stream for up to 2 ms (a short query's result is
+// ready within microseconds, and a sleeping host thread takes longer than that to wake), then
+// block.
+void wait_stream(hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady)
+            throw Error(HJ_E_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000)) break;
+    }
+    HJ_HIP(hipStreamSynchronize(s));
+}
+
 template <class T>
 T fetch(hj_context* ctx, const T* dptr) {
     HJ_HIP(hipMemcpyAsync(ctx->pinned, dptr, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-    HJ_HIP(hipStreamSynchronize(ctx->stream));
+    wait_stream(ctx->stream);
     T v;
     std::memcpy(&v, ctx->pinned, sizeof(T));
     return v;
+}
+
+// host-side timing of a query's phases (environment HJ_TRACE_HOST=1): microseconds since the
+// call began, printed to stderr when it returns
+struct HostTrace {
+    bool on = std::getenv("HJ_TRACE_HOST") != nullptr;
+    std::chrono::steady_clock::time_point t0;
+    std::vector<std::pair<const char*, double>> marks;
+    void begin() {
+        if (!on) return;
+        marks.clear();
+        t0 = std::chrono::steady_clock::now();
+    }
+    void mark(const char* what) {
+        if (!on) return;
+        marks.push_back({what, std::chrono::duration<double, std::micro>(
+                                   std::chrono::steady_clock::now() - t0).count()});
+    }
+    void print() {
+        if (!on) return;
+        std::string line = "[host]";
+        char b[64];
+        for (auto& m : marks) {
+            std::snprintf(b, sizeof b, " %s %.1f", m.first, m.second);
+            line += b;
+        }
+        std::fprintf(stderr, "%s\n", line.c_str());
+    }
+};
+HostTrace& host_trace() {
+    static thread_local HostTrace t;
+    return t;
 }
 
 uint32_t log2_pow2_ceil(uint64_t n) {
@@ -142,13 +190,16 @@ class Executor {
     Executor(hj_context* ctx, const hj_plan* plan, const hj_dtable* const* tables)
         : ctx_(ctx), plan_(plan), tables_(tables), s_(ctx->stream), A_(ctx->arena) {}
 
+
     hj_dresult* run() {
         if (!plan_->nodes || plan_->root >= plan_->num_nodes)
             throw Error(HJ_E_INVALID, "plan root out of range");
         depth_ = 0;
         check_strrefs(plan_->root);
         depth_ = 0;
+        host_trace().mark("checked");
         prepare();
+        host_trace().mark("prepared");
         if (ctx_->use_pipeline) {
             for (int attempt = 0; attempt < 2; ++attempt) {
                 Rel pr;
@@ -160,6 +211,7 @@ class Executor {
                 if (st != Pipe::kRetry) break;  // not a chain, or a duplicate build key
             }
         }
+        flush_pending();
         Rel r = node(plan_->root);
         if (direct_) return direct_;  // the root join wrote its output columns itself
         return materialize(r, plan_->nodes[plan_->root]);
@@ -202,9 +254,26 @@ class Executor {
     // and VARCHAR column that a scan of the plan reads: outputs, filters and predicates. This is
     // the query's first device work; the reference's copy_scan does the same job in its execute()
     // (src/execute.cpp:265-369).
+    //
+    // A left-deep chain's builds read only the build scans' INT32 keys and filters, so the
+    // probe side's columns (the source scan's, and strings that are only output) are indexed by
+    // extra blocks of the build's filter launch (pending_), after it on the join-at-a-time path.
+    // Dense INT32 columns of uniform pages need no index (arithmetic) unless the fused probe
+    // streams them (the source table's).
     void prepare() {
         qcols_.clear();
-        std::vector<std::pair<uint32_t, uint32_t>> want;
+        std::memset(&pending_, 0, sizeof pending_);
+        uint64_t source = 0;
+        std::vector<uint64_t> joins;
+        int64_t src_table = -1;
+        if (ctx_->use_pipeline && chain(plan_->root, source, joins, 0) && !joins.empty() &&
+            plan_->nodes[source].base_table_id < plan_->num_inputs)
+            src_table = (int64_t)plan_->nodes[source].base_table_id;
+        struct Want {
+            uint32_t t, c;
+            bool late;  // may wait for the build launch
+        };
+        std::vector<Want> want;
         for (uint64_t i = 0; i < plan_->num_nodes; ++i) {
             const hj_node& nd = plan_->nodes[i];
             if (nd.kind != HJ_NODE_SCAN || nd.base_table_id >= plan_->num_inputs ||
@@ -212,37 +281,47 @@ class Executor {
                 continue;
             const uint32_t t = (uint32_t)nd.base_table_id;
             const size_t nc = tables_[t]->cols.size();
-            auto add = [&](uint64_t c) {
-                if (c < nc) want.push_back({t, (uint32_t)c});
+            const bool src = (int64_t)t == src_table;
+            auto add = [&](uint64_t c, bool late) {
+                if (c < nc) want.push_back({t, (uint32_t)c, src_table >= 0 && late});
             };
-            for (uint64_t a = 0; a < nd.num_output_attrs && nd.output_attrs; ++a)
-                add(nd.output_attrs[a].index);
-            for (uint64_t f = 0; f < nd.num_filters && nd.filters; ++f) add(nd.filters[f].column);
+            for (uint64_t a = 0; a < nd.num_output_attrs && nd.output_attrs; ++a) {
+                const uint64_t c = nd.output_attrs[a].index;
+                add(c, src || (c < nc && tables_[t]->cols[c].type == HJ_VARCHAR));
+            }
+            for (uint64_t f = 0; f < nd.num_filters && nd.filters; ++f) add(nd.filters[f].column, src);
+            // predicate programs run on the scan's rows before the builds
             for (uint64_t e = 0; e < nd.num_preds && nd.preds; ++e)
-                if (nd.preds[e].op < HJ_P_AND) add(nd.preds[e].column);
+                if (nd.preds[e].op < HJ_P_AND) add(nd.preds[e].column, false);
         }
-        GroupIndexBatch gb;
-        std::memset(&gb, 0, sizeof gb);
-        auto flush = [&] {
-            if (!gb.n) return;
-            { Prof p_(ctx_, "group_index"); launch_group_index(gb, s_); }
-            std::memset(&gb, 0, sizeof gb);
+        for (auto& w : want)  // wanted early anywhere: early
+            for (auto& o : want)
+                if (!o.late && o.t == w.t && o.c == w.c) w.late = false;
+        GroupIndexBatch now;
+        std::memset(&now, 0, sizeof now);
+        auto launch_now = [&](GroupIndexBatch& b) {
+            if (!b.n) return;
+            { Prof p_(ctx_, "group_index"); launch_group_index(b, s_); }
+            std::memset(&b, 0, sizeof b);
         };
-        for (auto& tc : want) {
-            const uint64_t key = qkey(tc.first, tc.second);
+        for (auto& w : want) {
+            const uint64_t key = qkey(w.t, w.c);
             if (qcols_.count(key)) continue;
-            const DevColumn& dc = column(tc.first, tc.second);
+            const DevColumn& dc = column(w.t, w.c);
             if (dc.type != HJ_INT32 && dc.type != HJ_VARCHAR) continue;
             QCol q;
             const uint64_t G = (dc.num_rows + 63) / 64;
-            if (dc.view().umagic) continue;  // dense, uniform pages: addressed by arithmetic
+            const bool dense = dc.view().umagic != 0;
+            if (dense && (int64_t)w.t != src_table) continue;  // addressed by arithmetic
             if (dc.regular) {
                 q.gidx = scratch<uint64_t>(2 * G + 2);
                 if (G) {
-                    gb.col[gb.n] = dc.view();
-                    gb.out[gb.n] = q.gidx;
-                    gb.blk0[gb.n + 1] = gb.blk0[gb.n] + (uint32_t)((G + 255) / 256);
-                    if (++gb.n == kIndexCols) flush();
+                    GroupIndexBatch& b = w.late ? pending_ : now;
+                    if (b.n == kIndexCols) launch_now(b);  // a full batch runs at once
+                    b.col[b.n] = dc.view();
+                    b.out[b.n] = q.gidx;
+                    b.blk0[b.n + 1] = b.blk0[b.n] + (uint32_t)((G + 255) / 256);
+                    ++b.n;
                 }
             } else if (dc.type == HJ_INT32) {
                 q.flat = scratch<int32_t>(dc.num_rows + 1);
@@ -257,8 +336,16 @@ class Executor {
             }
             qcols_[key] = q;
         }
-        flush();
+        launch_now(now);
         prep_mark_ = A_.mark();
+    }
+
+    // the probe side's index work not yet run (a pipeline build takes it into its filter launch)
+    GroupIndexBatch pending_;
+    void flush_pending() {
+        if (!pending_.n) return;
+        { Prof p_(ctx_, "group_index"); launch_group_index(pending_, s_); }
+        std::memset(&pending_, 0, sizeof pending_);
     }
 
     template <class T>
@@ -713,13 +800,21 @@ class Executor {
         return true;
     }
 
-    // the pipeline's kernels read a column in 64-row groups with 32-bit buffer offsets: its
-    // row-space array, or its pages through the group index
-    static bool wave_ok(const DCol& d) {
+    // the pipeline's kernels stream keys in 64-row groups from their pages through the group
+    // index, with 32-bit buffer offsets; a filter column may also be a row-space array
+    static bool idx_ok(const DCol& d) {
         const uint64_t G = (d.num_rows + 63) / 64;
-        if (d.flat) return d.num_rows * 4 < (1ull << 32);
-        return (d.gidx || d.umagic) && (uint64_t)d.num_pages * HJ_PAGE_SIZE < (1ull << 32) &&
+        return !d.flat && d.gidx && (uint64_t)d.num_pages * HJ_PAGE_SIZE < (1ull << 32) &&
                G * 16 < (1ull << 32);
+    }
+    // a build scan's key or filter column: also a dense column of uniform pages (arithmetic)
+    static bool build_ok(const DCol& d) {
+        if (d.flat) return false;
+        return (d.gidx ? idx_ok(d) : d.umagic != 0) &&
+               (uint64_t)d.num_pages * HJ_PAGE_SIZE < (1ull << 32);
+    }
+    static bool filter_ok(const DCol& d) {
+        return d.flat ? d.num_rows * 4 < (1ull << 32) : build_ok(d);
     }
 
     static uint32_t lg2ceil(uint64_t n) {
@@ -785,12 +880,12 @@ class Executor {
 
             BuildJob& jb = hb.job[i];
             jb.key = qview(slot_table[bk.slot], bk.col);
-            if (!wave_ok(jb.key)) return Pipe::kNo;
+            if (!build_ok(jb.key)) return Pipe::kNo;
             jb.nf = (int32_t)brel.filters.size();
             if (!brel.filters.empty()) any_filter = true;
             for (size_t f = 0; f < brel.filters.size(); ++f) {
                 jb.fcol[f] = filter_view(brel, brel.filters[f].column);
-                if (!wave_ok(jb.fcol[f])) return Pipe::kNo;
+                if (!filter_ok(jb.fcol[f])) return Pipe::kNo;
                 jb.flo[f] = brel.filters[f].lo;
                 jb.fhi[f] = brel.filters[f].hi;
             }
@@ -816,7 +911,7 @@ class Executor {
             st.job = (int32_t)i;
             st.key_slot = pk.slot;
             st.keycol = qview(slot_table[pk.slot], pk.col);
-            if (pk.slot == 0 && !wave_ok(st.keycol)) return Pipe::kNo;
+            if (pk.slot == 0 && !idx_ok(st.keycol)) return Pipe::kNo;
 
             const std::vector<AttrSlot>& L = bl ? bat : cur;
             const std::vector<AttrSlot>& R = bl ? cur : bat;
@@ -847,6 +942,11 @@ class Executor {
                 jb.poff = scratch<uint16_t>((uint64_t)jb.nblk * (P + 1));
             }
         hb.sgrid = (uint32_t)ctx_->num_cus * 4;  // segment pass: a grid-stride loop
+        if (pending_.n) {  // the probe side's index, in extra blocks of the filter launch
+            hb.gib = pending_;
+            hb.igrid = pending_.blk0[pending_.n];
+            std::memset(&pending_, 0, sizeof pending_);
+        }
         hb.pcnt = metas + kMetaWords + 2 * kEmitOut;
 
         hp.nf = (int32_t)srel.filters.size();
@@ -961,10 +1061,13 @@ class Executor {
         std::memset(pin, 0, desc_bytes);
         std::memcpy(pin, &hb, bb_bytes);
         std::memcpy(pin + pd_off, &hp, sizeof(PipeDesc));
+        host_trace().mark("planned");
         HJ_HIP(hipMemcpyAsync(ddesc, pin, desc_bytes, hipMemcpyHostToDevice, s_));
+        host_trace().mark("desc_copy");
         const BuildBatch* dbb = (const BuildBatch*)ddesc;
         const PipeDesc* dpd = (const PipeDesc*)(ddesc + pd_off);
         { Prof p_(ctx_, "pipe_build"); pipe_build(hb, dbb, s_); }
+        host_trace().mark("build_q");
         {
             Prof p_(ctx_, "probe_fused", true);
             pipe_probe_fused(hp, dpd, s_, p_.a, p_.b);
@@ -1056,10 +1159,12 @@ class Executor {
                                               std::to_string(kEmitMaxBlocks));
         }
         // one round trip: pipeline counters and the direct output's totals
+        host_trace().mark("emit_q");
         uint8_t* pin = (uint8_t*)ctx_->pinned;
         const size_t bytes = kMetaWords * 4 + kEmitOut * 8;
         HJ_HIP(hipMemcpyAsync(pin, metas, bytes, hipMemcpyDeviceToHost, s_));
-        HJ_HIP(hipStreamSynchronize(s_));
+        wait_stream(s_);
+        host_trace().mark("synced");
         uint32_t m[kMetaWords];
         uint64_t o[kEmitOut];
         std::memcpy(m, pin, sizeof m);
@@ -1775,11 +1880,14 @@ int hj_execute_device(hj_context* ctx, const hj_plan* plan, const hj_dtable* con
     return guarded(ctx, [&] {
         if (ctx->live_result)
             throw Error(HJ_E_BUSY, "free the previous device result before the next execute");
+        host_trace().begin();
         ctx->arena.clear();
         Executor ex(ctx, plan, tables);
         hj_dresult* r = nullptr;
         try {
             r = ex.run();
+            host_trace().mark("done");
+            host_trace().print();
         } catch (...) {
             (void)hipStreamSynchronize(ctx->stream);
             ctx->arena.clear();

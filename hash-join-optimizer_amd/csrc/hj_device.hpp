@@ -11,10 +11,14 @@
 // Every query then reads the pages themselves. Before its first kernel, a query indexes the
 // columns it reads in 64-row groups of row space (k_group_index, in the query's arena):
 //   gidx        u64[2 * groups]      [2g] the validity bits of rows 64g .. 64g + 63 (0 past the
-//                                    last row), [2g + 1] where the group's values are: the page
-//                                    of row 64g, the index of its first row among that page's
-//                                    non-NULL values, and how many of its rows are in that page
-//                                    (the rest open the next page)
+//                                    last row), [2g + 1] where the group's values are, from the
+//                                    page of row 64g, the index of the group's first row among
+//                                    that page's non-NULL values and how many of the group's
+//                                    rows are in that page (`split`; the rest open the next page):
+//                                    INT32: the byte offset A of the group's first value in the
+//                                    column's pages, and the step D from a value index in the
+//                                    first page to the same index in the next (gidx_offset);
+//                                    VARCHAR: page | first index << 32 | split << 48
 // so that 64 consecutive rows are one 16-byte record load and one load of packed values per
 // page they touch. A column with a page of fewer than 64 rows before its last page has no such
 // index: an INT32 column is decoded into a row-space array for the query (flat, vbits), and a
@@ -188,15 +192,28 @@ __device__ inline bool flat_valid(const DCol& c, uint32_t r) {
     return !c.vbits || ((gp(c.vbits)[r >> 6] >> (r & 63u)) & 1ull);
 }
 
+// INT32 group record: A = byte offset of the value of the group's first non-NULL row in its
+// page (page * 8192 + 4 + 4 * first index), E = D | split << 16 with D = 8192 - 4 * (the page's
+// non-NULL values up to the group's first row + those among the group's first split rows):
+// the value of non-NULL row l of the group sits at A + 4 * (valid rows of the group before l),
+// plus D when l >= split (the next page; 32-bit arithmetic wraps to the right offset)
+__host__ __device__ inline uint64_t gint_pack(uint32_t A, uint32_t D, uint32_t split) {
+    return (uint64_t)A | ((uint64_t)(D | (split << 16)) << 32);
+}
+__device__ inline uint32_t gidx_offset(uint64_t vw, uint64_t d, uint32_t l) {
+    const uint32_t E = (uint32_t)(d >> 32);
+    uint32_t off = (uint32_t)d + 4u * (uint32_t)__popcll(vw & ((1ull << l) - 1ull));
+    if (l >= (E >> 16)) off += E & 0xffffu;
+    return off;
+}
+
 // INT32 value of global row r through the group index (one 16-byte record, then the value)
 __device__ inline bool gidx_int32(const uint8_t* pages, const uint64_t* gidx, uint32_t r,
                                   int32_t* v) {
     const u64x2 e = *(const HJ_GLOBAL u64x2*)(gp(gidx) + 2u * (r >> 6));
     const uint32_t l = r & 63u;
     if (!((e.x >> l) & 1ull)) return false;
-    uint32_t page;
-    const uint32_t vi = gloc_value(e.x, e.y, l, &page);
-    *v = *(const HJ_GLOBAL int32_t*)(gp(pages) + (size_t)page * HJ_PAGE_BYTES + 4u + 4u * vi);
+    *v = *(const HJ_GLOBAL int32_t*)(gp(pages) + gidx_offset(e.x, e.y, l));
     return true;
 }
 
@@ -212,12 +229,12 @@ __device__ inline bool get_int32(const DCol& c, uint32_t r, int32_t* v) {
         *v = gp(c.flat)[r];
         return flat_valid(c, r);
     }
-    if (c.gidx) return gidx_int32(c.pages, c.gidx, r, v);
     if (c.umagic) {
         const uint32_t p = dense_page(c.umagic, r);
         *v = *(const HJ_GLOBAL int32_t*)(page_ptr(c, p) + 4u + 4u * (r - p * c.uniform_rows));
         return true;
     }
+    if (c.gidx) return gidx_int32(c.pages, c.gidx, r, v);
     uint32_t p = find_page(c, r);
     const HJ_GLOBAL uint8_t* pg = page_ptr(c, p);
     uint32_t hdr = *(const HJ_GLOBAL uint32_t*)pg;

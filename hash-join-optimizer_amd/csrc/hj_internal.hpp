@@ -186,6 +186,7 @@ struct Prof {
     hj_context* ctx;
     const char* name;
     bool embed;
+    hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
     static hipEvent_t get(hj_context* c) {
         if (!c->event_pool.empty()) {
@@ -200,7 +201,8 @@ struct Prof {
         if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
         return e;
     }
-    Prof(hj_context* c, const char* n, bool embed_events = false) : ctx(c), name(n), embed(embed_events) {
+    Prof(hj_context* c, const char* n, bool embed_events = false, hipStream_t on = nullptr)
+        : ctx(c), name(n), embed(embed_events), st(on ? on : c->stream) {
         if (!ctx->prof_on || std::strncmp(n, ctx->prof_prefix.c_str(), ctx->prof_prefix.size()))
             return;
         a = get(ctx);
@@ -209,11 +211,11 @@ struct Prof {
             a = b = nullptr;
             return;
         }
-        if (!embed) (void)hipEventRecord(a, ctx->stream);
+        if (!embed) (void)hipEventRecord(a, st);
     }
     ~Prof() {
         if (!a || !b) return;
-        if (!embed) (void)hipEventRecord(b, ctx->stream);
+        if (!embed) (void)hipEventRecord(b, st);
         ctx->prof.push_back({name, a, b});
     }
 };

@@ -75,8 +75,9 @@ void launch_rank_index(const DCol& c, uint16_t* rank_out, hipStream_t s);
 void launch_decode_int32(const DCol& c, int32_t* flat, uint64_t* vbits, hipStream_t s);
 // VARCHAR pages -> sref[rows] (kNullRef / kLongRef | page / offset << 16 | length)
 void launch_decode_varchar(const DCol& c, uint64_t* sref, hipStream_t s);
-// group index of up to kIndexCols columns in one launch: out[c] receives 2 words per 64-row
-// group of col[c] (see hj_device.hpp); every page of col[c] but the last holds >= 64 rows
+// group index of up to kIndexCols columns in one launch: out[c] receives 2
+// words per 64-row group of col[c] (see hj_device.hpp); every page of col[c] but the last holds
+// >= 64 rows
 constexpr int kIndexCols = 16;
 struct GroupIndexBatch {
     DCol col[kIndexCols];

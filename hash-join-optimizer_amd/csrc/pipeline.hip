@@ -15,6 +15,7 @@
 // Joins keyed by a column of a build table run afterwards over the survivor list, marking
 // failures, and an ordered compaction writes the output row ids.
 #include "pipeline.hpp"
+#include "gindex.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -26,6 +27,7 @@ namespace {
 
 constexpr int kB = 256;
 constexpr uint32_t kNW = kB / HJ_WAVE;
+static_assert(kGroupIndexBlock == (uint32_t)kB, "index blocks of the filter launch: one group per thread");
 
 __device__ inline uint32_t log2ceil(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
@@ -122,60 +124,45 @@ __device__ inline uint32_t mask_rank(uint64_t m) {
 }
 
 // ---------------------------------------------------------------- columns read by waves
-// An INT32 column read in 64-row groups, straight from its pages: a wave loads the group
-// records it needs (lane u: one 16-byte record of the group index), then, per group, each lane
-// computes where its row's value sits in the packed page (the page of the group's first row for
-// the group's first `split` rows, the next page for the rest; the index among non-NULL values
-// from the record's first value index and a popcount of the validity bits below the lane) and
-// loads it. A dense column of uniform pages needs no index: its record is computed (every row
-// valid, the page by a multiply with the rounded-up reciprocal of the rows per page). A column
-// the query decoded into a row-space array is read from it: its record is the validity word,
-// the value at row * 4.
-enum : uint32_t { kWaveIndexed = 0, kWaveDense = 1, kWaveFlat = 2 };
-struct WaveCol {
-    __amdgpu_buffer_rsrc_t val;  // the pages (or the row-space array)
-    __amdgpu_buffer_rsrc_t rec;  // the group index (or the row-space validity words)
-    uint32_t mode;
-    uint32_t all_valid;          // ~0: a row-space array without validity words
-    uint32_t upr;                // dense: rows per page
-    uint64_t umagic;             // dense: ceil(2^64 / upr)
+// An INT32 column read in 64-row groups straight from its pages through the query's group
+// index: a wave loads the group records it needs (lane u: the 16-byte record of group u), then,
+// per group, each lane finds its row's value at A + 4 * (valid rows of the group before it), plus
+// D for the rows in the next page (A, D and split from the record, wave-uniform), and loads it.
+// The record also holds the group's validity bits. Branch-free: every column the pipeline
+// streams has a group index.
+struct IdxCol {
+    __amdgpu_buffer_rsrc_t val;  // the pages
+    __amdgpu_buffer_rsrc_t rec;  // the group index (empty for a dense column of uniform pages)
+    uint64_t umagic;             // dense, uniform pages: ceil(2^64 / upr), else 0
+    uint32_t upr;
 };
 
-__device__ inline WaveCol wave_col(const DCol& c, uint32_t rows) {
-    const uint32_t G = (rows + 63u) >> 6;
-    WaveCol w;
-    w.all_valid = 0;
-    w.upr = c.uniform_rows;
-    w.umagic = c.umagic;
-    if (c.flat) {
-        w.val = make_rsrc(c.flat, rows * 4u);
-        w.rec = make_rsrc(c.vbits ? (const void*)c.vbits : (const void*)c.flat,
-                          c.vbits ? G * 8u : 0u);
-        w.mode = kWaveFlat;
-        w.all_valid = c.vbits ? 0u : ~0u;
-    } else {
-        w.val = make_rsrc(c.pages, c.num_pages * HJ_PAGE_BYTES);
-        w.rec = make_rsrc(c.gidx, c.gidx ? G * 16u : 0u);
-        w.mode = c.gidx ? kWaveIndexed : kWaveDense;
-    }
-    return w;
+__device__ inline IdxCol idx_col(const DCol& c, uint32_t rows) {
+    return IdxCol{make_rsrc(c.pages, c.num_pages * HJ_PAGE_BYTES),
+                  make_rsrc(c.gidx, c.gidx ? ((rows + 63u) >> 6) * 16u : 0u),
+                  c.gidx ? 0ull : c.umagic, c.uniform_rows};
 }
 
 // the record of group grp (on the lane that loads it): validity bits in .x/.y, location in .z/.w
-__device__ inline u32x4 wave_rec(const WaveCol& w, uint32_t grp) {
-    if (w.mode == kWaveFlat) {
-        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(w.rec, grp * 8u, 0, 0);
-        return u32x4{v.x | w.all_valid, v.y | w.all_valid, 0u, 0u};
-    }
-    if (w.mode == kWaveDense) {
-        const uint32_t r = grp * 64u;
-        const uint32_t p = dense_page(w.umagic, r);
-        const uint32_t local0 = r - p * w.upr;
-        const uint32_t split = w.upr - local0 < 64u ? w.upr - local0 : 64u;
-        const uint64_t d = gloc_pack(p, local0, split);
-        return u32x4{~0u, ~0u, (uint32_t)d, (uint32_t)(d >> 32)};
-    }
+__device__ inline u32x4 idx_rec(const IdxCol& w, uint32_t grp) {
     return __builtin_amdgcn_raw_buffer_load_b128(w.rec, grp * 16u, 0, 0);
+}
+
+// idx_rec for a column that may be dense with pages of one row count (the build side): its
+// record by arithmetic (every row valid; rows past the end are cut by the caller's range). The
+// load is issued either way (it reads nothing from an empty resource) and the choice is a
+// select, so no branch holds the load's wait.
+__device__ inline u32x4 idx_rec_any(const IdxCol& w, uint32_t grp) {
+    const u32x4 ld = idx_rec(w, grp);
+    const uint32_t r = grp * 64u;
+    const uint32_t p = dense_page(w.umagic, r);
+    const uint32_t local0 = r - p * w.upr;
+    const uint32_t split = w.upr - local0 < 64u ? w.upr - local0 : 64u;
+    const uint64_t d = gint_pack(p * HJ_PAGE_BYTES + 4u + 4u * local0,
+                                 HJ_PAGE_BYTES - 4u * (local0 + split), split);
+    const bool dense = w.umagic != 0;
+    return u32x4{dense ? ~0u : ld.x, dense ? ~0u : ld.y, dense ? (uint32_t)d : ld.z,
+                 dense ? (uint32_t)(d >> 32) : ld.w};
 }
 
 // group u's validity bits and location (wave-uniform) from the records held by lanes 0 .. U-1
@@ -183,24 +170,30 @@ __device__ inline uint64_t rec_valid(const u32x4& r, int u) {
     return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)r.y, u) << 32) |
            (uint32_t)__builtin_amdgcn_readlane((int)r.x, u);
 }
-__device__ inline uint64_t rec_loc(const u32x4& r, int u) {
-    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)r.w, u) << 32) |
-           (uint32_t)__builtin_amdgcn_readlane((int)r.z, u);
+
+// this lane's value of a group whose record lanes 0 .. U-1 hold (group u); its validity bits vw
+__device__ inline int32_t idx_load(const IdxCol& w, const u32x4& r, int u, uint64_t vw,
+                                   uint32_t lane) {
+    const uint32_t A = (uint32_t)__builtin_amdgcn_readlane((int)r.z, u);
+    const uint32_t E = (uint32_t)__builtin_amdgcn_readlane((int)r.w, u);
+    // the whole offset in the per-lane operand, which the buffer's range check covers: a row
+    // past the last page reads 0
+    const uint32_t off = A + 4u * mask_rank(vw) + (lane >= (E >> 16) ? (E & 0xffffu) : 0u);
+    return __builtin_amdgcn_raw_buffer_load_b32(w.val, off, 0, 0);
 }
 
-// this lane's byte offset in w.val for row 64 * grp + lane, given the group's validity bits vw
-// and location d (wave-uniform)
-__device__ inline uint32_t wave_offset(const WaveCol& w, uint32_t grp, uint64_t vw, uint64_t d,
-                                       uint32_t lane) {
-    if (w.mode == kWaveFlat) return (grp * 64u + lane) * 4u;
-    const uint32_t split = (uint32_t)(d >> 48) & 0xffu;
-    const uint32_t below = mask_rank(vw);  // valid rows of the group before this lane
-    const uint32_t in_first = split >= 64u ? (uint32_t)__popcll(vw)
-                                           : (uint32_t)__popcll(vw & ((1ull << split) - 1ull));
-    const bool first = lane < split;
-    const uint32_t vi = first ? ((uint32_t)(d >> 32) & 0xffffu) + below : below - in_first;
-    const uint32_t page = (uint32_t)d + (first ? 0u : 1u);
-    return page * HJ_PAGE_BYTES + 4u + 4u * vi;
+// A column of the query's row-space arrays (a predicate's results, a decoded column): the value
+// of row r at r * 4, validity words beside it (none: every row valid)
+struct FlatCol {
+    __amdgpu_buffer_rsrc_t val;
+    __amdgpu_buffer_rsrc_t vb;
+    uint32_t all_valid;
+};
+__device__ inline FlatCol flat_col(const DCol& c, uint32_t rows) {
+    return FlatCol{make_rsrc(c.flat, rows * 4u),
+                   make_rsrc(c.vbits ? (const void*)c.vbits : (const void*)c.flat,
+                             c.vbits ? ((rows + 63u) >> 6) * 8u : 0u),
+                   c.vbits ? 0u : ~0u};
 }
 
 // the lanes of m run what the condition guards (m is wave-uniform)
@@ -252,6 +245,10 @@ __device__ inline int build_job_of(const BuildBatch* bb, uint32_t b) {
 
 // filter pass: rows with a non-NULL key that pass the scan's range filters -> (key, row)
 __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__ bb) {
+    if (blockIdx.x >= bb->grid) {  // the probe side's group index
+        group_index_thread(bb->gib, blockIdx.x - bb->grid, threadIdx.x);
+        return;
+    }
     __shared__ uint32_t s_cnt;
     const uint32_t t = threadIdx.x, lane = lane_id();
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
@@ -262,35 +259,45 @@ __global__ __launch_bounds__(kB) void k_sb_filter(const BuildBatch* __restrict__
     const uint32_t r1 = r0 + kBuildRows < rows ? r0 + kBuildRows : rows;
     if (t == 0) s_cnt = 0;
     __syncthreads();
-    const WaveCol wk = wave_col(jb.key, rows);
+    const IdxCol wk = idx_col(jb.key, rows);
     const int nf = jb.nf;
     // lane u < kFU reads the record of group u of the wave's rows
     const uint32_t lu = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 4u;
     for (uint32_t base = r0; base < r1; base += kB * kFU) {
         // wave w takes groups base/64 + 4u + w, u < kFU
         const uint32_t grp0 = base / 64u + w;
-        const u32x4 kr = wave_rec(wk, grp0 + lu);
+        const u32x4 kr = idx_rec_any(wk, grp0 + lu);
         int32_t key[kFU];
         uint64_t live[kFU];
 #pragma unroll
         for (int u = 0; u < kFU; ++u) {
             const uint64_t vw = rec_valid(kr, u);
-            key[u] = __builtin_amdgcn_raw_buffer_load_b32(
-                wk.val, wave_offset(wk, grp0 + 4u * u, vw, rec_loc(kr, u), lane), 0, 0);
+            key[u] = idx_load(wk, kr, u, vw, lane);
             const uint32_t g0 = base + u * 256u + w * 64u;
             live[u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) & vw;
         }
         for (int f = 0; f < nf; ++f) {
-            const WaveCol wf = wave_col(jb.fcol[f], rows);
+            const DCol& fc = jb.fcol[f];
             const int32_t lo = jb.flo[f], hi = jb.fhi[f];
-            const u32x4 fr = wave_rec(wf, grp0 + lu);
             int32_t v[kFU];
             uint64_t fvw[kFU];
+            if (fc.flat) {
+                const FlatCol ff = flat_col(fc, rows);
+                const u32x2 vv = __builtin_amdgcn_raw_buffer_load_b64(ff.vb, (grp0 + lu) * 8u, 0, 0);
 #pragma unroll
-            for (int u = 0; u < kFU; ++u) {
-                fvw[u] = rec_valid(fr, u);
-                v[u] = __builtin_amdgcn_raw_buffer_load_b32(
-                    wf.val, wave_offset(wf, grp0 + 4u * u, fvw[u], rec_loc(fr, u), lane), 0, 0);
+                for (int u = 0; u < kFU; ++u) {
+                    v[u] = __builtin_amdgcn_raw_buffer_load_b32(
+                        ff.val, ((grp0 + 4u * u) * 64u + lane) * 4u, 0, 0);
+                    fvw[u] = lane_word(vv.x | ff.all_valid, vv.y | ff.all_valid, u);
+                }
+            } else {
+                const IdxCol wf = idx_col(fc, rows);
+                const u32x4 fr = idx_rec_any(wf, grp0 + lu);
+#pragma unroll
+                for (int u = 0; u < kFU; ++u) {
+                    fvw[u] = rec_valid(fr, u);
+                    v[u] = idx_load(wf, fr, u, fvw[u], lane);
+                }
             }
 #pragma unroll
             for (int u = 0; u < kFU; ++u) live[u] &= __ballot(v[u] >= lo && v[u] <= hi) & fvw[u];
@@ -318,6 +325,10 @@ constexpr int kFR = (int)(kBuildRows / (kB * kFU));  // rounds per block
 static_assert(kFR * kB * kFU == (int)kBuildRows, "build rows per block");
 
 __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restrict__ bb) {
+    if (blockIdx.x >= bb->grid) {  // the probe side's group index
+        group_index_thread(bb->gib, blockIdx.x - bb->grid, threadIdx.x);
+        return;
+    }
     __shared__ uint32_t hist[(1u << kPartBits) + 1];
     __shared__ uint32_t lds[kNW];
     const uint32_t t = threadIdx.x, lane = lane_id();
@@ -333,74 +344,83 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
         for (uint32_t i = t; i < kBloomWords; i += kB) gp(jb.bf)[i] = 0;
     for (uint32_t i = t; i <= P; i += kB) hist[i] = 0;
     __syncthreads();
-    const WaveCol wk = wave_col(jb.key, rows);
+    const IdxCol wk = idx_col(jb.key, rows);
     const int nf = jb.nf;
+    // the first filter streams with the keys when it has a group index
+    const int f0 = nf && !jb.fcol[0].flat ? 1 : 0;
     const uint32_t lu = (lane < (uint32_t)kFU ? lane : (uint32_t)kFU - 1u) * 4u;
     int32_t key[kFR][kFU];
     uint32_t rank[kFR][kFU];
     uint32_t alive = 0;  // bit r * kFU + u: this lane's row of round r, group u passes
     // every round's group records at once (key column and first filter column), then every
-    // round's keys and filter values, then the other filters' columns the same way
+    // round's keys and filter values, then the other filters' columns
     uint64_t live[kFR][kFU];
     {
-        const WaveCol wf = nf ? wave_col(jb.fcol[0], rows) : wk;
-        const int32_t lo = nf ? jb.flo[0] : INT32_MIN, hi = nf ? jb.fhi[0] : INT32_MAX;
+        const IdxCol wf = idx_col(f0 ? jb.fcol[0] : jb.key, rows);
+        const int32_t lo = f0 ? jb.flo[0] : INT32_MIN, hi = f0 ? jb.fhi[0] : INT32_MAX;
         u32x4 kr[kFR], fr[kFR];
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd) {
             const uint32_t grp0 = (r0 + rd * (kB * kFU)) / 64u + w;
-            kr[rd] = wave_rec(wk, grp0 + lu);
-            if (nf) fr[rd] = wave_rec(wf, grp0 + lu);
+            kr[rd] = idx_rec_any(wk, grp0 + lu);
+            fr[rd] = idx_rec_any(wf, grp0 + lu);
         }
         int32_t v[kFR][kFU];
+        uint64_t kvw[kFR][kFU], fvw[kFR][kFU];
 #pragma unroll
-        for (int rd = 0; rd < kFR; ++rd) {
-            const uint32_t grp0 = (r0 + rd * (kB * kFU)) / 64u + w;
+        for (int rd = 0; rd < kFR; ++rd)
 #pragma unroll
             for (int u = 0; u < kFU; ++u) {
-                const uint32_t g = grp0 + 4u * u;
-                key[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(
-                    wk.val, wave_offset(wk, g, rec_valid(kr[rd], u), rec_loc(kr[rd], u), lane), 0, 0);
-                v[rd][u] = nf ? __builtin_amdgcn_raw_buffer_load_b32(
-                                    wf.val,
-                                    wave_offset(wf, g, rec_valid(fr[rd], u), rec_loc(fr[rd], u), lane),
-                                    0, 0)
-                              : 0;
+                kvw[rd][u] = rec_valid(kr[rd], u);
+                fvw[rd][u] = rec_valid(fr[rd], u);
+                key[rd][u] = idx_load(wk, kr[rd], u, kvw[rd][u], lane);
+                v[rd][u] = idx_load(wf, fr[rd], u, fvw[rd][u], lane);
             }
-        }
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd)
 #pragma unroll
             for (int u = 0; u < kFU; ++u) {
                 const uint32_t g0 = r0 + rd * (kB * kFU) + u * 256u + w * 64u;
-                live[rd][u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) & rec_valid(kr[rd], u) &
-                              (nf ? __ballot(v[rd][u] >= lo && v[rd][u] <= hi) &
-                                        rec_valid(fr[rd], u)
-                                  : ~0ull);
+                live[rd][u] = (g0 < r1 ? range_mask(r1 - g0) : 0ull) & kvw[rd][u] & fvw[rd][u] &
+                              __ballot(v[rd][u] >= lo && v[rd][u] <= hi);
             }
     }
-    for (int f = 1; f < nf; ++f) {
-        const WaveCol wf = wave_col(jb.fcol[f], rows);
+    for (int f = f0; f < nf; ++f) {
+        const DCol& fc = jb.fcol[f];
         const int32_t lo = jb.flo[f], hi = jb.fhi[f];
-        u32x4 fr[kFR];
-#pragma unroll
-        for (int rd = 0; rd < kFR; ++rd) fr[rd] = wave_rec(wf, (r0 + rd * (kB * kFU)) / 64u + w + lu);
         int32_t v[kFR][kFU];
+        uint64_t fvw[kFR][kFU];
+        if (fc.flat) {
+            const FlatCol ff = flat_col(fc, rows);
 #pragma unroll
-        for (int rd = 0; rd < kFR; ++rd) {
-            const uint32_t grp0 = (r0 + rd * (kB * kFU)) / 64u + w;
+            for (int rd = 0; rd < kFR; ++rd) {
+                const uint32_t grp0 = (r0 + rd * (kB * kFU)) / 64u + w;
+                const u32x2 vv = __builtin_amdgcn_raw_buffer_load_b64(ff.vb, (grp0 + lu) * 8u, 0, 0);
 #pragma unroll
-            for (int u = 0; u < kFU; ++u)
-                v[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(
-                    wf.val,
-                    wave_offset(wf, grp0 + 4u * u, rec_valid(fr[rd], u), rec_loc(fr[rd], u), lane),
-                    0, 0);
+                for (int u = 0; u < kFU; ++u) {
+                    v[rd][u] = __builtin_amdgcn_raw_buffer_load_b32(
+                        ff.val, ((grp0 + 4u * u) * 64u + lane) * 4u, 0, 0);
+                    fvw[rd][u] = lane_word(vv.x | ff.all_valid, vv.y | ff.all_valid, u);
+                }
+            }
+        } else {
+            const IdxCol wf = idx_col(fc, rows);
+            u32x4 fr[kFR];
+#pragma unroll
+            for (int rd = 0; rd < kFR; ++rd) fr[rd] = idx_rec_any(wf, (r0 + rd * (kB * kFU)) / 64u + w + lu);
+#pragma unroll
+            for (int rd = 0; rd < kFR; ++rd)
+#pragma unroll
+                for (int u = 0; u < kFU; ++u) {
+                    fvw[rd][u] = rec_valid(fr[rd], u);
+                    v[rd][u] = idx_load(wf, fr[rd], u, fvw[rd][u], lane);
+                }
         }
 #pragma unroll
         for (int rd = 0; rd < kFR; ++rd)
 #pragma unroll
             for (int u = 0; u < kFU; ++u)
-                live[rd][u] &= __ballot(v[rd][u] >= lo && v[rd][u] <= hi) & rec_valid(fr[rd], u);
+                live[rd][u] &= __ballot(v[rd][u] >= lo && v[rd][u] <= hi) & fvw[rd][u];
     }
 #pragma unroll
     for (int rd = 0; rd < kFR; ++rd)
@@ -851,13 +871,13 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     __syncthreads();
     const uint32_t nsrc = pd.src_rows;
     const uint32_t G = (nsrc + 63u) >> 6;
-    WaveCol wc[NS];  // the key columns, read from their pages
+    IdxCol wc[NS];  // the key columns, read from their pages
     const HJ_GLOBAL uint32_t* tkey[NS];
     uint32_t tk[NS], tsp[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
         const StreamJoin& sj = pd.sj[q];
-        wc[q] = wave_col(sj.key, nsrc);
+        wc[q] = idx_col(sj.key, nsrc);
         tkey[q] = gp(sj.table);
         tk[q] = table_bits(gp(sj.meta)[0]);
         tsp[q] = gp(sj.meta)[2];
@@ -880,18 +900,14 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     u32x4 ra[NS], rb[NS];
     auto fetch_rec = [&](u32x4 (&rx)[NS], uint32_t gx) {
 #pragma unroll
-        for (int q = 0; q < NS; ++q) rx[q] = wave_rec(wc[q], gx * U + lu);
+        for (int q = 0; q < NS; ++q) rx[q] = idx_rec(wc[q], gx * U + lu);
     };
-    auto fetch = [&](int32_t (&kx)[NS][U], u32x2 (&vx)[NS], const u32x4 (&rx)[NS], uint32_t gx) {
+    auto fetch = [&](int32_t (&kx)[NS][U], u32x2 (&vx)[NS], const u32x4 (&rx)[NS]) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
             vx[q] = u32x2{rx[q].x, rx[q].y};
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                kx[q][u] = __builtin_amdgcn_raw_buffer_load_b32(
-                    wc[q].val,
-                    wave_offset(wc[q], gx * U + u, rec_valid(rx[q], u), rec_loc(rx[q], u), lane),
-                    0, 0);
+            for (int u = 0; u < U; ++u) kx[q][u] = idx_load(wc[q], rx[q], u, rec_valid(rx[q], u), lane);
         }
     };
     // one iteration: the U groups of wave-iteration gx, whose keys and validity words are in
@@ -1001,15 +1017,15 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     uint32_t gb = g_lo / U + w;
     fetch_rec(ra, gb);
     fetch_rec(rb, gb + kNW);
-    fetch(ka, va, ra, gb);
+    fetch(ka, va, ra);
     while (gb * U < g_hi) {
         fetch_rec(ra, gb + 2u * kNW);
-        fetch(kb, vb, rb, gb + kNW);
+        fetch(kb, vb, rb);
         step(ka, va, gb);
         gb += kNW;
         if (gb * U >= g_hi) break;
         fetch_rec(rb, gb + 2u * kNW);
-        fetch(ka, va, ra, gb + kNW);
+        fetch(ka, va, ra);
         step(kb, vb, gb);
         gb += kNW;
     }
@@ -1219,11 +1235,11 @@ uint32_t probe_resident_blocks(int ns, uint32_t num_cus) {
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s) {
     if (hb.njobs == 0) return;
     if (hb.part) {
-        hipLaunchKernelGGL(k_sb_filter_part, dim3(hb.grid), dim3(kB), 0, s, db);
+        hipLaunchKernelGGL(k_sb_filter_part, dim3(hb.grid + hb.igrid), dim3(kB), 0, s, db);
         hipLaunchKernelGGL(k_sb_segbuild, dim3(hb.sgrid), dim3(kB), 0, s, db);
         return;
     }
-    hipLaunchKernelGGL(k_sb_filter, dim3(hb.grid), dim3(kB), 0, s, db);
+    hipLaunchKernelGGL(k_sb_filter, dim3(hb.grid + hb.igrid), dim3(kB), 0, s, db);
     hipLaunchKernelGGL(k_sb_clear, dim3(hb.cgrid, hb.njobs), dim3(kB), 0, s, db);
     hipLaunchKernelGGL(k_sb_insert, dim3(hb.grid), dim3(kB), 0, s, db);
 }

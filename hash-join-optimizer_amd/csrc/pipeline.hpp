@@ -96,6 +96,10 @@ struct BuildBatch {
     int32_t njobs;
     uint32_t grid;      // blocks of the filter and insert passes (every job's nblk)
     uint32_t cgrid;     // blocks per job of the clear pass (grid-stride)
+    // the probe side's group index, built by igrid extra blocks of the filter pass (the builds
+    // do not read those columns; the probe launch comes after the build's last pass)
+    GroupIndexBatch gib;
+    uint32_t igrid;
 };
 
 struct PipeStage {
@@ -153,7 +157,7 @@ struct PipeDesc {
 };
 
 // 64-row groups per wave and iteration of the fused pass over NS streamed joins
-constexpr int probe_unroll(int ns) { return ns == 1 ? 8 : (ns == 2 ? 6 : (ns <= 4 ? 4 : 2)); }
+constexpr int probe_unroll(int ns) { return ns == 1 ? 4 : (ns == 2 ? 3 : (ns <= 4 ? 2 : 2)); }
 
 // blocks of the fused pass over ns streamed joins that are resident at once on the device (the
 // pass gives each a contiguous share of the source, so a second round of blocks would double
