@@ -157,52 +157,10 @@ def serve_workload(codec, args):
 
 
 def batch_workload(codec, args):
-    """256 independent 2- and 3-way INT32 joins: build sides of unique keys, sizes log-uniform
-    in [1K, 4M] x scale; probe sides log-uniform in [10K, 40M] x scale with FKs uniform or Zipf
-    (s = 0.8 or 1.2) over the build keys, match rate ~ U[0.1, 1.0] (other FKs miss every build),
-    5 % NULL FKs; output the payloads of every side"""
-    rng = np.random.default_rng(48)
-    I32 = hjo.DataType.INT32
-    sc = args.scale
-    plans = []
+    """the batch of independent 2- and 3-way INT32 joins (datagen.batch_joins, seed 48): builds
+    log-uniform in [1K, 4M] x scale rows, probes log-uniform in [10K, 40M] x scale"""
     t = time.time()
-    for q in range(args.queries):
-        ways = int(rng.integers(2, 4))
-        nb = [int(np.exp(rng.uniform(np.log(1_000 * sc + 1), np.log(4_000_000 * sc + 1))))
-              for _ in range(ways - 1)]
-        npr = int(np.exp(rng.uniform(np.log(10_000 * sc + 1), np.log(40_000_000 * sc + 1))))
-        plan = hjo.Plan()
-        fk_cols, fk_valid = [], []
-        for n in nb:
-            m = rng.uniform(0.1, 1.0)
-            dist = rng.choice(["uniform", "zipf0.8", "zipf1.2"])
-            if dist == "uniform":
-                fk = rng.integers(0, n, npr)
-            else:
-                fk, _ = dg.zipf_keys(rng, n, 0.8 if dist == "zipf0.8" else 1.2, npr)
-            miss = rng.random(npr) >= m
-            fk = np.where(miss, n + rng.integers(0, n + 1, npr), fk).astype(np.int32)
-            fk_cols.append(fk)
-            fk_valid.append((rng.random(npr) >= 0.05).astype(np.uint8))
-        probe_cols = [codec.encode_int32(fk, v) for fk, v in zip(fk_cols, fk_valid)]
-        probe_cols.append(codec.encode_int32(rng.integers(0, 2**31, npr).astype(np.int32)))
-        plan.inputs.append(hjo.ColumnarTable(npr, probe_cols))
-        for n in nb:
-            plan.inputs.append(hjo.ColumnarTable(n, [
-                codec.encode_int32(rng.permutation(n).astype(np.int32)),
-                codec.encode_int32(rng.integers(0, 2**31, n).astype(np.int32))]))
-        nk = len(nb)
-        cur = plan.new_scan_node(0, [(j, I32) for j in range(nk + 1)])
-        width = nk + 1
-        for j in range(nk):
-            b = plan.new_scan_node(1 + j, [(0, I32), (1, I32)])
-            # the probe side's attributes: the j build payloads prepended so far, the FKs, its
-            # payload; FK j is attribute 2j
-            cur = plan.new_join_node(True, b, cur, 0, 2 * j,
-                                     [(1, I32)] + [(2 + i, I32) for i in range(width)])
-            width += 1
-        plan.root = cur
-        plans.append(plan)
+    plans = dg.batch_joins(seed=48, n=args.queries, scale=args.scale, codec=codec)
     log(f"batch of {len(plans)} joins generated+encoded in {time.time() - t:.1f}s")
     return plans
 

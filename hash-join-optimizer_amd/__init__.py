@@ -7,8 +7,8 @@ from . import abi  # noqa: F401
 from .abi import EngineMissing  # noqa: F401
 from .plan import (CacheError, Codec, Column, ColumnarTable, DataType, JoinNode,  # noqa: F401
                    Plan, PlanNode, ScanNode, Table, read_cache, sorted_rows, write_cache)
-from .engine import (Context, DeviceResult, DeviceTable, Directory, EngineError,  # noqa: F401
-                     Server)
+from .engine import (Batch, Context, DeviceResult, DeviceTable, Directory,  # noqa: F401
+                     EngineError, Server)
 
 from .build import hash_algo as _hash_algo
 

@@ -74,6 +74,14 @@ class hj_result(C.Structure):
                 ("reserved", C.c_uint32), ("columns", C.POINTER(hj_out_column))]
 
 
+class hj_batch_params(C.Structure):
+    _fields_ = [("probe_item_groups", C.c_uint32), ("insert_item_rows", C.c_uint32),
+                ("radix_bits", C.c_uint32), ("load_shift", C.c_uint32),
+                ("large_build_rows", C.c_uint32), ("large_load_shift", C.c_uint32),
+                ("unroll", C.c_uint32), ("coop", C.c_uint32), ("use_graph", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+
 P = C.POINTER
 _SIGS = {
     "hj_context_create": (C.c_int, [C.c_uint64, P(C.c_void_p)]),
@@ -124,6 +132,15 @@ _SIGS = {
                                  P(C.c_uint64)]),
     "hj_server_slots": (C.c_uint32, [C.c_void_p]),
     "hj_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "hj_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, P(P(hj_plan)), P(P(C.c_void_p)),
+                                  P(hj_batch_params), P(C.c_void_p)]),
+    "hj_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hj_batch_size": (C.c_uint32, [C.c_void_p]),
+    "hj_batch_info": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_int32), P(C.c_uint64)]),
+    "hj_batch_result": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p)]),
+    "hj_batch_result_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(hj_result)]),
+    "hj_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "hj_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hj_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "hj_profile_enable": (C.c_int, [C.c_void_p, C.c_char_p]),
     "hj_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64]),

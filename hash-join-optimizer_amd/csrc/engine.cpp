@@ -1920,7 +1920,7 @@ int hj_dresult_to_host(hj_context* ctx, const hj_dresult* r, hj_result* result) 
 }
 
 int hj_dresult_free(hj_context* ctx, hj_dresult* r) {
-    if (!r) return HJ_E_INVALID;
+    if (!r || r->batch_view) return HJ_E_INVALID;  // a batch's results go with the batch
     // no wait: the result lives in the context's arena, which the next query reuses in stream
     // order, after this query's kernels
     free_dresult(ctx, r);

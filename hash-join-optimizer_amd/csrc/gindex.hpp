@@ -14,11 +14,7 @@ constexpr uint32_t kGroupIndexBlock = 256;
 // page's group ranks. Reads the bitmaps once (1 bit per row) and writes 16 bytes per 64 rows.
 // (One wave per page, with the first value indexes from a wave prefix sum, measured slower:
 // a page holds 3-33 groups, so most lanes idle and the launch needs 90K short waves.)
-__device__ inline void group_index_thread(const GroupIndexBatch& b, uint32_t blk, uint32_t tid) {
-    int ci = 0;
-    for (int i = 1; i < b.n; ++i) ci += blk >= b.blk0[i] ? 1 : 0;
-    const DCol& c = b.col[ci];
-    const uint32_t g = (blk - b.blk0[ci]) * kGroupIndexBlock + tid;
+__device__ inline void group_index_group(const DCol& c, uint64_t* gout, uint32_t g) {
     const uint64_t rows = c.num_rows;
     if ((uint64_t)g * 64u >= rows) return;
     const uint32_t r = g * 64u;
@@ -57,7 +53,7 @@ __device__ inline void group_index_thread(const GroupIndexBatch& b, uint32_t blk
                                 : group_bits(pq, ((hq & 0xffffu) + 7u) >> 3, 0) & low_avail;
         bits |= bq << split;
     }
-    HJ_GLOBAL uint64_t* out = gp(b.out[ci]) + 2u * (uint64_t)g;
+    HJ_GLOBAL uint64_t* out = gp(gout) + 2u * (uint64_t)g;
     out[0] = bits;
     if (c.type == 0) {  // INT32: byte offsets (the engine keeps such columns' pages < 4 GiB)
         const uint32_t in_first = (uint32_t)__popcll(bits & low_split);
@@ -66,6 +62,12 @@ __device__ inline void group_index_thread(const GroupIndexBatch& b, uint32_t blk
     } else {
         out[1] = gloc_pack(p, rank0, split);
     }
+}
+
+__device__ inline void group_index_thread(const GroupIndexBatch& b, uint32_t blk, uint32_t tid) {
+    int ci = 0;
+    for (int i = 1; i < b.n; ++i) ci += blk >= b.blk0[i] ? 1 : 0;
+    group_index_group(b.col[ci], b.out[ci], (blk - b.blk0[ci]) * kGroupIndexBlock + tid);
 }
 
 }  // namespace hj

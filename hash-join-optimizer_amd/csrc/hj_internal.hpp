@@ -146,6 +146,7 @@ struct hj_dresult {
     };
     std::vector<Col> cols;
     bool live = false;
+    bool batch_view = false;  // columns owned by an hj_batch (not freed with the result)
 };
 
 struct hj_context {

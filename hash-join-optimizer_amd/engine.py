@@ -259,3 +259,71 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Batch:
+    """Many independent queries over resident tables in one persistent launch (hj_batch_*).
+
+    queries: list of (plan, dtables). params: None (the engine's defaults), a dict of
+    hj_batch_params fields, or "tuned" (the committed autotuner table, tuning/batch_mi355x.json).
+    run() executes every query; result(i) returns query i's rows as a ColumnarTable;
+    info(i) -> (ran in the persistent launch, rows)."""
+
+    def __init__(self, ctx, queries, params=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        if params == "tuned":
+            from .autotune import tuned_params
+            params = tuned_params(queries)
+        self.params = dict(params or {})
+        self._keep = []
+        n = len(queries)
+        plans = (C.POINTER(abi.hj_plan) * max(1, n))()
+        tabs = (C.POINTER(C.c_void_p) * max(1, n))()
+        for i, (plan, dts) in enumerate(queries):
+            p, keep = plan.to_c(with_inputs=False)
+            arr = (C.c_void_p * max(1, len(dts)))(*[t.handle for t in dts])
+            self._keep.append((p, keep, arr))
+            plans[i] = C.pointer(p)
+            tabs[i] = C.cast(arr, C.POINTER(C.c_void_p))
+        self._keep.append((plans, tabs))
+        prm = None
+        if params is not None:
+            prm = abi.hj_batch_params()
+            prm.use_graph = 1
+            for k, v in self.params.items():
+                setattr(prm, k, int(v))
+        h = C.c_void_p()
+        ctx._check(self.lib.hj_batch_create(ctx.handle, n, plans, tabs,
+                                            C.byref(prm) if prm is not None else None,
+                                            C.byref(h)))
+        self.handle = h
+        self.n = n
+
+    def run(self):
+        self.ctx._check(self.lib.hj_batch_run(self.ctx.handle, self.handle))
+
+    def info(self, i):
+        b, r = C.c_int32(), C.c_uint64()
+        self.ctx._check(self.lib.hj_batch_info(self.handle, i, C.byref(b), C.byref(r)))
+        return bool(b.value), int(r.value)
+
+    def device_bytes(self):
+        return int(self.lib.hj_batch_device_bytes(self.handle))
+
+    def result(self, i):
+        res = abi.hj_result()
+        self.ctx._check(self.lib.hj_batch_result_to_host(self.ctx.handle, self.handle, i,
+                                                         C.byref(res)))
+        return result_from_c(res, self.lib.hj_result_free)
+
+    def free(self):
+        if self.handle:
+            self.lib.hj_batch_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()

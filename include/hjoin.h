@@ -247,6 +247,52 @@ int hj_server_wait(hj_server* server, uint64_t ticket, hj_result* result, uint64
 uint32_t hj_server_slots(const hj_server* server);
 const char* hj_server_last_error(hj_server* server);
 
+/* ---- batched mode (no counterpart in the reference, whose execute() runs one plan at a time and
+ * spreads one join's probe over threads by fetch_add on a 4096-row chunk counter,
+ * src/execute.cpp:87-117): many independent queries over resident tables run in ONE persistent
+ * launch. Their work (group indexes of the columns they read, build inserts, probe chunks) is one
+ * list of items that the launch's workgroups take in order from a device work queue; a probe item
+ * waits until its query's inserts are done. The launch is captured once as a HIP graph and
+ * replayed by every hj_batch_run. A query is batched when it is a left-deep chain whose build
+ * sides are base scans keyed by INT32 columns of the chain's source scan (a star join), with
+ * range filters at most and INT32 outputs; every other query (and a batched one whose build keys
+ * repeat) runs on its own through hj_execute_device after the launch. Results are the same either
+ * way. Tables must stay resident and unchanged while the batch exists. ---- */
+typedef struct hj_batch hj_batch;
+typedef struct hj_batch_params {
+    uint32_t probe_item_groups; /* 64-row groups of a query's source per probe item (0: 256) */
+    uint32_t insert_item_rows;  /* build rows per insert item, a multiple of 1024 (0: 8192) */
+    uint32_t radix_bits;        /* 0-12: each 1024-tuple round of an insert item is ordered by this
+                                   many top hash bits before its inserts (0: unordered) */
+    uint32_t load_shift;        /* a build of n rows gets 2^k >= n << load_shift key slots
+                                   (0 or 1: load <= 1/2; 2: <= 1/4; 3: <= 1/8) */
+    uint32_t large_build_rows;  /* builds of at least this many rows use large_load_shift (0: none) */
+    uint32_t large_load_shift;
+    uint32_t unroll;            /* 2 or 4 (0: 4): 64-row groups per wave and round of a probe item */
+    uint32_t coop;              /* probe lookups: 0 one lane per probe row, 1 four lanes per row */
+    uint32_t use_graph;         /* 1: the launch is a HIP graph instantiated on the first run */
+    uint32_t reserved[3];
+} hj_batch_params;
+/* plans[i] runs over tables[i][0 .. plans[i]->num_inputs); plans and tables stay owned by the
+ * caller and must outlive the batch. params NULL: the defaults above with use_graph = 1. */
+int hj_batch_create(hj_context* ctx, uint32_t n, const hj_plan* const* plans,
+                    const hj_dtable* const* const* tables, const hj_batch_params* params,
+                    hj_batch** out);
+/* run every query of the batch (the persistent launch, then the queries that run on their own);
+ * returns when every result is complete. Results of the previous run are replaced. */
+int hj_batch_run(hj_context* ctx, hj_batch* b);
+uint32_t hj_batch_size(const hj_batch* b);
+/* after a run: whether query i ran in the persistent launch, and its result rows */
+int hj_batch_info(const hj_batch* b, uint32_t i, int32_t* batched, uint64_t* rows);
+/* the device result of batched query i (owned by the batch, valid until its next run or
+ * hj_batch_free; hj_dresult_free refuses it); HJ_E_INVALID for a query that ran on its own */
+int hj_batch_result(const hj_batch* b, uint32_t i, const hj_dresult** out);
+/* the result of query i as pages in host memory (free with hj_result_free); a query that ran on
+ * its own hands over its result once */
+int hj_batch_result_to_host(hj_context* ctx, hj_batch* b, uint32_t i, hj_result* out);
+uint64_t hj_batch_device_bytes(const hj_batch* b);
+int hj_batch_free(hj_context* ctx, hj_batch* b);
+
 /* ---- engine options (no counterpart in the reference):
  *   "pipeline" 1 (default) / 0: run left-deep chains whose build sides are base scans as one
  *              fused probe pipeline; 0 forces the join-at-a-time path for every plan.
@@ -273,7 +319,6 @@ int hj_directory_build(hj_context* ctx, const uint32_t* keys, uint64_t n, hj_dir
  * them (UINT64_MAX when count == 0); row ids are positions in the build key array */
 int hj_directory_probe(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
                        uint64_t m, uint64_t* counts, uint64_t* first_row);
-/* all matches of probe key i as (probe index, build row id) pairs, sorted by probe index */
 /* No counterpart in the reference: hj_directory_probe for callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops or harnesses that forbid hipMalloc). It
  * allocates nothing: the probe keys, counts and first rows are staged in one fixed device buffer
@@ -286,6 +331,7 @@ int hj_directory_probe(hj_context* ctx, const hj_directory* dir, const uint32_t*
 int hj_directory_probe_noalloc(hj_context* ctx, const hj_directory* dir,
                                const uint32_t* probe_keys, uint64_t m, uint64_t* counts,
                                uint64_t* first_row);
+/* all matches of probe key i as (probe index, build row id) pairs, sorted by probe index */
 int hj_directory_probe_pairs(hj_context* ctx, const hj_directory* dir, const uint32_t* probe_keys,
                              uint64_t m, uint64_t** pairs, uint64_t* num_pairs);
 int hj_directory_info(const hj_directory* dir, uint64_t* num_slots, uint64_t* num_tuples,
