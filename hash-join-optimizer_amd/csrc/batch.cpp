@@ -32,6 +32,7 @@ constexpr uint32_t kMaxQueries = 1u << 20;
 
 struct QueryPlan {
     bool batched = false;
+    bool launch = false;  // runs in the launches (else on its own, copied into its outputs)
     BatchQuery q{};
     // host-side sizes for the allocation
     uint64_t src_rows = 0;
@@ -106,6 +107,7 @@ bool plan_query(const hj_plan* p, const hj_dtable* const* tables, QueryPlan& qp)
         q.sfhi[f] = sn.filters[f].hi;
     }
     q.nj = (int32_t)joins.size();
+    std::vector<Slot> keys;
     for (size_t i = 0; i < joins.size(); ++i) {
         const hj_node& jn = p->nodes[joins[i]];
         const bool bl = jn.build_left != 0;
@@ -121,6 +123,8 @@ bool plan_query(const hj_plan* p, const hj_dtable* const* tables, QueryPlan& qp)
         if (battr >= bat.size() || pattr >= cur.size()) return false;
         const Slot bk = bat[battr], pk = cur[pattr];
         if (pk.slot != 0) return false;  // keyed by a build table: not a star join
+        keys.push_back(bk);  // a join key is never NULL in the output (NULL does not match)
+        keys.push_back(pk);
         const DevColumn& bc = bt->cols[bk.col];
         const DevColumn& pc = st->cols[pk.col];
         if (!streamable(bc) || !streamable(pc)) return false;
@@ -164,7 +168,10 @@ bool plan_query(const hj_plan* p, const hj_dtable* const* tables, QueryPlan& qp)
         q.oslot[c] = cur[c].slot;
         q.ocol[c] = dc.view();
         want_index(dc);
-        if (!dc.all_dense) q.onull |= 1u << c;
+        const bool key = std::any_of(keys.begin(), keys.end(), [&](const Slot& k) {
+            return k.slot == cur[c].slot && k.col == cur[c].col;
+        });
+        if (!dc.all_dense && !key) q.onull |= 1u << c;
     }
     q.src_rows = (uint32_t)st->num_rows;
     qp.src_rows = st->num_rows;
@@ -193,7 +200,7 @@ struct hj_batch {
     uint64_t ctr_words = 0;
     uint32_t nfill = 0;
     uint64_t max_fill = 0;
-    uint32_t nitems = 0, nidx = 0, probe0 = 0;
+    uint32_t nitems = 0, nidx = 0, probe0 = 0, wide0 = 0;
     BatchLaunch launch{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -262,6 +269,7 @@ void batch_build(hj_batch* b) {
     for (uint32_t i = 0; i < n; ++i) {
         if (plan_query(b->plans[i], b->tables[i].data(), b->qp[i])) {
             b->qidx[i] = (int32_t)b->nbatched++;
+            b->qp[i].launch = !b->prm.plain_probe_rows || b->qp[i].src_rows < b->prm.plain_probe_rows;
             order.push_back(i);
         }
     }
@@ -274,7 +282,10 @@ void batch_build(hj_batch* b) {
     std::vector<const DevColumn*> icols;
     std::map<const DevColumn*, uint32_t> icol_id;
     std::map<const uint8_t*, uint32_t> icol_pages;
+    std::vector<uint32_t> in_launch;
     for (uint32_t i : order)
+        if (b->qp[i].launch) in_launch.push_back(i);
+    for (uint32_t i : in_launch)
         for (const DevColumn* c : b->qp[i].idx_cols)
             if (!icol_id.count(c)) {
                 icol_id[c] = (uint32_t)icols.size();
@@ -295,6 +306,10 @@ void batch_build(hj_batch* b) {
         QueryPlan& qp = b->qp[i];
         const uint32_t qi = (uint32_t)b->qidx[i];
         for (int j = 0; j < qp.q.nj; ++j) {
+            if (!qp.launch) {
+                qp.k[j] = kMinTableBits;  // no table: the query runs on its own
+                continue;
+            }
             const uint32_t rows = qp.brows[j];
             const uint32_t shift = rows >= P.large_build_rows ? P.large_load_shift : P.load_shift;
             uint32_t k = kMinTableBits;
@@ -308,12 +323,24 @@ void batch_build(hj_batch* b) {
         }
     }
     b->probe0 = (uint32_t)items.size();
-    for (uint32_t i : order) {
-        const uint32_t qi = (uint32_t)b->qidx[i];
-        const uint64_t G = (b->qp[i].src_rows + 63) / 64;
-        for (uint64_t g = 0; g < G; g += P.probe_item_groups)
-            items.push_back({(kItemProbe << 24) | qi, 0, (uint32_t)g,
-                             (uint32_t)std::min<uint64_t>(G, g + P.probe_item_groups)});
+    // probe items: queries of one or two joins, then of three or four; within each, item k of
+    // every query before item k + 1 of any, so that the workgroups running at once reserve
+    // their output ranges from many queries' counters rather than all from the largest one's
+    for (int wide = 0; wide < 2; ++wide) {
+        if (wide) b->wide0 = (uint32_t)items.size();
+        for (uint64_t k = 0;; ++k) {
+            bool any = false;
+            for (uint32_t i : in_launch) {
+                if ((b->qp[i].q.nj > 2) != (wide != 0)) continue;
+                const uint64_t G = (b->qp[i].src_rows + 63) / 64;
+                const uint64_t g = k * P.probe_item_groups;
+                if (g >= G) continue;
+                any = true;
+                items.push_back({(kItemProbe << 24) | (uint32_t)b->qidx[i], 0, (uint32_t)g,
+                                 (uint32_t)std::min<uint64_t>(G, g + P.probe_item_groups)});
+            }
+            if (!any) break;
+        }
     }
     b->nitems = (uint32_t)items.size();
 
@@ -427,6 +454,7 @@ void batch_build(hj_batch* b) {
     d.iout = (uint64_t* const*)(D + o_iout);
     d.head = (uint32_t*)(D + o_ctr);
     d.probe0 = b->probe0;
+    d.wide0 = b->wide0;
     d.radix_bits = P.radix_bits;
     d.coop = P.coop;
     d.fill = (const FillSeg*)(D + o_fill);
@@ -441,7 +469,8 @@ void batch_build(hj_batch* b) {
     L.max_fill_words = b->max_fill;
     L.nidx = b->nidx;
     L.nins = b->probe0 - b->nidx;
-    L.nprobe = b->nitems - b->probe0;
+    L.nprobe = b->wide0 - b->probe0;
+    L.nwide = b->nitems - b->wide0;
     L.unroll = P.unroll;
     L.coop = P.coop;
     L.grids = batch_grids(P.unroll, P.coop, (uint32_t)ctx->num_cus);
@@ -469,8 +498,9 @@ int batch_trace() {
 }
 void trace(const char* what, const hj_batch* b) {
     if (batch_trace())
-        std::fprintf(stderr, "[hj_batch] %s: %u queries (%u batched), items %u index, %u insert, %u probe\n",
-                     what, b->n, b->nbatched, b->launch.nidx, b->launch.nins, b->launch.nprobe);
+        std::fprintf(stderr, "[hj_batch] %s: %u queries (%u batched), items %u index, %u insert, %u + %u probe\n",
+                     what, b->n, b->nbatched, b->launch.nidx, b->launch.nins, b->launch.nprobe,
+                     b->launch.nwide);
 }
 
 void batch_enqueue(hj_batch* b) {
@@ -491,6 +521,7 @@ void batch_enqueue(hj_batch* b) {
             }
             return;
         }
+        Prof p_(b->ctx, "batch_run");
         batch_launch(b->d_desc, b->launch, s);
         HJ_HIP(hipGetLastError());
         return;
@@ -506,6 +537,7 @@ void batch_enqueue(hj_batch* b) {
         HJ_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     }
     trace("graph launch", b);
+    Prof p_(b->ctx, "batch_run");  // the four launches, when kernel timing is on
     HJ_HIP(hipGraphLaunch(b->exec, s));
 }
 
@@ -549,6 +581,7 @@ int hj_batch_run(hj_context* ctx, hj_batch* b) {
             throw Error(HJ_E_BUSY, "free the previous device result before a batch runs");
         for (auto& r : b->own) hj_result_free(&r);
         std::fill(b->fallback.begin(), b->fallback.end(), 0);
+        std::fill(b->own_rc.begin(), b->own_rc.end(), HJ_OK);
         b->ran = false;
         batch_enqueue(b);
         HJ_HIP(hipMemcpyAsync(b->h_ctr, b->d_ctr, b->ctr_words * 4, hipMemcpyDeviceToHost,
@@ -557,22 +590,45 @@ int hj_batch_run(hj_context* ctx, hj_batch* b) {
         trace("done", b);
         for (uint32_t i = 0; i < b->n; ++i) {
             const int32_t qi = b->qidx[i];
-            if (qi >= 0) {
+            if (qi >= 0 && b->qp[i].launch) {
                 const uint32_t* c = b->h_ctr + 4 + (uint64_t)kQCtrWords * qi;
                 b->views[qi].num_rows = c[2];
-                if (c[1] == 0) continue;  // batched result is complete
+                if (c[1] == 0) continue;  // the launches' result is complete
             }
-            // on its own: not a star join, or a repeated build key
+            // on its own: not a star join, a source at or above plain_probe_rows, or a repeated
+            // build key
             b->fallback[i] = 1;
             hj_dresult* r = nullptr;
             int rc = hj_execute_device(ctx, b->plans[i], b->tables[i].data(), &r);
-            if (rc == HJ_OK) {
+            if (rc == HJ_OK && qi >= 0) {
+                // a star join: its columns copied into the batch's outputs (stream order)
+                rc = batch_guard(ctx, [&] {
+                    hj_dresult& v = b->views[qi];
+                    const uint64_t rows = r->num_rows;
+                    if (r->cols.size() != v.cols.size() || rows > b->qp[i].src_rows)
+                        throw Error(HJ_E_INVALID, "batch result shape differs from the plan's");
+                    for (size_t c = 0; c < v.cols.size(); ++c) {
+                        if (rows)
+                            HJ_HIP(hipMemcpyAsync(v.cols[c].values, r->cols[c].values, rows * 4,
+                                                  hipMemcpyDeviceToDevice, ctx->stream));
+                        const uint64_t vb = (rows + 63) / 64 * 8;
+                        if (vb && r->cols[c].validity)
+                            HJ_HIP(hipMemcpyAsync(v.cols[c].validity, r->cols[c].validity, vb,
+                                                  hipMemcpyDeviceToDevice, ctx->stream));
+                        else if (vb)
+                            HJ_HIP(hipMemsetAsync(v.cols[c].validity, 0xff, vb, ctx->stream));
+                    }
+                    v.num_rows = rows;
+                });
+                hj_dresult_free(ctx, r);
+            } else if (rc == HJ_OK) {
                 rc = hj_dresult_to_host(ctx, r, &b->own[i]);
                 hj_dresult_free(ctx, r);
             }
             b->own_rc[i] = rc;
             b->own_err[i] = rc ? ctx->last_error : std::string();
         }
+        HJ_HIP(hipStreamSynchronize(ctx->stream));
         b->ran = true;
     });
 }
@@ -581,23 +637,16 @@ uint32_t hj_batch_size(const hj_batch* b) { return b ? b->n : 0; }
 
 int hj_batch_info(const hj_batch* b, uint32_t i, int32_t* batched, uint64_t* rows) {
     if (!b || i >= b->n) return HJ_E_INVALID;
-    const bool in = b->qidx[i] >= 0 && !(b->ran && b->fallback[i]);
-    if (batched) *batched = in ? 1 : 0;
-    if (rows) {
-        if (!b->ran)
-            *rows = 0;
-        else if (in)
-            *rows = b->views[b->qidx[i]].num_rows;
-        else
-            *rows = b->own[i].num_rows;
-    }
+    const int32_t qi = b->qidx[i];
+    if (batched) *batched = (qi >= 0 && b->qp[i].launch && !(b->ran && b->fallback[i])) ? 1 : 0;
+    if (rows) *rows = !b->ran ? 0 : qi >= 0 ? b->views[qi].num_rows : b->own[i].num_rows;
     return HJ_OK;
 }
 
 int hj_batch_result(const hj_batch* b, uint32_t i, const hj_dresult** out) {
     if (!b || !out || i >= b->n || !b->ran) return HJ_E_INVALID;
     *out = nullptr;
-    if (b->qidx[i] < 0 || b->fallback[i]) return HJ_E_INVALID;
+    if (b->qidx[i] < 0 || b->own_rc[i]) return HJ_E_INVALID;
     *out = &b->views[b->qidx[i]];
     return HJ_OK;
 }
@@ -605,12 +654,11 @@ int hj_batch_result(const hj_batch* b, uint32_t i, const hj_dresult** out) {
 int hj_batch_result_to_host(hj_context* ctx, hj_batch* b, uint32_t i, hj_result* out) {
     if (!ctx || !b || !out || i >= b->n || !b->ran || ctx != b->ctx) return HJ_E_INVALID;
     std::memset(out, 0, sizeof *out);
-    if (b->qidx[i] >= 0 && !b->fallback[i])
-        return hj_dresult_to_host(ctx, &b->views[b->qidx[i]], out);
     if (b->own_rc[i]) {
         ctx->last_error = b->own_err[i];
         return b->own_rc[i];
     }
+    if (b->qidx[i] >= 0) return hj_dresult_to_host(ctx, &b->views[b->qidx[i]], out);
     if (!b->own[i].columns && b->own[i].num_rows == 0 && b->own[i].num_columns == 0)
         return HJ_E_INVALID;  // handed over already
     *out = b->own[i];

@@ -88,6 +88,7 @@ struct BatchDesc {
     uint64_t* const* iout;       // ... and their group index arrays
     uint32_t* head;              // counters of the batch (unused words, zeroed by the clear pass)
     uint32_t probe0;             // first probe item (the insert items are [nidx, probe0))
+    uint32_t wide0;              // first probe item of a query with three or four joins
     uint32_t radix_bits;         // insert items order their tuples by this many hash bits
     uint32_t coop;               // probe lookups: 0 one lane per row, 1 four lanes per row
     const FillSeg* fill;
@@ -96,14 +97,15 @@ struct BatchDesc {
 
 // blocks resident at once of each launch (probe: the variant for unroll x lane mapping)
 struct BatchGrids {
-    uint32_t index, insert, probe;
+    uint32_t index, insert, probe, wide;
 };
 BatchGrids batch_grids(uint32_t unroll, uint32_t coop, uint32_t num_cus);
 
 struct BatchLaunch {
     uint32_t nfill;
     uint64_t max_fill_words;
-    uint32_t nidx, nins, nprobe;  // items of each kind
+    uint32_t nidx, nins, nprobe, nwide;  // items of each kind (probe: one or two joins, wide:
+                                         // three or four)
     uint32_t unroll, coop;
     BatchGrids grids;
 };

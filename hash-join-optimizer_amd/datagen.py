@@ -204,51 +204,61 @@ def many_to_many_expected_count(d):
     return int((b * p).sum())
 
 
-def batch_joins(seed=48, n=256, scale=1.0, codec=None, null_p=0.05):
+def _batch_query(ss, scale, codec, null_p):
+    """one query of batch_joins from its own seed sequence"""
+    rng = np.random.default_rng(ss)
+    ways = int(rng.integers(2, 4))
+    nb = [int(np.exp(rng.uniform(np.log(1_000 * scale + 1), np.log(4_000_000 * scale + 1))))
+          for _ in range(ways - 1)]
+    npr = int(np.exp(rng.uniform(np.log(10_000 * scale + 1), np.log(40_000_000 * scale + 1))))
+    plan = Plan()
+    probe_cols = []
+    for m_rows in nb:
+        m = rng.uniform(0.1, 1.0)
+        dist = rng.choice(["uniform", "zipf0.8", "zipf1.2"])
+        if dist == "uniform":
+            fk = rng.integers(0, m_rows, npr, dtype=np.int32)
+        else:
+            w = np.arange(1, m_rows + 1, dtype=np.float64) ** (-(0.8 if dist == "zipf0.8" else 1.2))
+            cdf = np.cumsum(w)
+            cdf /= cdf[-1]
+            perm = rng.permutation(m_rows).astype(np.int32)
+            ranks = np.minimum(np.searchsorted(cdf, rng.random(npr), side="right"), m_rows - 1)
+            fk = perm[ranks]
+        miss = rng.random(npr) >= m
+        fk = np.where(miss, (m_rows + rng.integers(0, m_rows + 1, npr)).astype(np.int32), fk)
+        valid = (rng.random(npr) >= null_p).astype(np.uint8)
+        probe_cols.append(codec.encode_int32(fk.astype(np.int32), valid))
+    probe_cols.append(codec.encode_int32(rng.integers(0, 2**31, npr, dtype=np.int32)))
+    plan.inputs.append(ColumnarTable(npr, probe_cols))
+    for m_rows in nb:
+        plan.inputs.append(ColumnarTable(m_rows, [
+            codec.encode_int32(rng.permutation(m_rows).astype(np.int32)),
+            codec.encode_int32(rng.integers(0, 2**31, m_rows, dtype=np.int32))]))
+    nk = len(nb)
+    cur = plan.new_scan_node(0, [(j, I32) for j in range(nk + 1)])
+    width = nk + 1
+    for j in range(nk):
+        b = plan.new_scan_node(1 + j, [(0, I32), (1, I32)])
+        # the probe side's attributes: the j build payloads prepended so far, the FKs, its
+        # payload; FK j is attribute 2j
+        cur = plan.new_join_node(True, b, cur, 0, 2 * j,
+                                 [(1, I32)] + [(2 + i, I32) for i in range(width)])
+        width += 1
+    plan.root = cur
+    return plan
+
+
+def batch_joins(seed=48, n=256, scale=1.0, codec=None, null_p=0.05, threads=8):
     """n independent 2- and 3-way INT32 joins, each over its own tables: build sides of unique
     shuffled keys + an INT32 payload, rows log-uniform in [1K, 4M] x scale; a probe side of
     rows log-uniform in [10K, 40M] x scale with one FK per join (uniform, Zipf(0.8) or Zipf(1.2)
     over the build keys, match rate ~ U[0.1, 1.0]: the other FKs miss every build key; NULL with
     probability null_p) + an INT32 payload. Join j is keyed by the probe side's FK j; the output
-    is every build payload and every probe attribute. Returns a list of Plans with inputs."""
+    is every build payload and every probe attribute. Query i is drawn from child i of the seed's
+    sequence (numpy SeedSequence.spawn), so the queries are generated in parallel and the batch
+    is the same for any thread count. Returns a list of Plans with inputs."""
     codec = codec or Codec()
-    rng = np.random.default_rng(seed)
-    plans = []
-    for _ in range(n):
-        ways = int(rng.integers(2, 4))
-        nb = [int(np.exp(rng.uniform(np.log(1_000 * scale + 1), np.log(4_000_000 * scale + 1))))
-              for _ in range(ways - 1)]
-        npr = int(np.exp(rng.uniform(np.log(10_000 * scale + 1), np.log(40_000_000 * scale + 1))))
-        plan = Plan()
-        fk_cols, fk_valid = [], []
-        for m_rows in nb:
-            m = rng.uniform(0.1, 1.0)
-            dist = rng.choice(["uniform", "zipf0.8", "zipf1.2"])
-            if dist == "uniform":
-                fk = rng.integers(0, m_rows, npr)
-            else:
-                fk, _ = zipf_keys(rng, m_rows, 0.8 if dist == "zipf0.8" else 1.2, npr)
-            miss = rng.random(npr) >= m
-            fk = np.where(miss, m_rows + rng.integers(0, m_rows + 1, npr), fk).astype(np.int32)
-            fk_cols.append(fk)
-            fk_valid.append((rng.random(npr) >= null_p).astype(np.uint8))
-        probe_cols = [codec.encode_int32(fk, v) for fk, v in zip(fk_cols, fk_valid)]
-        probe_cols.append(codec.encode_int32(rng.integers(0, 2**31, npr).astype(np.int32)))
-        plan.inputs.append(ColumnarTable(npr, probe_cols))
-        for m_rows in nb:
-            plan.inputs.append(ColumnarTable(m_rows, [
-                codec.encode_int32(rng.permutation(m_rows).astype(np.int32)),
-                codec.encode_int32(rng.integers(0, 2**31, m_rows).astype(np.int32))]))
-        nk = len(nb)
-        cur = plan.new_scan_node(0, [(j, I32) for j in range(nk + 1)])
-        width = nk + 1
-        for j in range(nk):
-            b = plan.new_scan_node(1 + j, [(0, I32), (1, I32)])
-            # the probe side's attributes: the j build payloads prepended so far, the FKs, its
-            # payload; FK j is attribute 2j
-            cur = plan.new_join_node(True, b, cur, 0, 2 * j,
-                                     [(1, I32)] + [(2 + i, I32) for i in range(width)])
-            width += 1
-        plan.root = cur
-        plans.append(plan)
-    return plans
+    seqs = np.random.SeedSequence(seed).spawn(n)
+    with ThreadPoolExecutor(max_workers=max(1, min(threads, n))) as pool:
+        return list(pool.map(lambda ss: _batch_query(ss, scale, codec, null_p), seqs))

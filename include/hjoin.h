@@ -271,7 +271,11 @@ typedef struct hj_batch_params {
     uint32_t unroll;            /* 2 or 4 (0: 4): 64-row groups per wave and round of a probe item */
     uint32_t coop;              /* probe lookups: 0 one lane per probe row, 1 four lanes per row */
     uint32_t use_graph;         /* 1: the launch is a HIP graph instantiated on the first run */
-    uint32_t reserved[3];
+    uint32_t plain_probe_rows;  /* star joins whose source has at least this many rows run one at
+                                   a time through the fused pipeline after the launches, their
+                                   results copied into the batch (0: every star join in the
+                                   launches) */
+    uint32_t reserved[2];
 } hj_batch_params;
 /* plans[i] runs over tables[i][0 .. plans[i]->num_inputs); plans and tables stay owned by the
  * caller and must outlive the batch. params NULL: the defaults above with use_graph = 1. */
@@ -282,10 +286,11 @@ int hj_batch_create(hj_context* ctx, uint32_t n, const hj_plan* const* plans,
  * returns when every result is complete. Results of the previous run are replaced. */
 int hj_batch_run(hj_context* ctx, hj_batch* b);
 uint32_t hj_batch_size(const hj_batch* b);
-/* after a run: whether query i ran in the persistent launch, and its result rows */
+/* after a run: whether query i ran in the batch's launches, and its result rows */
 int hj_batch_info(const hj_batch* b, uint32_t i, int32_t* batched, uint64_t* rows);
-/* the device result of batched query i (owned by the batch, valid until its next run or
- * hj_batch_free; hj_dresult_free refuses it); HJ_E_INVALID for a query that ran on its own */
+/* the device result of query i (owned by the batch, valid until its next run or
+ * hj_batch_free; hj_dresult_free refuses it): every star join has one, whether it ran in the
+ * launches or on its own; HJ_E_INVALID for any other query (its result is in host pages) */
 int hj_batch_result(const hj_batch* b, uint32_t i, const hj_dresult** out);
 /* the result of query i as pages in host memory (free with hj_result_free); a query that ran on
  * its own hands over its result once */

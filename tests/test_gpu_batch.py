@@ -69,6 +69,7 @@ def test_batch_matches_oracle(hjo, gpu_ctx, oracle, small_batch):
     {"large_build_rows": 5000, "large_load_shift": 3},
     {"probe_item_groups": 1},
     {"probe_item_groups": 3},
+    {"plain_probe_rows": 5000},
 ], ids=lambda p: ",".join(f"{k}={v}" for k, v in p.items()))
 def test_batch_parameters(hjo, gpu_ctx, oracle, small_batch, params):
     plans, want = small_batch
@@ -76,7 +77,8 @@ def test_batch_parameters(hjo, gpu_ctx, oracle, small_batch, params):
     print(f"params {params}", flush=True)
     [(info, res)] = _run(hjo, gpu_ctx, plans, dict(params, use_graph=0))
     for i in range(len(plans)):
-        assert info[i][0], i
+        # sources of at least plain_probe_rows rows run through the fused pipeline
+        assert info[i][0] == (plans[i].inputs[0].num_rows < params.get("plain_probe_rows", 1 << 31)), i
         assert oracle.rows(res[i]) == want[i], i
 
 
