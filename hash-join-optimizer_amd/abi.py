@@ -142,6 +142,11 @@ _SIGS = {
     "hj_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "hj_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hj_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "hj_dir_cache_stats": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
+                                     P(C.c_uint64)]),
+    "hj_server_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "hj_server_cache_stats": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
+                                        P(C.c_uint64)]),
     "hj_profile_enable": (C.c_int, [C.c_void_p, C.c_char_p]),
     "hj_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64]),
 }

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstddef>
 #include <cstdio>
@@ -189,6 +190,12 @@ class Executor {
   public:
     Executor(hj_context* ctx, const hj_plan* plan, const hj_dtable* const* tables)
         : ctx_(ctx), plan_(plan), tables_(tables), s_(ctx->stream), A_(ctx->arena) {}
+    // cached tables read by this query's kernels are released once those kernels are done
+    ~Executor() {
+        if (cache_refs_.empty()) return;
+        (void)hipStreamSynchronize(s_);
+        for (auto& k : cache_refs_) ctx_->dcache->release(k);
+    }
 
 
     hj_dresult* run() {
@@ -418,7 +425,7 @@ class Executor {
         return r;
     }
 
-    Rel scan(const hj_node& nd) {
+    Rel scan(const hj_node& nd, bool run_preds = true) {
         const hj_dtable* tb = table(nd.base_table_id);
         Rel r;
         r.is_scan = true;
@@ -438,7 +445,7 @@ class Executor {
                 throw Error(HJ_E_TYPE, "filter column must be an INT32 column of the scanned table");
             r.filters.push_back(fl);
         }
-        if (nd.num_preds) predicate(nd, tb, r);
+        if (nd.num_preds && run_preds) predicate(nd, tb, r);
         return r;
     }
 
@@ -783,6 +790,24 @@ class Executor {
         uint32_t col;
     };
     std::vector<AttrSlot> pipe_keys_;  // the pipeline's probe keys (slot, column)
+    std::vector<std::string> cache_refs_;  // directory cache entries this query holds
+    std::vector<uint32_t> job_tuples_;     // tuples of each built pipeline job, after the pass
+
+    // directory cache key of a build scan keyed by column c: the table's upload id, the column,
+    // the scan's filters and its predicate program
+    std::string cache_key(const hj_node& bn, uint32_t c) const {
+        std::string k = std::to_string(table(bn.base_table_id)->uid) + ":" + std::to_string(c);
+        for (uint64_t f = 0; f < bn.num_filters; ++f)
+            k += "|f" + std::to_string(bn.filters[f].column) + "," +
+                 std::to_string(bn.filters[f].lo) + "," + std::to_string(bn.filters[f].hi);
+        for (uint64_t e = 0; e < bn.num_preds && bn.preds; ++e) {
+            const hj_pred& p = bn.preds[e];
+            k += "|p" + std::to_string(p.op) + "," + std::to_string(p.column) + "," +
+                 std::to_string(p.ival) + "," + std::to_string(p.slen) + ":";
+            if (p.sval && p.slen) k.append(p.sval, p.slen);
+        }
+        return k;
+    }
 
     bool chain(uint64_t idx, uint64_t& source, std::vector<uint64_t>& joins, int depth) {
         if (idx >= plan_->num_nodes || depth > kMaxStages + 1) return false;
@@ -845,8 +870,11 @@ class Executor {
         bool any_filter = false;  // a build scan with filters or predicates
         std::memset(&hb, 0, sizeof hb);
         std::memset(&hp, 0, sizeof hp);
-        hb.njobs = (int32_t)joins.size();
+        hb.njobs = 0;  // build jobs: the joins whose table is not in the directory cache
         hp.nst = (int32_t)joins.size();
+        std::vector<const DirCache::Entry*> cached(joins.size(), nullptr);
+        std::vector<std::string> ckeys(joins.size());
+        std::vector<uint64_t> stage_rows(joins.size(), 0);
         // descriptors (build batch, pipeline) and the counters (pipeline, jobs, direct output) in
         // one block, uploaded together before the first kernel (the counters as zeros)
         const size_t bb_bytes = sizeof(BuildBatch);
@@ -861,7 +889,22 @@ class Executor {
         for (size_t i = 0; i < joins.size(); ++i) {
             const hj_node& jn = plan_->nodes[joins[i]];
             const bool bl = jn.build_left != 0;
-            const Rel brel = scan(plan_->nodes[bl ? jn.left : jn.right]);
+            const hj_node& bn = plan_->nodes[bl ? jn.left : jn.right];
+            // a build side keyed by the source may come from the directory cache
+            {
+                const uint64_t ba = bl ? jn.left_attr : jn.right_attr;
+                const uint64_t pa = bl ? jn.right_attr : jn.left_attr;
+                if (ctx_->dcache && pa < cur.size() && cur[pa].slot == 0 &&
+                    ba < bn.num_output_attrs && bn.output_attrs) {
+                    ckeys[i] = cache_key(bn, (uint32_t)bn.output_attrs[ba].index);
+                    cached[i] = ctx_->dcache->acquire(ckeys[i]);
+                    if (cached[i]) cache_refs_.push_back(ckeys[i]);
+                }
+            }
+            // a cached build side needs only its shape: its filters and predicate program ran
+            // when its table was built
+            const Rel brel = scan(bn, !cached[i]);
+            if (bn.num_filters || bn.num_preds) any_filter = true;
             std::vector<AttrSlot> bat;
             for (auto& a : brel.attrs) bat.push_back({(int)i + 1, a.col});
             slot_table.push_back(brel.table);
@@ -878,40 +921,44 @@ class Executor {
             if (i == 0 && pk.slot != 0) return Pipe::kNo;
             if (brel.n >= (1ull << 28)) return Pipe::kNo;  // key slots of the table
 
-            BuildJob& jb = hb.job[i];
-            jb.key = qview(slot_table[bk.slot], bk.col);
-            if (!build_ok(jb.key)) return Pipe::kNo;
-            jb.nf = (int32_t)brel.filters.size();
-            if (!brel.filters.empty()) any_filter = true;
-            for (size_t f = 0; f < brel.filters.size(); ++f) {
-                jb.fcol[f] = filter_view(brel, brel.filters[f].column);
-                if (!filter_ok(jb.fcol[f])) return Pipe::kNo;
-                jb.flo[f] = brel.filters[f].lo;
-                jb.fhi[f] = brel.filters[f].hi;
-            }
-            const uint32_t rows = (uint32_t)brel.n;
-            jb.rows = rows;
-            jb.kmax = table_bits(rows);
-            jb.meta = metas + 4 * (i + 1);
-            jb.nblk = std::max<uint32_t>(1, (rows + kBuildRows - 1) / kBuildRows);
-            jb.blk0 = hb.grid;
-            hb.grid += jb.nblk;
-            const uint64_t scap = (uint64_t)jb.nblk * kBuildRows;
-            jb.st_key = scratch<uint32_t>(scap);
-            jb.st_row = scratch<uint32_t>(scap);
-            jb.blk = scratch<uint32_t>(jb.nblk);
-            // clear pass: 16-byte stores, 4 per thread and round, over the largest table
-            const uint64_t stores = (1ull << jb.kmax) / 2;
-            hb.cgrid = (uint32_t)std::max<uint64_t>(
-                hb.cgrid, std::min<uint64_t>(1024, (stores + 4 * 256 - 1) / (4 * 256)));
-            jb.table = scratch<uint32_t>(2ull * (1ull << jb.kmax) + 8);
-            jb.bf = scratch<uint32_t>(kBloomWords);
-
             PipeStage& st = hp.st[i];
-            st.job = (int32_t)i;
             st.key_slot = pk.slot;
             st.keycol = qview(slot_table[pk.slot], pk.col);
             if (pk.slot == 0 && !idx_ok(st.keycol)) return Pipe::kNo;
+            if (cached[i]) {  // no build job: the stream join reads the cached table
+                st.job = -1;
+                stage_rows[i] = cached[i]->tuples;
+            } else {
+                stage_rows[i] = brel.n;
+                st.job = hb.njobs;
+                BuildJob& jb = hb.job[hb.njobs++];
+                jb.key = qview(slot_table[bk.slot], bk.col);
+                if (!build_ok(jb.key)) return Pipe::kNo;
+                jb.nf = (int32_t)brel.filters.size();
+                for (size_t f = 0; f < brel.filters.size(); ++f) {
+                    jb.fcol[f] = filter_view(brel, brel.filters[f].column);
+                    if (!filter_ok(jb.fcol[f])) return Pipe::kNo;
+                    jb.flo[f] = brel.filters[f].lo;
+                    jb.fhi[f] = brel.filters[f].hi;
+                }
+                const uint32_t rows = (uint32_t)brel.n;
+                jb.rows = rows;
+                jb.kmax = table_bits(rows);
+                jb.meta = metas + 4 * (st.job + 1);
+                jb.nblk = std::max<uint32_t>(1, (rows + kBuildRows - 1) / kBuildRows);
+                jb.blk0 = hb.grid;
+                hb.grid += jb.nblk;
+                const uint64_t scap = (uint64_t)jb.nblk * kBuildRows;
+                jb.st_key = scratch<uint32_t>(scap);
+                jb.st_row = scratch<uint32_t>(scap);
+                jb.blk = scratch<uint32_t>(jb.nblk);
+                // clear pass: 16-byte stores, 4 per thread and round, over the largest table
+                const uint64_t stores = (1ull << jb.kmax) / 2;
+                hb.cgrid = (uint32_t)std::max<uint64_t>(
+                    hb.cgrid, std::min<uint64_t>(1024, (stores + 4 * 256 - 1) / (4 * 256)));
+                jb.table = scratch<uint32_t>(2ull * (1ull << jb.kmax) + 8);
+                jb.bf = scratch<uint32_t>(kBloomWords);
+            }
 
             const std::vector<AttrSlot>& L = bl ? bat : cur;
             const std::vector<AttrSlot>& R = bl ? cur : bat;
@@ -927,6 +974,8 @@ class Executor {
             }
             cur.swap(nxt);
         }
+        stage_job_.assign(joins.size(), -1);
+        for (size_t i = 0; i < joins.size(); ++i) stage_job_[i] = hp.st[i].job;
         for (int j = 0; j < kMaxStages; ++j) hb.blk0[j] = j < hb.njobs ? hb.job[j].blk0 : ~0u;
         // partitioned build (segments built in LDS) when every job has at most 2^kMaxFineBits
         // fine partitions and kMaxPartBlocks filter blocks; else clear + compare-and-swap inserts
@@ -985,13 +1034,17 @@ class Executor {
                 ++hp.ndep;
         }
         std::stable_sort(hp.order, hp.order + hp.nstream, [&](int a, int b) {
-            return hb.job[hp.st[a].job].rows < hb.job[hp.st[b].job].rows;
+            return stage_rows[a] < stage_rows[b];
         });
         for (int q = 0; q < hp.nstream; ++q) {
             const int st = hp.order[q];
-            const BuildJob& jb = hb.job[hp.st[st].job];
             hp.qpos[st] = q;
-            hp.sj[q] = {hp.st[st].keycol, jb.table, jb.meta, jb.bf};
+            if (cached[st]) {
+                hp.sj[q] = {hp.st[st].keycol, cached[st]->table, cached[st]->meta, cached[st]->bf};
+            } else {
+                const BuildJob& jb = hb.job[hp.st[st].job];
+                hp.sj[q] = {hp.st[st].keycol, jb.table, jb.meta, jb.bf};
+            }
         }
         // output: written directly from the fused pass's staged survivors when every join is
         // fused (at most one output row per source row: cap rows), else compacted row ids
@@ -1066,13 +1119,23 @@ class Executor {
         host_trace().mark("desc_copy");
         const BuildBatch* dbb = (const BuildBatch*)ddesc;
         const PipeDesc* dpd = (const PipeDesc*)(ddesc + pd_off);
-        { Prof p_(ctx_, "pipe_build"); pipe_build(hb, dbb, s_); }
+        if (hb.njobs) {
+            Prof p_(ctx_, "pipe_build");
+            pipe_build(hb, dbb, s_);
+        } else if (hb.igrid) {  // every table cached: the probe side's index on its own
+            Prof p_(ctx_, "group_index");
+            launch_group_index(hb.gib, s_);
+        }
         host_trace().mark("build_q");
         {
             Prof p_(ctx_, "probe_fused", true);
             pipe_probe_fused(hp, dpd, s_, p_.a, p_.b);
         }
-        if (emit) return emit_result(hp, oslots, slot_table, cur, metas, any_filter, dres);
+        if (emit) {
+            const Pipe r = emit_result(hp, oslots, slot_table, cur, metas, any_filter, dres);
+            if (r == Pipe::kResult) publish(hb, ckeys, cached);
+            return r;
+        }
         { Prof p_(ctx_, "probe_tail"); pipe_probe_tail(hp, dpd, s_); }
         struct F4 {
             uint32_t v[4];
@@ -1090,8 +1153,30 @@ class Executor {
         if (fl.v[0]) return Pipe::kNo;
         res.n = fl.v[3];
         out = std::move(res);
+        if (ctx_->dcache && hb.njobs) {
+            job_tuples_.assign(hb.njobs, 0);
+            for (int j = 0; j < hb.njobs; ++j)
+                job_tuples_[j] = fetch(ctx_, (const F4*)hb.job[j].meta).v[0];
+            publish(hb, ckeys, cached);
+        }
         return Pipe::kRel;
     }
+
+    // the tables this query built for stream joins, copied into the directory cache
+    void publish(const BuildBatch& hb, const std::vector<std::string>& ckeys,
+                 const std::vector<const DirCache::Entry*>& cached) {
+        if (!ctx_->dcache) return;
+        for (size_t i = 0; i < ckeys.size(); ++i) {
+            if (ckeys[i].empty() || cached[i]) continue;
+            const int j = plan_stage_job(i);
+            if (j < 0 || j >= (int)job_tuples_.size()) continue;
+            const BuildJob& jb = hb.job[j];
+            ctx_->dcache->insert(ckeys[i], jb.table, table_bits(job_tuples_[j]), jb.meta, jb.bf,
+                                 job_tuples_[j], s_);
+        }
+    }
+    std::vector<int32_t> stage_job_;  // pipeline stage -> its build job (-1: cached)
+    int plan_stage_job(size_t i) const { return i < stage_job_.size() ? stage_job_[i] : -1; }
 
     // the root's columns straight from the fused pass (emit_rows), then one round trip for the
     // counters: fallback flag, rows, string totals, overflow
@@ -1169,6 +1254,8 @@ class Executor {
         uint64_t o[kEmitOut];
         std::memcpy(m, pin, sizeof m);
         std::memcpy(o, pin + kMetaWords * 4, sizeof o);
+        job_tuples_.assign(kMaxStages, 0);
+        for (int j = 0; j < kMaxStages; ++j) job_tuples_[j] = m[4 + 4 * j];
         if (std::getenv("HJ_DEBUG_PIPE"))
             std::fprintf(stderr, "[pipe] fallback %u rows %llu overflow %llu\n", m[0],
                          (unsigned long long)o[0], (unsigned long long)o[1 + kEmitCols]);
@@ -1769,8 +1856,37 @@ int hj_set_option(hj_context* ctx, const char* name, int64_t value) {
         ctx->gpu_pages = value != 0;
         return HJ_OK;
     }
+    if (std::strcmp(name, "dir_cache_bytes") == 0) {
+        if (value < 0) return HJ_E_INVALID;
+        (void)hipStreamSynchronize(ctx->stream);
+        if (value == 0) {
+            ctx->dcache.reset();
+        } else {
+            ctx->dcache = std::make_shared<DirCache>();
+            ctx->dcache->budget = (uint64_t)value;
+        }
+        return HJ_OK;
+    }
     ctx->last_error = std::string("unknown option ") + name;
     return HJ_E_INVALID;
+}
+
+int hj_dir_cache_stats(hj_context* ctx, uint64_t* hits, uint64_t* misses, uint64_t* entries,
+                       uint64_t* bytes) {
+    if (!ctx) return HJ_E_INVALID;
+    uint64_t h = 0, m = 0, e = 0, b = 0;
+    if (ctx->dcache) {
+        std::lock_guard<std::mutex> lk(ctx->dcache->mu);
+        h = ctx->dcache->hits;
+        m = ctx->dcache->misses;
+        e = ctx->dcache->map.size();
+        b = ctx->dcache->used;
+    }
+    if (hits) *hits = h;
+    if (misses) *misses = m;
+    if (entries) *entries = e;
+    if (bytes) *bytes = b;
+    return HJ_OK;
 }
 
 int hj_profile_enable(hj_context* ctx, const char* prefix) {
@@ -1821,6 +1937,8 @@ int hj_table_upload(hj_context* ctx, const hj_table* table, hj_dtable** out) {
     if (!ctx || !table || !out) return HJ_E_INVALID;
     *out = nullptr;
     auto* t = new hj_dtable();
+    static std::atomic<uint64_t> next_uid{1};
+    t->uid = next_uid.fetch_add(1);
     int rc = guarded(ctx, [&] {
         if (table->num_columns && !table->columns) throw Error(HJ_E_INVALID, "columns missing");
         t->num_rows = table->num_rows;

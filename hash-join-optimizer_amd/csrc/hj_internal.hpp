@@ -4,6 +4,9 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -128,6 +131,7 @@ struct DevColumn {
 }  // namespace hj
 
 struct hj_dtable {
+    uint64_t uid = 0;  // unique per upload in the process (a key of the directory cache)
     uint64_t num_rows = 0;
     std::vector<hj::DevColumn> cols;
     std::vector<void*> allocations;
@@ -149,8 +153,42 @@ struct hj_dresult {
     bool batch_view = false;  // columns owned by an hj_batch (not freed with the result)
 };
 
+namespace hj {
+// Built pipeline tables kept in HBM across queries (no counterpart in the reference, which builds
+// every join's table in its execute()). An entry is one build side's bucketized table, its
+// counters (tuple count, the INT32_MIN flag) and Bloom words, keyed by the build scan: the
+// table's upload id, the key column and the scan's filters and predicate program. Entries in
+// use by a running query hold a reference; the least recently used unreferenced ones are freed
+// when a new entry would exceed the byte budget. Shared by the contexts of one device (a
+// server's slots); every method takes the lock.
+struct DirCache {
+    struct Entry {
+        uint32_t* table = nullptr;  // 2 * 2^k + 8 words
+        uint32_t* meta = nullptr;   // 4 words, then kBloomWords Bloom words (one allocation)
+        uint32_t* bf = nullptr;
+        uint32_t tuples = 0;
+        uint64_t bytes = 0;
+        uint64_t last_use = 0;
+        int refs = 0;
+    };
+    std::mutex mu;
+    uint64_t budget = 0, used = 0, clock = 0;
+    uint64_t hits = 0, misses = 0, evictions = 0;
+    std::map<std::string, Entry> map;
+    ~DirCache();
+    // the entry for key with a reference taken, or nullptr
+    const Entry* acquire(const std::string& key);
+    void release(const std::string& key);
+    // publish a copy of a built table (device pointers, complete in stream order on s): allocate,
+    // copy on s, wait for s, then insert; false when it does not fit the budget
+    bool insert(const std::string& key, const uint32_t* table, uint32_t k, const uint32_t* meta,
+                const uint32_t* bf, uint32_t tuples, hipStream_t s);
+};
+}  // namespace hj
+
 struct hj_context {
     int device = 0;
+    std::shared_ptr<hj::DirCache> dcache;  // built tables kept across queries (nullptr: none)
     hipStream_t stream = nullptr;
     hj::Arena arena;
     uint8_t* arena_mem = nullptr;

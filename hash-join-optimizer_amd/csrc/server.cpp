@@ -2,10 +2,12 @@
 //
 // No counterpart in the reference (its execute() runs one query at a time). Each slot is a
 // context (its own stream, HBM arena and pinned staging) driven by its own host thread; queries
-// wait in one FIFO queue and a free slot takes the next one, so queries that arrive while others
-// run are admitted as soon as a slot frees up and their kernels overlap on the device. A query
-// is admitted only if the memory it reserves (an estimate of its build tables, staging and
-// output) fits a slot's arena; otherwise hj_server_submit refuses it with HJ_E_BUSY.
+// wait in one FIFO queue and the next one is admitted as soon as a slot is free and the memory it
+// reserves (an estimate of its staging, build tables and output) fits what the running queries
+// leave of the server's budget, so queries that arrive while others run overlap on the device. A
+// query whose reservation exceeds a slot's arena or the whole budget is refused with HJ_E_BUSY.
+// The slots share one directory cache: a dimension's table, built once for a (table, key,
+// filters) triple, is probed from HBM by every later query that uses it.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -31,6 +33,7 @@ uint64_t now_ns() {
 
 struct Job {
     uint64_t ticket;
+    uint64_t reserve = 0;
     const hj_plan* plan;
     std::vector<const hj_dtable*> tables;
     uint64_t submit_ns = 0, start_ns = 0, done_ns = 0;
@@ -46,6 +49,9 @@ struct hj_server {
     std::vector<hj_context*> ctx;
     std::vector<std::thread> workers;
     uint64_t arena_bytes = 0;
+    uint64_t budget = 0;    // bytes the running queries may reserve together
+    uint64_t reserved = 0;  // what they reserve now
+    std::shared_ptr<hj::DirCache> dcache;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Job*> queue;
@@ -76,11 +82,17 @@ void worker(hj_server* s, hj_context* ctx) {
     for (;;) {
         Job* j = nullptr;
         {
+            // FIFO: the queue's first query runs once its reservation fits what the running
+            // queries leave of the budget
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [s] { return s->stop || !s->queue.empty(); });
-            if (s->stop && s->queue.empty()) return;
+            s->cv_work.wait(lk, [s] {
+                return (s->stop && s->queue.empty()) ||
+                       (!s->queue.empty() && s->reserved + s->queue.front()->reserve <= s->budget);
+            });
+            if (s->queue.empty()) return;
             j = s->queue.front();
             s->queue.pop_front();
+            s->reserved += j->reserve;
         }
         j->start_ns = now_ns();
         hj_dresult* r = nullptr;
@@ -95,8 +107,10 @@ void worker(hj_server* s, hj_context* ctx) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             j->done = true;
+            s->reserved -= j->reserve;
         }
         s->cv_done.notify_all();
+        s->cv_work.notify_all();  // the memory it reserved may admit the next query
     }
 }
 
@@ -109,6 +123,9 @@ int hj_server_create(uint32_t slots, uint64_t arena_bytes_per_slot, hj_server** 
     *out = nullptr;
     auto* s = new hj_server();
     s->arena_bytes = arena_bytes_per_slot ? arena_bytes_per_slot : (4ull << 30);
+    s->budget = s->arena_bytes * slots;
+    s->dcache = std::make_shared<hj::DirCache>();
+    s->dcache->budget = 1ull << 30;
     for (uint32_t i = 0; i < slots; ++i) {
         hj_context* c = nullptr;
         const int rc = hj_context_create(s->arena_bytes, &c);
@@ -117,6 +134,7 @@ int hj_server_create(uint32_t slots, uint64_t arena_bytes_per_slot, hj_server** 
             delete s;
             return rc;
         }
+        c->dcache = s->dcache;
         s->ctx.push_back(c);
     }
     for (auto* c : s->ctx) s->workers.emplace_back(worker, s, c);
@@ -149,8 +167,14 @@ int hj_server_submit(hj_server* s, const hj_plan* plan, const hj_dtable* const* 
     for (uint64_t i = 0; i < plan->num_nodes; ++i)
         if (plan->nodes[i].kind == HJ_NODE_SCAN && plan->nodes[i].base_table_id >= plan->num_inputs)
             return HJ_E_INVALID;
-    if (reserve_bytes(plan, tables) > s->arena_bytes) return HJ_E_BUSY;
+    const uint64_t reserve = reserve_bytes(plan, tables);
+    if (reserve > s->arena_bytes) return HJ_E_BUSY;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (reserve > s->budget) return HJ_E_BUSY;  // could never be admitted
+    }
     auto* j = new Job();
+    j->reserve = reserve;
     j->plan = plan;
     j->tables.assign(tables, tables + plan->num_inputs);
     j->submit_ns = now_ns();
@@ -191,6 +215,28 @@ int hj_server_wait(hj_server* s, uint64_t ticket, hj_result* result, uint64_t* q
 }
 
 uint32_t hj_server_slots(const hj_server* s) { return s ? (uint32_t)s->ctx.size() : 0; }
+
+int hj_server_set_option(hj_server* s, const char* name, int64_t value) {
+    if (!s || !name || value < 0) return HJ_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (std::strcmp(name, "budget_bytes") == 0) {
+        s->budget = (uint64_t)value;
+        s->cv_work.notify_all();
+        return HJ_OK;
+    }
+    if (std::strcmp(name, "dir_cache_bytes") == 0) {
+        std::lock_guard<std::mutex> ck(s->dcache->mu);
+        s->dcache->budget = (uint64_t)value;  // 0: nothing is cached from now on
+        return HJ_OK;
+    }
+    return HJ_E_INVALID;
+}
+
+int hj_server_cache_stats(hj_server* s, uint64_t* hits, uint64_t* misses, uint64_t* entries,
+                          uint64_t* bytes) {
+    if (!s || s->ctx.empty()) return HJ_E_INVALID;
+    return hj_dir_cache_stats(s->ctx[0], hits, misses, entries, bytes);
+}
 
 const char* hj_server_last_error(hj_server* s) {
     if (!s) return "";

@@ -262,3 +262,80 @@ def batch_joins(seed=48, n=256, scale=1.0, codec=None, null_p=0.05, threads=8):
     seqs = np.random.SeedSequence(seed).spawn(n)
     with ThreadPoolExecutor(max_workers=max(1, min(threads, n))) as pool:
         return list(pool.map(lambda ss: _batch_query(ss, scale, codec, null_p), seqs))
+
+
+# the serving workload's fact table: five FK columns over the three dimensions of star4 (the
+# fourth and fifth are second roles of dimensions 0 and 1, under their own Zipf permutations)
+SERVE_FK_DIM = (0, 1, 2, 0, 1)
+
+
+def serve_star(seed=47, n_fact=36_000_000, dims=STAR_DIMS, s=1.1, null_p=0.03):
+    """dimension tables as star4 (unique shuffled keys, an INT32 attribute ~ U[0, 10000), a
+    VARCHAR of 8-64 chars) and a fact table with the five FK columns of SERVE_FK_DIM, each
+    Zipf(s) over its dimension's keys with NULL probability null_p"""
+    rng = np.random.default_rng(seed)
+    out = {"fact": [], "fact_valid": [], "dims": []}
+    for nd in dims:
+        soff, schars = random_strings(rng, nd)
+        out["dims"].append(dict(key=rng.permutation(nd).astype(np.int32),
+                                attr=rng.integers(0, 10000, nd).astype(np.int32),
+                                soff=soff, schars=schars))
+    for di in SERVE_FK_DIM:
+        fk, _ = zipf_keys(rng, dims[di], s, n_fact)
+        out["fact"].append(fk.astype(np.int32))
+        out["fact_valid"].append((rng.random(n_fact) >= null_p).astype(np.uint8))
+    return out
+
+
+def serve_tables(d, fact_sizes, codec=None):
+    """the resident tables of the serving workload: the fact table's prefixes of fact_sizes rows,
+    then the dimensions (key, attribute, string)"""
+    codec = codec or Codec()
+    facts = [ColumnarTable(n, [codec.encode_int32(fk[:n], v[:n])
+                               for fk, v in zip(d["fact"], d["fact_valid"])]) for n in fact_sizes]
+    dims = [ColumnarTable(len(dm["key"]), [codec.encode_int32(dm["key"]),
+                                           codec.encode_int32(dm["attr"]),
+                                           codec.encode_varchar(dm["soff"], dm["schars"])])
+            for dm in d["dims"]]
+    return facts + dims
+
+
+def serve_queries(rng, n, fact_sizes, n_dims=3, pool=16, max_joins=5):
+    """n star joins over serve_tables' tables: a fact prefix of log-uniform [100K, 36M] rows
+    (the nearest of fact_sizes), 1..max_joins of its FK columns (2- to 6-way), each joined with its
+    dimension under a range filter on the attribute of 1-20 % selectivity drawn from a pool of
+    `pool` filters per dimension (Zipf(1) over the pool: popular filters repeat, as parameterized
+    queries do); output the FKs used and the joined dimensions' strings. Returns Plans without
+    inputs (input i is table i of serve_tables)."""
+    prng = np.random.default_rng(rng.integers(1 << 62))
+    filters = []
+    for _ in range(n_dims):
+        f = []
+        for _ in range(pool):
+            w = int(prng.integers(100, 2001))
+            lo = int(prng.integers(0, 10_000 - w))
+            f.append((lo, lo + w - 1))
+        filters.append(f)
+    wpool = 1.0 / np.arange(1, pool + 1)
+    wpool /= wpool.sum()
+    plans = []
+    nf = len(fact_sizes)
+    for _ in range(n):
+        size = float(np.exp(rng.uniform(np.log(100_000), np.log(36_000_000))))
+        fi = int(np.argmin([abs(np.log(size / f)) for f in fact_sizes]))
+        k = int(rng.integers(1, max_joins + 1))
+        fks = sorted(rng.choice(len(SERVE_FK_DIM), size=k, replace=False).tolist())
+        plan = Plan()
+        plan.inputs = [None] * (nf + n_dims)
+        cur = plan.new_scan_node(fi, [(j, I32) for j in fks])
+        types = [I32] * len(fks)
+        for pos, j in enumerate(fks):
+            di = SERVE_FK_DIM[j]
+            lo, hi = filters[di][int(rng.choice(pool, p=wpool))]
+            ds = plan.new_scan_node(nf + di, [(0, I32), (2, STR)], [(1, lo, hi)])
+            cur = plan.new_join_node(False, cur, ds, pos, 0,
+                                     [(i, t) for i, t in enumerate(types)] + [(len(types) + 1, STR)])
+            types.append(STR)
+        plan.root = cur
+        plans.append(plan)
+    return plans

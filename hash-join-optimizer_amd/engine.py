@@ -116,6 +116,12 @@ class Context:
         the device, 0 = flat columns copied out and paged on the host; same bytes)"""
         self._check(self.lib.hj_set_option(self.handle, name.encode(), int(value)))
 
+    def cache_stats(self):
+        """directory cache: {"hits", "misses", "entries", "bytes"}"""
+        v = [C.c_uint64() for _ in range(4)]
+        self._check(self.lib.hj_dir_cache_stats(self.handle, *[C.byref(x) for x in v]))
+        return dict(zip(("hits", "misses", "entries", "bytes"), (x.value for x in v)))
+
     def profile(self, prefix):
         """time engine kernels whose name starts with prefix ("" = all, None = off)"""
         self._check(self.lib.hj_profile_enable(
@@ -206,7 +212,7 @@ class Server:
     query and returns a ticket, wait(ticket) returns (ColumnarTable, queued_ms, run_ms). Tables
     are uploaded with any Context and shared by every query."""
 
-    def __init__(self, slots=4, arena_bytes=0):
+    def __init__(self, slots=4, arena_bytes=0, budget_bytes=None, dir_cache_bytes=None):
         self.lib = abi.load()
         h = C.c_void_p()
         rc = self.lib.hj_server_create(int(slots), int(arena_bytes), C.byref(h))
@@ -214,10 +220,28 @@ class Server:
             raise EngineError(rc, self.lib.hj_global_error().decode())
         self.handle = h
         self._pending = {}
+        if budget_bytes is not None:
+            self.set_option("budget_bytes", budget_bytes)
+        if dir_cache_bytes is not None:
+            self.set_option("dir_cache_bytes", dir_cache_bytes)
 
     @property
     def slots(self):
         return int(self.lib.hj_server_slots(self.handle))
+
+    def set_option(self, name, value):
+        """"budget_bytes" (reservations of the running queries together), "dir_cache_bytes"
+        (the slots' shared directory cache); set before the first submit"""
+        rc = self.lib.hj_server_set_option(self.handle, name.encode(), int(value))
+        if rc:
+            raise EngineError(rc, f"server option {name}")
+
+    def cache_stats(self):
+        v = [C.c_uint64() for _ in range(4)]
+        rc = self.lib.hj_server_cache_stats(self.handle, *[C.byref(x) for x in v])
+        if rc:
+            raise EngineError(rc, "server cache stats")
+        return dict(zip(("hits", "misses", "entries", "bytes"), (x.value for x in v)))
 
     def submit(self, plan, dtables):
         sig = (len(plan.nodes), plan.root)

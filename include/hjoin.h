@@ -226,11 +226,13 @@ int hj_table_upload_cache(hj_context* ctx, const char* path, hj_dtable** out);
 
 /* ---- serving (no counterpart in the reference, whose execute() runs one query at a time): a
  * pool of `slots` contexts on the device, each with its own stream, arena (arena_bytes, 0 = 4 GiB)
- * and host thread. Submitted queries wait in one FIFO queue; a free slot takes the next, so
- * queries arriving while others run are admitted as slots free up and overlap on the GPU. A
- * query whose reserved memory (its scans' staging and build tables, and its output at the widest
- * scan's rows) exceeds a slot's arena is refused with HJ_E_BUSY. Tables are uploaded once with
- * any context of the device (hj_table_upload) and shared read-only by every query. ---- */
+ * and host thread, sharing one directory cache. Submitted queries wait in one FIFO queue; the
+ * first is admitted when a slot is free and its reserved memory (its scans' staging and build
+ * tables, and its output at the widest scan's rows) fits what the running queries leave of the
+ * server's budget, so queries arriving while others run are admitted as slots and memory free up
+ * and overlap on the GPU. A query whose reservation exceeds a slot's arena or the whole budget is
+ * refused with HJ_E_BUSY. Tables are uploaded once with any context of the device
+ * (hj_table_upload) and shared read-only by every query. ---- */
 typedef struct hj_server hj_server;
 int hj_server_create(uint32_t slots, uint64_t arena_bytes, hj_server** out);
 int hj_server_destroy(hj_server* server);
@@ -246,6 +248,17 @@ int hj_server_wait(hj_server* server, uint64_t ticket, hj_result* result, uint64
                    uint64_t* run_ns);
 uint32_t hj_server_slots(const hj_server* server);
 const char* hj_server_last_error(hj_server* server);
+/* server options, set before the first submit:
+ *   "budget_bytes"    bytes the running queries may reserve together (default: slots x arena);
+ *                     a query waits in the queue until its reservation fits what the running
+ *                     ones leave, and is refused (HJ_E_BUSY) only if it exceeds the whole budget
+ *                     or a slot's arena
+ *   "dir_cache_bytes" HBM for the slots' shared directory cache (default 1 GiB, 0: none) */
+int hj_server_set_option(hj_server* server, const char* name, int64_t value);
+/* the shared directory cache: lookups that found a table, lookups that did not, tables held and
+ * their bytes */
+int hj_server_cache_stats(hj_server* server, uint64_t* hits, uint64_t* misses, uint64_t* entries,
+                          uint64_t* bytes);
 
 /* ---- batched mode (no counterpart in the reference, whose execute() runs one plan at a time and
  * spreads one join's probe over threads by fetch_add on a 4096-row chunk counter,
@@ -306,8 +319,18 @@ int hj_batch_free(hj_context* ctx, hj_batch* b);
  *              into cleared tables.
  *   "gpu_pages" 1 (default) / 0: hj_execute and hj_dresult_to_host write the result pages on the
  *              device and copy them out; 0 copies the flat columns and pages them on the host.
- *              Both give the same bytes. ---- */
+ *              Both give the same bytes.
+ *   "dir_cache_bytes" N > 0: keep the tables the fused pipeline builds for joins keyed by the
+ *              source scan (a dimension's key column with its scan's filters and predicate
+ *              program) in a directory cache of N bytes of HBM, so that a later query over the
+ *              same resident table, key and filters probes the cached table instead of building
+ *              it (least recently used tables make room); 0 (default): no cache. A table freed
+ *              and uploaded again is a new table to the cache. ---- */
 int hj_set_option(hj_context* ctx, const char* name, int64_t value);
+/* the context's directory cache: lookups that found a table, lookups that did not, tables held
+ * and their bytes (all 0 without a cache) */
+int hj_dir_cache_stats(hj_context* ctx, uint64_t* hits, uint64_t* misses, uint64_t* entries,
+                       uint64_t* bytes);
 
 /* ---- kernel timing (no counterpart in the reference): when enabled, every engine kernel launch
  * whose name starts with `prefix` ("" = all, NULL = off) is bracketed by HIP events on the
