@@ -117,43 +117,19 @@ def m2m_workload(codec, scale):
 
 
 def serve_workload(codec, args):
-    """Serving: star joins (1-3 of the resident dimensions, 1-20 % range filters each) over a
-    fact table drawn log-uniformly in [100K, 36M] rows (the nearest of the resident facts:
-    100K, 1M, 4M, 16M, 36M rows), all tables resident. Measures queries/s of a batch submitted
-    at once and p50 / p99 latency of Poisson arrivals, for the plain one-query-at-a-time path
-    (one slot) and the served path (several slots)."""
-    rng = np.random.default_rng(47)
-    dims = dg.STAR_DIMS
+    """Serving: star joins over resident tables (datagen.serve_star / serve_queries): a fact
+    table whose prefixes of 100K, 1M, 4M, 16M and 36M rows are resident, with five FK columns
+    (Zipf(1.1), 3 % NULL) over the three dimensions of the 4-way configuration (2.5M / 4M / 100K
+    rows); each query joins 1-5 of the FKs (2- to 6-way) with range filters of 1-20 %
+    selectivity drawn from 16 per dimension (Zipf over the pool), fact prefix log-uniform in
+    [100K, 36M] rows."""
     fact_sizes = [100_000, 1_000_000, 4_000_000, 16_000_000, 36_000_000]
     t = time.time()
-    d = dg.star4(seed=47, n_fact=max(fact_sizes), dims=dims)
-    dim_tables = dg.star4_tables(d, codec)[1:]
-    facts = []
-    for n in fact_sizes:  # prefixes of the largest fact table
-        facts.append(hjo.ColumnarTable(n, [codec.encode_int32(fk[:n], v[:n])
-                                           for fk, v in zip(d["fact"], d["fact_valid"])]))
+    d = dg.serve_star(seed=47, n_fact=max(fact_sizes), dims=dg.STAR_DIMS)
+    tables = dg.serve_tables(d, fact_sizes, codec)
+    queries = dg.serve_queries(np.random.default_rng(49), args.queries, fact_sizes)
     log(f"serve tables generated+encoded in {time.time() - t:.1f}s")
-    I32, STR = hjo.DataType.INT32, hjo.DataType.VARCHAR
-    queries = []
-    for _ in range(args.queries):
-        n = float(np.exp(rng.uniform(np.log(100_000), np.log(36_000_000))))
-        fi = int(np.argmin([abs(np.log(n / f)) for f in fact_sizes]))
-        k = int(rng.integers(1, 4))
-        plan = hjo.Plan()
-        plan.inputs = [None] * (1 + len(fact_sizes) + len(dims))
-        cur = plan.new_scan_node(fi, [(0, I32), (1, I32), (2, I32)])
-        types = [I32, I32, I32]
-        for j in range(k):
-            w = int(rng.integers(100, 2001))
-            lo = int(rng.integers(0, 10_000 - w))
-            ds = plan.new_scan_node(len(fact_sizes) + j, [(0, I32), (2, STR)],
-                                    [(1, lo, lo + w - 1)])
-            cur = plan.new_join_node(False, cur, ds, j, 0,
-                                     [(i, ty) for i, ty in enumerate(types)] + [(len(types) + 1, STR)])
-            types.append(STR)
-        plan.root = cur
-        queries.append(plan)
-    return facts + dim_tables, queries
+    return tables, queries
 
 
 def batch_workload(codec, args):
@@ -268,30 +244,42 @@ def run_served(srv, dts, queries, rate=None, seed=0):
 
 
 def serve_main(args):
+    """the plain path (one slot, one query at a time, no directory cache) against the served
+    path (args.slots slots, shared directory cache, reserved-memory admission): a burst of all
+    queries at once (queries/s), then Poisson arrivals at each rate (p50 / p99 latency from
+    submission to the result in host memory)"""
     codec = hjo.Codec()
     tables, queries = serve_workload(codec, args)
     ctx = hjo.Context()
     dts = [ctx.upload(t) for t in tables]
+    rates = [float(r) for r in args.rates.split(",")]
     out = {"metric": "batched queries/s", "unit": "queries/s", "n_gpus": 1,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
            "data": "synthetic, seeded",
-           "config": {"workload": "served star joins: 1-3 resident dimensions (2.5M/4M/100K rows, "
-                                  "1-20 % range filters), fact 100K-36M rows log-uniform",
-                      "queries": len(queries), "slots": args.slots, "poisson_rate": args.rate}}
-    for name, slots in (("plain", 1), ("served", args.slots)):
-        with hjo.Server(slots=slots, arena_bytes=8 << 30) as srv:
-            run_served(srv, dts, queries[:8])  # warm-up
+           "config": {"workload": "served 2- to 6-way star joins over resident dimensions "
+                                  "(2.5M / 4M / 100K rows, 1-20 % range filters from a pool of 16 "
+                                  "per dimension), fact prefix 100K-36M rows log-uniform",
+                      "queries": len(queries), "slots": args.slots, "poisson_rates": rates,
+                      "results": "pages in host memory"}}
+    for name, slots, cache in (("plain", 1, 0), ("served", args.slots, 1 << 30)):
+        with hjo.Server(slots=slots, arena_bytes=16 << 30, dir_cache_bytes=cache) as srv:
+            run_served(srv, dts, queries[:8])  # warm-up (and, served, the first cached tables)
             wall, lat = run_served(srv, dts, queries)
             out[f"{name}_qps"] = len(queries) / wall
-            out["result_bytes"] = run_served.result_bytes
             out[f"{name}_batch_p50_ms"] = float(np.percentile(lat, 50))
             out[f"{name}_batch_p99_ms"] = float(np.percentile(lat, 99))
-            wall, lat = run_served(srv, dts, queries, rate=args.rate, seed=1)
-            out[f"{name}_poisson_p50_ms"] = float(np.percentile(lat, 50))
-            out[f"{name}_poisson_p99_ms"] = float(np.percentile(lat, 99))
-            out[f"{name}_poisson_qps"] = len(queries) / wall
-        log(f"{name} ({slots} slots): {out[name + '_qps']:.1f} q/s batched; Poisson {args.rate}/s "
-            f"p50 {out[name + '_poisson_p50_ms']:.2f} ms p99 {out[name + '_poisson_p99_ms']:.2f} ms")
+            out["result_bytes"] = run_served.result_bytes
+            for rate in rates:
+                wall, lat = run_served(srv, dts, queries, rate=rate, seed=1)
+                key = f"{name}_poisson_{int(rate)}"
+                out[key] = {"p50_ms": float(np.percentile(lat, 50)),
+                            "p99_ms": float(np.percentile(lat, 99)),
+                            "qps": len(queries) / wall}
+                log(f"{name} ({slots} slots) Poisson {rate:.0f}/s: p50 {out[key]['p50_ms']:.2f} ms "
+                    f"p99 {out[key]['p99_ms']:.2f} ms, {out[key]['qps']:.1f} q/s")
+            if cache:
+                out[f"{name}_dir_cache"] = srv.cache_stats()
+        log(f"{name} ({slots} slots): {out[name + '_qps']:.1f} q/s for a burst of {len(queries)}")
     out["value"] = out["served_qps"]
     out["speedup_vs_plain"] = out["served_qps"] / out["plain_qps"]
     for t_ in dts:
@@ -460,7 +448,7 @@ def main():
                     choices=["star4", "two_way", "m2m", "serve", "batch"])
     ap.add_argument("--queries", type=int, default=256, help="serve: queries per run")
     ap.add_argument("--slots", type=int, default=4, help="serve: contexts of the served path")
-    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--rates", default="50,200,500", help="serve: Poisson arrival rates per second")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reps", type=int, default=3, help="batch: timed repetitions (median)")

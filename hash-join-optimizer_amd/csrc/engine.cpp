@@ -29,6 +29,7 @@
 
 #include "../../include/hjoin.h"
 #include "hj_internal.hpp"
+#include "hostpool.hpp"
 #include "kernels.hpp"
 #include "pagecodec.hpp"
 #include "pagewrite.hpp"
@@ -1591,9 +1592,10 @@ void free_dresult(hj_context* ctx, hj_dresult* r) {
 }
 
 // Device page encoder: plan every column's pages (one wave per column), read the page counts,
-// then write the pages in HBM in batches of up to 32 MiB. Each batch goes to one half of a
-// pinned double buffer while the host copies the previous batch into the result columns'
-// malloc'd page blocks. Byte-identical to the host writer (pagecodec.cpp) on the same columns.
+// then write the pages in HBM in batches of up to 32 MiB, each batch copied by DMA straight into
+// the result columns' page blocks (pinned host memory recycled across results, hostpool.cpp);
+// stream order keeps a batch's stage until its copies are done. Byte-identical to the host
+// writer (pagecodec.cpp) on the same columns.
 void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
     constexpr uint64_t kStagePages = hj_context::kPageStageHalf / kPageBytes;
     const uint64_t n = r->num_rows;
@@ -1602,7 +1604,6 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
     out->num_columns = (uint32_t)ncols;
     out->columns = (hj_out_column*)std::calloc(ncols ? ncols : 1, sizeof(hj_out_column));
     if (!out->columns) throw std::bad_alloc();
-    ensure_page_stage(ctx);
     Arena& A = ctx->arena;
     const size_t mark = A.mark();
     try {
@@ -1644,7 +1645,7 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
                 oc.type = pb.c[i].type;
                 oc.num_pages = (uint32_t)cnt;
                 if (cnt) {
-                    oc.pages = (uint8_t*)std::malloc(cnt * HJ_PAGE_SIZE);
+                    oc.pages = host_pages_alloc(cnt * HJ_PAGE_SIZE);
                     if (!oc.pages) throw std::bad_alloc();
                 }
             }
@@ -1652,35 +1653,24 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
             const uint64_t stage = std::min(total, kStagePages);
             const uint64_t nbatch = (total + stage - 1) / stage;
             pb.pages = (uint8_t*)A.scratch(stage * kPageBytes);
-            auto issue = [&](uint64_t k) {
+            for (uint64_t k = 0; k < nbatch; ++k) {
                 const uint64_t p0 = k * stage, cnt = std::min(stage, total - p0);
                 {
                     Prof p_(ctx, "page_write");
                     page_write(pb, p0, cnt, ctx->stream);
                 }
-                HJ_HIP(hipMemcpyAsync(ctx->page_stage + (k & 1) * hj_context::kPageStageHalf,
-                                      pb.pages, cnt * kPageBytes, hipMemcpyDeviceToHost,
-                                      ctx->stream));
-                HJ_HIP(hipEventRecord(ctx->page_ev[k & 1], ctx->stream));
-            };
-            issue(0);
-            for (uint64_t k = 0; k < nbatch; ++k) {
-                // half (k + 1) & 1 was last read by the host copy of batch k - 1, finished above
-                if (k + 1 < nbatch) issue(k + 1);
-                HJ_HIP(hipEventSynchronize(ctx->page_ev[k & 1]));
-                const uint64_t p0 = k * stage, cnt = std::min(stage, total - p0);
-                const uint8_t* src = ctx->page_stage + (k & 1) * hj_context::kPageStageHalf;
-                std::vector<CopySeg> segs;
                 for (uint32_t i = 0; i < nc; ++i) {
                     const uint64_t a = std::max(p0, pb.c[i].first);
                     const uint64_t e =
                         std::min(p0 + cnt, pb.c[i].first + out->columns[c0 + i].num_pages);
                     if (a < e)
-                        segs.push_back({out->columns[c0 + i].pages + (a - pb.c[i].first) * kPageBytes,
-                                        src + (a - p0) * kPageBytes, (size_t)((e - a) * kPageBytes)});
+                        HJ_HIP(hipMemcpyAsync(
+                            out->columns[c0 + i].pages + (a - pb.c[i].first) * kPageBytes,
+                            pb.pages + (a - p0) * kPageBytes, (e - a) * kPageBytes,
+                            hipMemcpyDeviceToHost, ctx->stream));
                 }
-                parallel_copy(segs);
             }
+            HJ_HIP(hipStreamSynchronize(ctx->stream));
             A.reset(mark);
         }
     } catch (...) {
@@ -2047,7 +2037,7 @@ int hj_dresult_free(hj_context* ctx, hj_dresult* r) {
 
 void hj_result_free(hj_result* r) {
     if (!r || !r->columns) return;
-    for (uint32_t c = 0; c < r->num_columns; ++c) std::free(r->columns[c].pages);
+    for (uint32_t c = 0; c < r->num_columns; ++c) host_pages_free(r->columns[c].pages);
     std::free(r->columns);
     r->columns = nullptr;
     r->num_columns = 0;
@@ -2092,7 +2082,7 @@ int hj_encode_varchar(const uint64_t* offsets, const char* chars, const uint8_t*
 
 void hj_out_column_free(hj_out_column* col) {
     if (!col) return;
-    std::free(col->pages);
+    host_pages_free(col->pages);
     col->pages = nullptr;
     col->num_pages = 0;
 }

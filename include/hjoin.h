@@ -127,7 +127,9 @@ typedef struct hj_plan {
     uint64_t root;
 } hj_plan;
 
-/* Result column/table: pages are one contiguous malloc'd block of num_pages*HJ_PAGE_SIZE bytes. */
+/* Result column/table: pages are one contiguous block of num_pages*HJ_PAGE_SIZE bytes of host
+ * memory, owned by the library: release it with hj_result_free / hj_out_column_free only (result
+ * pages are pinned blocks that the library recycles for later results). */
 typedef struct hj_out_column {
     int32_t type;
     uint32_t num_pages;
