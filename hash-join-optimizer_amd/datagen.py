@@ -339,3 +339,139 @@ def serve_queries(rng, n, fact_sizes, n_dims=3, pool=16, max_joins=5):
         plan.root = cur
         plans.append(plan)
     return plans
+
+
+# ------------------------------------------------------------------ Join Order Benchmark schema
+JOB_ROWS = {"title": 500, "name": 600, "char_name": 400, "company_name": 200, "keyword": 300,
+            "aka_name": 400, "aka_title": 300, "cast_info": 2500, "movie_companies": 1200,
+            "movie_info": 2000, "movie_info_idx": 1000, "movie_keyword": 1500,
+            "movie_link": 300, "complete_cast": 400, "person_info": 1200, "comp_cast_type": 4,
+            "company_type": 4, "info_type": 113, "kind_type": 7, "link_type": 18,
+            "role_type": 12}
+JOB_FK = {"movie_id": "title", "linked_movie_id": "title", "episode_of_id": "title",
+          "person_id": "name", "person_role_id": "char_name", "company_id": "company_name",
+          "keyword_id": "keyword", "info_type_id": "info_type", "kind_id": "kind_type",
+          "role_id": "role_type", "company_type_id": "company_type",
+          "link_type_id": "link_type", "subject_id": "comp_cast_type",
+          "status_id": "comp_cast_type"}
+JOB_INT_RANGE = {"production_year": (1900, 2021), "season_nr": (1, 30), "episode_nr": (1, 100),
+                 "nr_order": (1, 50)}
+_JOB_ALPHA = np.frombuffer(b"abcdefghijklmnopqrstuvwxyz ()[]-'0123456789ABCDEFGHIJ", np.uint8)
+
+
+def _job_strings(rng, n, lo=1, hi=12):
+    lens = rng.integers(lo, hi + 1, n)
+    chars = rng.choice(_JOB_ALPHA, int(lens.sum()))
+    cuts = np.concatenate([[0], np.cumsum(lens)])
+    blob = bytes(chars)
+    return [blob[cuts[i]:cuts[i + 1]].decode() for i in range(n)]
+
+
+def _job_instantiate(rng, pattern):
+    """a string that LIKE `pattern` matches: '%' becomes 0-3 random characters, '_' one"""
+    out = []
+    for ch in pattern:
+        if ch == "%":
+            out.append(_job_strings(rng, 1, 0, 3)[0])
+        elif ch == "_":
+            out.append(_job_strings(rng, 1, 1, 1)[0])
+        else:
+            out.append(ch)
+    return "".join(out)
+
+
+def job_tables(schema_text, queries, seed=70, scale=1.0, null_p=0.1, hot_p=0.85, cold_p=0.2,
+               fk_hot_p=0.6):
+    """Synthetic tables of the Join Order Benchmark schema (CREATE TABLE text) for the queries
+    [(name, sql, plan)]: {table: (num_rows, [column])} for the tables they read, an INT32 column
+    a pair (int32 values, validity bool array), a VARCHAR column a list of str / None.
+
+    id columns are 1..n. A table's first rows are its hot rows (a fifth of a large table; in a
+    type table of at most 200 rows, as many rows as the queries name literals for it): a hot row
+    takes, column by column with probability hot_p (cold_p for other rows), one of the literals
+    the queries compare that column with (LIKE patterns instantiated), else a random value (an
+    INT32 uniform over JOB_INT_RANGE or [0, 10^6), a string of 1-12 characters); a type table's
+    row i takes the i-th literal. Foreign keys (JOB_FK by column name) point at a hot row with
+    probability fk_hot_p, else uniformly at any row, so that filtered rows join. Columns declared
+    without NOT NULL are NULL with probability null_p. Row counts are JOB_ROWS x scale (the type
+    tables keep theirs)."""
+    from . import jobfront
+    schema = jobfront.parse_schema(schema_text)
+    nullable = jobfront.nullable_columns(schema_text)
+    pools, used = {}, []
+    for _, sql, _ in queries:
+        parsed = jobfront.ParsedSQL(sql, schema)
+        for t, _ in parsed.entities:
+            if t not in used:
+                used.append(t)
+        for k, v in jobfront.literals(parsed).items():
+            pools.setdefault(k, []).extend(v)
+    rows = {t: (n if n <= 200 else max(1, int(n * scale))) for t, n in JOB_ROWS.items()}
+    rng = np.random.default_rng(seed)
+    bank = _job_strings(rng, 4096)
+    choices = {}  # (table, column) -> literal values a row may take
+    for (t, c), pool in pools.items():
+        ch = []
+        for x in dict.fromkeys(pool):
+            if isinstance(x, tuple):
+                ch.extend(_job_instantiate(rng, x[1]) for _ in range(4))
+            else:
+                ch.append(x)
+        choices[(t, c)] = ch
+    hot = {}
+    for t, n in rows.items():
+        if n <= 200:
+            k = max([len(v) for (tt, _), v in choices.items() if tt == t] or [1])
+            hot[t] = max(1, min(n, k))
+        else:
+            hot[t] = max(1, n // 5)
+    out = {}
+    for t in sorted(used, key=list(schema).index):
+        n = rows[t]
+        is_hot = np.arange(n) < hot[t]
+        data = []
+        for c, ty in schema[t]:
+            ch = choices.get((t, c), [])
+            nul = (rng.random(n) < null_p) if (t, c) in nullable else np.zeros(n, bool)
+            pick = rng.random(n) < np.where(is_hot, hot_p, cold_p)
+            if ty == I32:
+                if c == "id":
+                    v = np.arange(1, n + 1, dtype=np.int64)
+                elif c in JOB_FK:
+                    tgt = JOB_FK[c]
+                    v = np.where(rng.random(n) < fk_hot_p, rng.integers(1, hot[tgt] + 1, n),
+                                 rng.integers(1, rows[tgt] + 1, n))
+                else:
+                    lo, hi = JOB_INT_RANGE.get(c, (0, 1_000_000))
+                    v = rng.integers(lo, hi, n)
+                    ints = [x for x in ch if isinstance(x, int)]
+                    if ints:
+                        v[pick] = np.asarray(ints)[rng.integers(0, len(ints), int(pick.sum()))]
+                data.append((v.astype(np.int32), ~nul))
+                continue
+            strs = [x if isinstance(x, str) else str(x) for x in ch]
+            if n <= 200:
+                vals = [strs[i] if i < len(strs) else bank[(i * 7919) % len(bank)]
+                        for i in range(n)]
+            else:
+                ci = rng.integers(0, max(1, len(strs)), n)
+                bi = rng.integers(0, len(bank), n)
+                vals = [strs[ci[i]] if (pick[i] and strs) else bank[bi[i]] for i in range(n)]
+            data.append([None if nul[i] else vals[i] for i in range(n)])
+        out[t] = (n, data)
+    return out
+
+
+def job_columnar(tables, codec=None):
+    """{table: ColumnarTable} of job_tables' output"""
+    codec = codec or Codec()
+    out = {}
+    for t, (n, cols) in tables.items():
+        enc = []
+        for c in cols:
+            if isinstance(c, tuple):
+                enc.append(codec.encode_int32(c[0], c[1].astype(np.uint8)))
+            else:
+                enc.append(codec.encode_strings(c))
+        out[t] = ColumnarTable(n, enc)
+    return out
