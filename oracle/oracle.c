@@ -179,13 +179,15 @@ static inline int32_t rd32(const uint8_t* p) {
 static inline int bit_get(const uint8_t* bm, uint32_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
 
 /* ------------------------------------------------------------------ column_t */
-/* include/column_t.h:16-82. Pages of 1024 value_t are a storage detail; a growable array holds
- * the same values. Direct mode reads a dense INT32 input column in place, page row/1984. */
+/* include/column_t.h:16-82: values in pages of 1024 value_t (a page is allocated by the append
+ * that finds the last one full); direct mode reads a dense INT32 input column in place, at page
+ * row / 1984, slot row % 1984. */
+#define COLUMN_PAGE_LEN 1024u /* include/column_t.h:11-12 */
 typedef struct {
     uint64_t rows;
     int32_t type;
-    uint64_t* data;
-    uint64_t cap;
+    uint64_t** pages;
+    uint64_t npages, cap_pages;
     int direct;
     const hj_column* in;
 } vcol;
@@ -195,16 +197,23 @@ static void vcol_init(vcol* c, int32_t type) {
     c->type = type;
 }
 static void vcol_free(vcol* c) {
-    free(c->data);
-    c->data = NULL;
-    c->cap = c->rows = 0;
+    for (uint64_t i = 0; i < c->npages; ++i) free(c->pages[i]);
+    free(c->pages);
+    c->pages = NULL;
+    c->npages = c->cap_pages = c->rows = 0;
 }
+/* column_t::append_row, include/column_t.h:30-36 */
 static inline void vcol_append(vcol* c, uint64_t v) {
-    if (c->rows == c->cap) {
-        c->cap = c->cap ? c->cap * 2 : 1024;
-        c->data = (uint64_t*)xrealloc(c->data, c->cap * sizeof(uint64_t));
+    const uint64_t slot = c->rows & (COLUMN_PAGE_LEN - 1);
+    if (slot == 0) {
+        if (c->npages == c->cap_pages) {
+            c->cap_pages = c->cap_pages ? c->cap_pages * 2 : 16;
+            c->pages = (uint64_t**)xrealloc(c->pages, c->cap_pages * sizeof(uint64_t*));
+        }
+        c->pages[c->npages++] = (uint64_t*)xcalloc(COLUMN_PAGE_LEN, sizeof(uint64_t));
     }
-    c->data[c->rows++] = v;
+    c->pages[c->npages - 1][slot] = v;
+    c->rows++;
 }
 /* column_t::get_row_value, include/column_t.h:37-66 */
 static inline uint64_t vcol_get(const vcol* c, uint64_t row) {
@@ -213,7 +222,50 @@ static inline uint64_t vcol_get(const vcol* c, uint64_t row) {
         const uint8_t* pg = c->in->pages[p];
         return oracle_value_int32(rd32(pg + 4 + 4 * o));
     }
-    return c->data[row];
+    return c->pages[row >> 10][row & (COLUMN_PAGE_LEN - 1)];
+}
+/* column_t::use_input_col, include/column_t.h:76-80 */
+static void vcol_use_input(vcol* c, const hj_column* in, uint64_t rows) {
+    c->direct = 1;
+    c->in = in;
+    c->rows = rows;
+}
+
+/* merge_column_pages, include/column_merge.h:7-31: an empty destination takes the source's
+ * pages, otherwise the source's rows are appended one by one */
+static void vcol_merge(vcol* dst, vcol* src) {
+    if (!src->direct && dst->rows == 0 && dst->npages == 0 && !dst->direct) {
+        vcol_free(dst);
+        dst->pages = src->pages;
+        dst->npages = src->npages;
+        dst->cap_pages = src->cap_pages;
+        dst->rows = src->rows;
+        src->pages = NULL;
+        src->npages = src->cap_pages = src->rows = 0;
+        return;
+    }
+    for (uint64_t i = 0; i < src->rows; ++i) vcol_append(dst, vcol_get(src, i));
+}
+
+/* column_t exported for its unit tests (tests/columnt_tests.cpp) */
+void* oracle_vcol_new(int32_t type) {
+    vcol* c = (vcol*)xcalloc(1, sizeof(vcol));
+    vcol_init(c, type);
+    return c;
+}
+void oracle_vcol_append(void* c, uint64_t v) { vcol_append((vcol*)c, v); }
+uint64_t oracle_vcol_get(const void* c, uint64_t row) { return vcol_get((const vcol*)c, row); }
+uint64_t oracle_vcol_rows(const void* c) { return ((const vcol*)c)->rows; }
+uint64_t oracle_vcol_pages(const void* c) { return ((const vcol*)c)->npages; }
+int32_t oracle_vcol_type(const void* c) { return ((const vcol*)c)->type; }
+void oracle_vcol_use_input(void* c, const hj_column* in, uint64_t rows) {
+    vcol_use_input((vcol*)c, in, rows);
+}
+void oracle_vcol_merge(void* dst, void* src) { vcol_merge((vcol*)dst, (vcol*)src); }
+void oracle_vcol_free(void* c) {
+    if (!c) return;
+    vcol_free((vcol*)c);
+    free(c);
 }
 
 /* ------------------------------------------------------------------ copy_scan */
@@ -252,9 +304,7 @@ static void* scan_worker(void* arg) {
         const hj_column* col = &t->table->columns[in_idx];
         vcol* res = &t->out[ci];
         if (col->type == HJ_INT32 && col_is_dense(col)) {
-            res->direct = 1; /* column_t::use_input_col */
-            res->in = col;
-            res->rows = t->table->num_rows;
+            vcol_use_input(res, col, t->table->num_rows);
             continue;
         }
         for (uint32_t p = 0; p < col->num_pages; ++p) {
@@ -643,14 +693,7 @@ static vcol* join_run(int build_left, vcol* left, uint64_t nleft, vcol* right, u
         vcol* co = chunk_out[cid];
         if (!co) continue;
         for (uint64_t c = 0; c < nout; ++c) {
-            if (results[c].rows == 0 && results[c].data == NULL) {
-                results[c].data = co[c].data; /* first chunk: pages are moved */
-                results[c].rows = co[c].rows;
-                results[c].cap = co[c].cap;
-                co[c].data = NULL;
-            } else {
-                for (uint64_t i = 0; i < co[c].rows; ++i) vcol_append(&results[c], co[c].data[i]);
-            }
+            vcol_merge(&results[c], &co[c]);
             vcol_free(&co[c]);
         }
         free(co);

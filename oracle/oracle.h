@@ -27,6 +27,18 @@ uint64_t oracle_value_strref(uint64_t ref);
 /* returns 0 and the packed ref, or HJ_E_RANGE on field overflow */
 int oracle_strref_encode(int is_long, uint64_t table, uint64_t col, uint64_t page, uint64_t off,
                          uint64_t* out);
+
+/* column_t (include/column_t.h:16-82, include/column_merge.h:7-31), for its unit tests:
+ * pages of 1024 value_t, direct mode over a dense INT32 input column */
+void* oracle_vcol_new(int32_t type);
+void oracle_vcol_append(void* c, uint64_t value);
+uint64_t oracle_vcol_get(const void* c, uint64_t row);
+uint64_t oracle_vcol_rows(const void* c);
+uint64_t oracle_vcol_pages(const void* c);
+int32_t oracle_vcol_type(const void* c);
+void oracle_vcol_use_input(void* c, const hj_column* in, uint64_t rows);
+void oracle_vcol_merge(void* dst, void* src);
+void oracle_vcol_free(void* c);
 uint64_t oracle_splitmix64(uint64_t x);
 
 /* Contest::execute (src/execute.cpp:598-608). Filters on scan nodes are applied first, outside
