@@ -2,7 +2,8 @@
 dimensions 2.5M / 4M / 100K rows with range filters pushed into their scans, VARCHAR output)
 on one MI355X, inputs resident in HBM, output materialized in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload star4|two_way]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload star4|two_way|m2m|batch|serve|job]
 
 Prints one JSON line. value = join tuples/s = sum over joins of (|build_j| + |probe_j|) / step
 time; pct_hbm_peak = algorithmic bytes / step time / 8 TB/s (keys and filter attributes read once,
@@ -288,6 +289,82 @@ def serve_main(args):
     print(json.dumps(out), flush=True)
 
 
+def job_main(args):
+    """the Join Order Benchmark's 113 queries (SQL + PostgreSQL plans, tests/golden) through the
+    front-end on synthetic tables of the JOB schema (datagen.job_tables, one data set for all
+    queries, JOB_ROWS x scale rows): the total of the per-query times, resident tables with
+    results left in HBM (hj_execute_device, the steps) and the drop-in hj_execute (host pages in,
+    result pages out); filters run inside each call as the scans' predicate programs"""
+    jf = importlib.import_module("hash-join-optimizer_amd.jobfront")
+    jr = importlib.import_module("hash-join-optimizer_amd.jobrun")
+    scale = args.scale if args.scale != 1.0 else 200.0
+    schema_text, queries = jf.load_fixture(os.path.join(ROOT, "tests", "golden", "job_queries.json"))
+    schema = jf.parse_schema(schema_text)
+    codec = hjo.Codec()
+    t = time.time()
+    raw = dg.job_tables(schema_text, queries, seed=70, scale=scale)
+    tables = dg.job_columnar(raw, codec)
+    nrows = sum(n for n, _ in raw.values())
+    log(f"JOB tables: {nrows} rows at scale {scale} generated in {time.time() - t:.1f}s")
+    t0 = time.perf_counter()
+    ctx = hjo.Context()
+    ctx_ms = (time.perf_counter() - t0) * 1e3
+    dts = {k: ctx.upload(c) for k, c in tables.items()}
+    quiet = lambda m: None  # noqa: E731
+    for _ in range(args.warmup):
+        jr.run(ctx, queries, schema, tables, dts, log=quiet)
+    totals, per_query = [], None
+    for _ in range(args.steps):
+        ms, _, per_query = jr.run(ctx, queries, schema, tables, dts, log=quiet)
+        totals.append(ms)
+    step_ms = statistics.median(totals)
+    dropin = []
+    for _ in range(max(1, min(3, args.steps))):
+        ms, _, dq = jr.run(ctx, queries, schema, tables, host_pages=True, log=quiet)
+        dropin.append(ms)
+    gpu_rows = {name: rows for name, _, rows, _ in per_query}
+    assert all(gpu_rows[n] == r for n, _, r, _ in dq), "hj_execute and hj_execute_device differ"
+    slow = sorted(per_query, key=lambda x: -x[1])[:5]
+    log(f"JOB resident: {step_ms:.2f} ms total; slowest {[(n, round(m, 3)) for n, m, _, _ in slow]}")
+    log(f"JOB drop-in hj_execute: {statistics.median(dropin):.2f} ms total")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import hjtest  # test infrastructure: loads oracle/liboracle.so
+        o = hjtest.Oracle()
+        threads = cpu_threads()
+        ns_tot = 0
+        for name, sql, pg in queries:
+            plan, order, _ = jf.translate(sql, pg, schema, tables)
+            ct, ns = o.execute(plan, threads=threads, timed=True)
+            ns_tot += ns
+            assert ct.num_rows == gpu_rows[name], f"{name}: {gpu_rows[name]} rows, oracle {ct.num_rows}"
+        cpu = {"value": len(queries) / (ns_tot / 1e9), "unit": "queries/s", "cores": threads,
+               "kind": "port", "ms": ns_tot / 1e6,
+               "sample": "all 113 queries once, sum of the oracle's execute() times (scan filters "
+                         "applied before its timer, as the reference filters at load time); "
+                         "every query's row count equals the GPU's",
+               "host": host_cpu()}
+        log(f"JOB oracle execute(): {ns_tot / 1e6:.1f} ms total on {threads} threads")
+    out = {"metric": "JOB queries/s", "value": len(queries) / (step_ms / 1e3), "unit": "queries/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic JOB-schema tables (datagen.job_tables, seed 70); the queries and "
+                   "plans are the benchmark's own",
+           "config": {"workload": "Join Order Benchmark, 113 queries, one step = all of them "
+                                  "one after another over resident tables",
+                      "scale": scale, "table_rows": nrows, "queries": len(queries),
+                      "output_rows": int(sum(gpu_rows.values()))},
+           "total_ms_resident": step_ms, "total_ms_hj_execute": statistics.median(dropin),
+           "context_ms": ctx_ms,
+           "slowest_queries_ms": {n: m for n, m, _, _ in slow},
+           "roofline": None, "cpu_baseline": cpu}
+    for d in dts.values():
+        d.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 # ------------------------------------------------------------------ kernel byte model
 def kernel_bytes(name, info, launches):
     """algorithmic HBM bytes of all launches of one step of kernel `name` (None: no model)"""
@@ -445,7 +522,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="star4",
-                    choices=["star4", "two_way", "m2m", "serve", "batch"])
+                    choices=["star4", "two_way", "m2m", "serve", "batch", "job"])
     ap.add_argument("--queries", type=int, default=256, help="serve: queries per run")
     ap.add_argument("--slots", type=int, default=4, help="serve: contexts of the served path")
     ap.add_argument("--rates", default="50,200,500", help="serve: Poisson arrival rates per second")
@@ -478,6 +555,9 @@ def main():
         return
     if args.workload == "batch":
         batch_main(args)
+        return
+    if args.workload == "job":
+        job_main(args)
         return
     codec = hjo.Codec()
     if args.workload == "star4":

@@ -42,3 +42,26 @@ def test_job_family_matches_oracle(hjo, oracle, family):
             assert got == want, name
         for t in dts.values():
             t.free()
+
+
+def test_job_driver_runs_all_queries(hjo, oracle, capsys):
+    """jobrun.run (the reference's run(), read_sql.cpp:1313-1373) over all 113 queries on one
+    data set, resident and drop-in, every result checked against the oracle's"""
+    jr = importlib.import_module("hash-join-optimizer_amd.jobrun")
+    cols = dg.job_columnar(dg.job_tables(SCHEMA_TEXT, QUERIES, seed=71, scale=5.0), oracle.codec)
+
+    def check(plan, res):
+        ct = res.to_host() if hasattr(res, "to_host") else res
+        return oracle.rows(ct) == oracle.rows(oracle.execute(plan))
+
+    with hjo.Context(arena_bytes=2 << 30) as ctx:
+        dts = {t: ctx.upload(c) for t, c in cols.items()}
+        total, ok, per = jr.run(ctx, QUERIES, SCHEMA, cols, dts, check=check)
+        assert ok and len(per) == 113 and total > 0
+        assert sum(r for _, _, r, _ in per) > 0
+        total2, ok2, per2 = jr.run(ctx, QUERIES[:20], SCHEMA, cols, check=check, host_pages=True)
+        assert ok2 and [r for _, _, r, _ in per2] == [r for _, _, r, _ in per[:20]]
+        for t in dts.values():
+            t.free()
+    out = capsys.readouterr().out
+    assert "Query 1a >> \t\t Runtime: " in out and "Result correct: true" in out
