@@ -40,8 +40,8 @@ __device__ inline IdxCol uidx_col(const DCol& c, uint32_t rows) {
 }
 
 // The rare continuations of a lookup (the home bucket is full) and of an insert (the home slot
-// is taken) are calls, not inlined copies: the persistent kernel holds every item kind, and one
-// inlined copy per unrolled row group would set its register count for all of them.
+// is taken) are calls, not inlined copies: one inlined copy per unrolled row group would set the
+// kernel's register count for its common path too.
 __device__ __noinline__ uint32_t lookup_rest(const HJ_GLOBAL uint32_t* K, uint32_t k, uint32_t hb,
                                              uint32_t kk) {
     const uint32_t smask = seg_mask(k);

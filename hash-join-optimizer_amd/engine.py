@@ -286,12 +286,12 @@ class Server:
 
 
 class Batch:
-    """Many independent queries over resident tables in one persistent launch (hj_batch_*).
+    """Many independent queries over resident tables in four shared launches (hj_batch_*).
 
     queries: list of (plan, dtables). params: None (the engine's defaults), a dict of
     hj_batch_params fields, or "tuned" (the committed autotuner table, tuning/batch_mi355x.json).
     run() executes every query; result(i) returns query i's rows as a ColumnarTable;
-    info(i) -> (ran in the persistent launch, rows)."""
+    info(i) -> (ran in the batch's launches, rows)."""
 
     def __init__(self, ctx, queries, params=None):
         self.ctx = ctx

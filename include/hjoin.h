@@ -264,11 +264,12 @@ int hj_server_cache_stats(hj_server* server, uint64_t* hits, uint64_t* misses, u
 
 /* ---- batched mode (no counterpart in the reference, whose execute() runs one plan at a time and
  * spreads one join's probe over threads by fetch_add on a 4096-row chunk counter,
- * src/execute.cpp:87-117): many independent queries over resident tables run in ONE persistent
- * launch. Their work (group indexes of the columns they read, build inserts, probe chunks) is one
- * list of items that the launch's workgroups take in order from a device work queue; a probe item
- * waits until its query's inserts are done. The launch is captured once as a HIP graph and
- * replayed by every hj_batch_run. A query is batched when it is a left-deep chain whose build
+ * src/execute.cpp:87-117): many independent queries over resident tables run in four launches
+ * shared by all of them: counters and tables cleared, group indexes of the columns they read,
+ * build inserts, probes. Each launch is one list of work items (of every query, interleaved)
+ * that its workgroups walk grid-stride, with no wait between workgroups: the launch order is
+ * what makes a query's inserts complete before its probes. The four launches are captured once
+ * as a HIP graph and replayed by every hj_batch_run. A query is batched when it is a left-deep chain whose build
  * sides are base scans keyed by INT32 columns of the chain's source scan (a star join), with
  * range filters at most and INT32 outputs; every other query (and a batched one whose build keys
  * repeat) runs on its own through hj_execute_device after the launch. Results are the same either
@@ -297,7 +298,7 @@ typedef struct hj_batch_params {
 int hj_batch_create(hj_context* ctx, uint32_t n, const hj_plan* const* plans,
                     const hj_dtable* const* const* tables, const hj_batch_params* params,
                     hj_batch** out);
-/* run every query of the batch (the persistent launch, then the queries that run on their own);
+/* run every query of the batch (the four launches, then the queries that run on their own);
  * returns when every result is complete. Results of the previous run are replaced. */
 int hj_batch_run(hj_context* ctx, hj_batch* b);
 uint32_t hj_batch_size(const hj_batch* b);
