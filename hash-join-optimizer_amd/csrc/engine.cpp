@@ -1084,7 +1084,26 @@ class Executor {
             gpb = (gpb + per - 1) / per * per;
             hp.probe_gpb = (uint32_t)gpb;
             hp.probe_blocks = (uint32_t)std::max<uint64_t>(1, (groups + gpb - 1) / gpb);
-            const uint64_t cap = (uint64_t)hp.probe_blocks * gpb * 64;
+            uint64_t cap = (uint64_t)hp.probe_blocks * gpb * 64;
+            // one join over a table larger than an XCD's L2 (more than 2^18 build rows: at
+            // least 2^20 key slots, 8 MiB): the sliced probe, one table slice per XCD
+            const uint64_t nbp = (nsrc + kPartRows - 1) / kPartRows;
+            if (ctx_->xcd_slices && hp.nstream == 1 && hp.ndep == 0 && emit &&
+                stage_rows[hp.order[0]] > (1u << 18) && nbp >= 8 &&
+                nbp <= (uint64_t)(kEmitMaxBlocks / 8) * kSliceMaxK) {
+                const uint64_t chunks0 = std::min<uint64_t>(kEmitMaxBlocks / 8, nbp);
+                const uint64_t K = (nbp + chunks0 - 1) / chunks0;
+                const uint64_t chunks = (nbp + K - 1) / K;
+                hp.xslices = 8;
+                hp.part_blocks = (uint32_t)nbp;
+                hp.part_k = (uint32_t)K;
+                hp.probe_blocks = (uint32_t)(8 * chunks);
+                cap = nbp * kPartRows;
+                hp.ptk = scratch<uint32_t>(cap);
+                hp.ptr = scratch<uint32_t>(cap);
+                hp.pcnt = scratch<uint32_t>(nbp * 8);
+                hp.blk_origin = scratch<uint32_t>(hp.probe_blocks);
+            }
             for (int q = 0; q <= hp.nstream; ++q) hp.stage[q] = scratch<uint32_t>(cap);
             hp.blk_cnt = scratch<uint32_t>(hp.probe_blocks);
             hp.blk_off = scratch<uint32_t>(hp.probe_blocks);
@@ -1229,6 +1248,7 @@ class Executor {
         ed.nstr = nstr;
         ed.blocks = hp.probe_blocks;
         ed.rows_per_block = hp.probe_gpb * 64;
+        ed.blk_origin = hp.xslices ? hp.blk_origin : nullptr;
         ed.blk_cnt = hp.blk_cnt;
         for (int q = 0; q <= hp.nstream; ++q) ed.stage[q] = hp.stage[q];
         ed.meta = hp.meta;
@@ -1844,6 +1864,10 @@ int hj_set_option(hj_context* ctx, const char* name, int64_t value) {
     }
     if (std::strcmp(name, "gpu_pages") == 0) {
         ctx->gpu_pages = value != 0;
+        return HJ_OK;
+    }
+    if (std::strcmp(name, "xcd_slices") == 0) {
+        ctx->xcd_slices = value != 0;
         return HJ_OK;
     }
     if (std::strcmp(name, "dir_cache_bytes") == 0) {

@@ -201,6 +201,7 @@ struct hj_context {
     bool use_pipeline = true;  // fused probe pipeline for left-deep chains (hj_set_option)
     bool part_build = true;    // pipeline builds partitioned in LDS (hj_set_option)
     bool gpu_pages = true;     // result pages written on the device (hj_set_option)
+    bool xcd_slices = true;    // one-join pipelines over large tables: per-XCD table slices
     // pinned double buffer for result pages on their way to host memory (allocated on first use)
     uint8_t* page_stage = nullptr;
     hipEvent_t page_ev[2] = {nullptr, nullptr};

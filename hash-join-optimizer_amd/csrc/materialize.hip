@@ -164,6 +164,7 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
     __shared__ uint64_t s_part[kB / HJ_WAVE][KP];
     __shared__ uint64_t s_sum[KP];
     __shared__ uint32_t s_rb[kMaxGroup + 1];
+    __shared__ uint32_t s_org[kMaxGroup];  // first stage row of each block of the group
     const uint32_t t = threadIdx.x, G = d.group;
     const uint32_t j0 = blockIdx.x * G;
     const uint32_t j1 = j0 + G < d.blocks ? j0 + G : d.blocks;
@@ -189,7 +190,11 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
             acc[h] += r[u];
             acc[2 + h] += v[u];
         }
-        if (t < G) s_rb[1 + t] = j0 + t < j1 ? gp(d.blk_cnt)[j0 + t] : 0u;
+        if (t < G) {
+            s_rb[1 + t] = j0 + t < j1 ? gp(d.blk_cnt)[j0 + t] : 0u;
+            s_org[t] = d.blk_origin ? (j0 + t < j1 ? gp(d.blk_origin)[j0 + t] : 0u)
+                                    : (j0 + t) * d.rows_per_block;
+        }
         block_sums<KP>(acc, s_part, s_sum);
         if (t == 0) {
             s_rb[0] = 0;
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
                 const uint32_t f = f0 + u * kB + t;
                 uint32_t jj = 0;
                 for (uint32_t x = 1; x < G; ++x) jj = f >= s_rb[x] ? x : jj;
-                rid[u] = f < R ? rids[(uint64_t)(j0 + jj) * d.rows_per_block + (f - s_rb[jj])] : 0u;
+                rid[u] = f < R ? rids[(uint64_t)s_org[jj] + (f - s_rb[jj])] : 0u;
             }
 #pragma unroll
             for (int u = 0; u < kIntRows; ++u) {
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
         uint32_t jj = 0;
         for (uint32_t x = 1; x < G; ++x) jj = f >= s_rb[x] ? x : jj;
         const uint64_t o = off + f;
-        const uint32_t rid = have ? rids[(uint64_t)(j0 + jj) * d.rows_per_block + (f - s_rb[jj])] : 0u;
+        const uint32_t rid = have ? rids[(uint64_t)s_org[jj] + (f - s_rb[jj])] : 0u;
         if (!str) {
             if (!have) continue;
             int32_t v = 0;

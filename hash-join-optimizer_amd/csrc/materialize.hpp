@@ -80,6 +80,7 @@ struct EmitDesc {
     uint32_t bytes_stride;       // entries per block in blk_bytes
     uint64_t cap_rows;
     const uint32_t* blk_cnt;     // survivors per block of the fused pass
+    const uint32_t* blk_origin;  // first stage row of each block (null: block * rows_per_block)
     const uint64_t* blk_bytes;   // string bytes per block of the fused pass and VARCHAR column
     const uint32_t* stage[kMaxStageArrays];
     const uint32_t* meta;        // pipeline counters: [0] != 0 voids the pass (nothing written)

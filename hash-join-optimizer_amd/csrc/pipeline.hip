@@ -828,6 +828,248 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
         gp(pd.blk_bytes)[(uint64_t)blockIdx.x * kMaxOutStr + t] = s_bytes[t];
 }
 
+// ---------------------------------------------------------------- sliced probe
+// One streamed join over a table larger than an XCD's L2 (the two-way shape): the table's 8
+// slices (the top 3 bits of the key hash, i.e. of the home bucket) are probed by the XCDs one
+// each. Workgroups are dispatched to the 8 XCDs in turn (block b runs on XCD b % 8), so a probe
+// block b with b % 8 == x reads only slice x, which stays in that XCD's L2.
+//   k_sp_part: block p takes the source rows [p * kPartRows, (p + 1) * kPartRows), keeps the
+//     live ones (valid key, in range, source filters) and writes their (key, row) pairs to the
+//     same range ordered by slice, with the 8 slice counts in pcnt[p * 8 ..].
+//   k_sp_probe_sl: block b = c * 8 + x takes slice x of the partition blocks
+//     [c * part_k, (c + 1) * part_k) (one run each), looks the keys up and stages the survivors
+//     from blk_origin[b] on: the position of slice x in that chunk of rows if its pairs were
+//     ordered by slice, which has room for every pair of the run set.
+constexpr uint32_t kPartGroupsPerWave = kPartRows / 64 / kNW;
+constexpr int kPartU = 4;  // groups per wave and round of the partition pass
+static_assert(kPartGroupsPerWave % kPartU == 0, "partition rounds");
+
+template <bool FILT>
+__global__ __launch_bounds__(kB) void k_sp_part(const PipeDesc* __restrict__ pdp) {
+    __shared__ uint32_t s_key[kPartRows];
+    __shared__ uint8_t s_sl[kPartRows];    // slice of a live row, 0xff for any other row
+    __shared__ uint32_t o_key[kPartRows];  // the live pairs ordered by slice ...
+    __shared__ uint16_t o_row[kPartRows];  // ... (row - first row of the block)
+    __shared__ uint32_t s_wc[kNW][8], s_wb[kNW][8], s_tot;  // per wave: slice counts, starts
+    const PipeDesc& pd = *pdp;
+    const uint32_t t = threadIdx.x, lane = lane_id();
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t / HJ_WAVE);
+    const uint32_t nsrc = pd.src_rows;
+    const IdxCol wc = idx_col(pd.sj[0].key, nsrc);
+    const uint32_t r0 = blockIdx.x * kPartRows;
+    const int nf = FILT ? pd.nf : 0;
+    // the wave's groups in rounds of kPartU: one record load (lane u: group u), then the
+    // round's key loads at once
+    uint32_t c8[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) c8[x] = 0;
+    const uint32_t lu = lane < (uint32_t)kPartU ? lane : (uint32_t)kPartU - 1u;
+    for (uint32_t it = 0; it < kPartGroupsPerWave / kPartU; ++it) {
+        const uint32_t lg0 = w * kPartGroupsPerWave + it * kPartU;  // block-local group
+        const uint32_t g0 = r0 / 64u + lg0;
+        const u32x4 rec = idx_rec(wc, g0 + lu);
+        int32_t key[kPartU];
+        uint64_t vw[kPartU];
+#pragma unroll
+        for (int u = 0; u < kPartU; ++u) {
+            vw[u] = rec_valid(rec, u);
+            key[u] = idx_load(wc, rec, u, vw[u], lane);
+        }
+#pragma unroll
+        for (int u = 0; u < kPartU; ++u) {
+            const uint32_t g = g0 + u;
+            const uint32_t rem = g * 64u < nsrc ? nsrc - g * 64u : 0u;
+            uint64_t live = (rem >= 64u ? ~0ull : (1ull << rem) - 1ull) & vw[u];
+            const uint32_t r = g * 64u + lane;
+            for (int f = 0; f < nf; ++f) {
+                bool ok = false;
+                if (in_mask(live)) {
+                    int32_t v = 0;
+                    ok = get_int32(pd.fcol[f], r, &v) && v >= pd.flo[f] && v <= pd.fhi[f];
+                }
+                live &= __ballot(ok);
+            }
+            const uint32_t sl = fib32((uint32_t)key[u]) >> 29;
+            const bool on = in_mask(live);
+            const uint32_t i = (lg0 + u) * 64u + lane;
+            s_key[i] = (uint32_t)key[u];
+            s_sl[i] = on ? (uint8_t)sl : (uint8_t)0xff;
+#pragma unroll
+            for (int x = 0; x < 8; ++x)
+                c8[x] += (uint32_t)__popcll(__ballot(on && sl == (uint32_t)x));
+        }
+    }
+    // each wave's slice counts, then every wave's start in each slice (slices in order, waves
+    // in order within a slice): the wave places its pairs with ballots alone
+    if (lane == 0)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) s_wc[w][x] = c8[x];
+    __syncthreads();
+    if (t == 0) {
+        uint32_t o = 0;
+        for (int x = 0; x < 8; ++x) {
+            uint32_t n = 0;
+            for (uint32_t v = 0; v < kNW; ++v) {
+                s_wb[v][x] = o + n;
+                n += s_wc[v][x];
+            }
+            gp(pd.pcnt)[blockIdx.x * 8u + x] = n;
+            o += n;
+        }
+        s_tot = o;
+    }
+    __syncthreads();
+    uint32_t base[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) base[x] = s_wb[w][x];
+    // order the live pairs by slice in LDS, then store them with coalesced writes (stores
+    // scattered over 8 runs would cost a store instruction per run and 64 rows)
+    for (uint32_t i0 = w * (kPartRows / kNW); i0 < (w + 1) * (kPartRows / kNW); i0 += HJ_WAVE) {
+        const uint32_t i = i0 + lane;
+        const uint32_t sl = s_sl[i];
+        const uint32_t key = s_key[i];
+#pragma unroll
+        for (uint32_t x = 0; x < 8; ++x) {
+            const uint64_t m = __ballot(sl == x);
+            if (sl == x) {
+                const uint32_t pos = base[x] + mask_rank(m);
+                o_key[pos] = key;
+                o_row[pos] = (uint16_t)i;
+            }
+            base[x] += (uint32_t)__popcll(m);
+        }
+    }
+    __syncthreads();
+    HJ_GLOBAL uint32_t* tk = gp(pd.ptk) + r0;
+    HJ_GLOBAL uint32_t* tr = gp(pd.ptr) + r0;
+    const uint32_t tot = s_tot;
+    for (uint32_t j = t; j < tot; j += kB) {
+        tk[j] = o_key[j];
+        tr[j] = r0 + o_row[j];
+    }
+}
+
+constexpr int kSlU = 4;  // pairs per thread and round of the sliced probe
+
+__global__ __launch_bounds__(kB) void k_sp_probe_sl(const PipeDesc* __restrict__ pdp) {
+    __shared__ uint32_t s_pc[kSliceMaxK * 8];  // the chunk's slice counts
+    __shared__ uint32_t s_pre[kSliceMaxK + 1];  // slice x's pairs before each run
+    __shared__ uint32_t s_run[kSliceMaxK];      // first pair of each run
+    __shared__ uint32_t s_cnt, s_origin;
+    __shared__ uint64_t s_bytes[kMaxOutStr];
+    const PipeDesc& pd = *pdp;
+    const uint32_t t = threadIdx.x;
+    uint32_t bad = 0;
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) bad |= gp(pd.meta)[4 + 4 * j + 1];
+    if (bad) {
+        if (t == 0) {
+            if (blockIdx.x == 0) gp(pd.meta)[0] = 1;
+            gp(pd.blk_cnt)[blockIdx.x] = 0;
+            gp(pd.blk_origin)[blockIdx.x] = 0;
+        }
+        return;
+    }
+    const uint32_t x = blockIdx.x & 7u, c = blockIdx.x >> 3;
+    const uint32_t b0 = c * pd.part_k;
+    const uint32_t b1 = b0 + pd.part_k < pd.part_blocks ? b0 + pd.part_k : pd.part_blocks;
+    const uint32_t nr = b1 > b0 ? b1 - b0 : 0u;  // runs (partition blocks) of this chunk
+    for (uint32_t i = t; i < nr * 8u; i += kB) s_pc[i] = gp(pd.pcnt)[b0 * 8u + i];
+    if (t < (uint32_t)kMaxOutStr) s_bytes[t] = 0;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t pre = 0, before = 0;
+        for (uint32_t j = 0; j < nr; ++j) {
+            uint32_t off = 0;
+            for (uint32_t y = 0; y < x; ++y) off += s_pc[j * 8u + y];
+            before += off;
+            s_run[j] = (b0 + j) * kPartRows + off;
+            s_pre[j] = pre;
+            pre += s_pc[j * 8u + x];
+        }
+        s_pre[nr] = pre;
+        s_origin = b0 * kPartRows + before;
+        s_cnt = 0;
+    }
+    __syncthreads();
+    const uint32_t total = s_pre[nr], origin = s_origin;
+    const HJ_GLOBAL uint32_t* tkey = gp(pd.sj[0].table);
+    const uint32_t tk = table_bits(gp(pd.sj[0].meta)[0]);
+    const uint32_t tsp = gp(pd.sj[0].meta)[2];
+    const HJ_GLOBAL uint32_t* pk = gp(pd.ptk);
+    const HJ_GLOBAL uint32_t* pr = gp(pd.ptr);
+    for (uint32_t f0 = 0; f0 < total; f0 += kB * kSlU) {
+        uint32_t key[kSlU], row[kSlU], brow[kSlU];
+        uint64_t live[kSlU];
+#pragma unroll
+        for (int u = 0; u < kSlU; ++u) {
+            const uint32_t f = f0 + u * kB + t;
+            const bool have = f < total;
+            uint32_t lo = 0, hi = nr;  // the run holding pair f: the last j with s_pre[j] <= f
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= f)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t idx = have ? s_run[lo] + (f - s_pre[lo]) : 0u;
+            key[u] = have ? pk[idx] : 0u;
+            row[u] = have ? pr[idx] : 0u;
+            live[u] = __ballot(have);
+        }
+        // each home bucket's keys and row words together (one 32-byte line): the build row of
+        // a match in its home bucket needs no further load
+        u32x4 kv[kSlU], rv[kSlU];
+#pragma unroll
+        for (int u = 0; u < kSlU; ++u) {
+            const uint32_t hb = home_bucket(key[u], tk);
+            kv[u] = u32x4{0u, 0u, 0u, 0u};
+            rv[u] = kv[u];
+            if (in_mask(live[u])) {
+                kv[u] = load_bucket(tkey, hb);
+                rv[u] = *(const HJ_GLOBAL u32x4*)(tkey + 8u * hb + 4u);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSlU; ++u) {
+            const uint32_t hb = home_bucket(key[u], tk);
+            uint32_t sl = kDead;
+            if (in_mask(live[u])) sl = resolve_home(kv[u], key[u], hb, tk, tsp);
+            const uint32_t j = sl & 3u;
+            uint32_t br = j == 0 ? rv[u].x : (j == 1 ? rv[u].y : (j == 2 ? rv[u].z : rv[u].w));
+            const bool far = sl == kDead - 1u || (sl != kDead && (sl >> 2) != hb);
+            if (in_mask(live[u]) && far) {  // beyond the home bucket (rare), or INT32_MIN
+                if (sl == kDead - 1u) {
+                    const uint32_t smask = seg_mask(tk);
+                    const uint32_t nb = next_bucket(hb, smask);
+                    sl = bucket_resolve(tkey, smask, nb, key[u], load_bucket(tkey, nb));
+                }
+                if (sl != kDead) br = tkey[row_word(sl)];
+            }
+            brow[u] = br;
+            live[u] &= __ballot(sl != kDead);
+        }
+        uint32_t pos[kSlU];
+        block_append_masks<kSlU>(&s_cnt, live, pos);
+#pragma unroll
+        for (int u = 0; u < kSlU; ++u) {
+            if (!in_mask(live[u])) continue;
+            gp(pd.stage[0])[origin + pos[u]] = row[u];
+            gp(pd.stage[1])[origin + pos[u]] = brow[u];
+            const uint32_t br1[1] = {brow[u]};
+            count_bytes<1>(pd, s_bytes, row[u], br1);
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        gp(pd.blk_cnt)[blockIdx.x] = s_cnt;
+        gp(pd.blk_origin)[blockIdx.x] = origin;
+    }
+    if (pd.blk_bytes && t < (uint32_t)pd.nostr)
+        gp(pd.blk_bytes)[(uint64_t)blockIdx.x * kMaxOutStr + t] = s_bytes[t];
+}
+
 // the fused pass's staged survivors -> the outputs (no joins keyed by a build table) or the
 // dense survivor list, at the block's offset from the scan of the block counts
 __global__ __launch_bounds__(kB) void k_sp_compact(const PipeDesc* __restrict__ pdp) {
@@ -981,10 +1223,28 @@ void launch_timed(K kernel, const PipeDesc& hp, const PipeDesc* dp, hipStream_t 
         hipLaunchKernelGGL(kernel, dim3(hp.probe_blocks), dim3(kB), 0, s, dp);
 }
 
+// the sliced probe's two launches, the first event before the partition pass and the second
+// after the probe
+void launch_sliced(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s, hipEvent_t a,
+                   hipEvent_t b) {
+    const dim3 pg(hp.part_blocks), sg(hp.probe_blocks), bs(kB);
+    if (hp.nf)
+        hipExtLaunchKernelGGL(k_sp_part<true>, pg, bs, 0, s, a, nullptr, 0, dp);
+    else
+        hipExtLaunchKernelGGL(k_sp_part<false>, pg, bs, 0, s, a, nullptr, 0, dp);
+    hipExtLaunchKernelGGL(k_sp_probe_sl, sg, bs, 0, s, nullptr, b, 0, dp);
+}
+
 template <int NS>
 void launch_probe(const PipeDesc& hp, const PipeDesc* dp, hipStream_t s, hipEvent_t a,
                   hipEvent_t b) {
     constexpr int U = probe_unroll(NS);
+    if constexpr (NS == 1) {
+        if (hp.xslices) {
+            launch_sliced(hp, dp, s, a, b);
+            return;
+        }
+    }
     if (hp.nf)
         launch_timed(k_sp_probe<NS, U, true>, hp, dp, s, a, b);
     else

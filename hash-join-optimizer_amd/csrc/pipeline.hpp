@@ -19,6 +19,8 @@ constexpr uint32_t kBloomWords = 4096;       // 16 KiB: the LDS-resident filter 
 constexpr uint32_t kBucketKeys = 4;          // one 16-byte load reads a whole bucket
 constexpr uint32_t kMinTableBits = 6;
 constexpr int kMaxOutStr = 16;                // output VARCHAR columns counted by the fused pass
+constexpr uint32_t kSliceMaxK = 128;          // sliced probe: partition blocks per probe block
+constexpr uint32_t kPartRows = 2048;          // sliced probe: source rows per partition block
 
 // word index of slot s's row in a table
 __host__ __device__ inline uint32_t row_word(uint32_t s) { return 8u * (s >> 2) + 4u + (s & 3u); }
@@ -139,6 +141,16 @@ struct PipeDesc {
     uint32_t* blk_off;
     uint32_t probe_blocks;        // blocks of the fused pass ...
     uint32_t probe_gpb;           // ... and 64-row groups per block
+    // sliced probe (xslices = 8: one streamed join over a large table, probed slice by slice,
+    // one slice per XCD): the partition pass's (key, row) pairs (part_blocks * kPartRows each)
+    // and slice counts (8 per partition block); probe block c * 8 + x takes slice x of the
+    // partition blocks [c * part_k, (c + 1) * part_k) and stages from blk_origin[block] on
+    uint32_t xslices;
+    uint32_t part_blocks, part_k;
+    uint32_t* ptk;
+    uint32_t* ptr;
+    uint32_t* pcnt;
+    uint32_t* blk_origin;
     uint32_t* l_row;              // ndep > 0: survivors of the fused pass
     uint32_t* l_b[kMaxStages];    // build row per stage, kDead once a join failed
     uint32_t* tile_cnt;           // final compaction, per 4096 survivors

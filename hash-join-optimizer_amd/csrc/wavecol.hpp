@@ -73,14 +73,9 @@ __device__ inline IdxCol idx_col(const DCol& c, uint32_t rows) {
                   c.gidx ? 0ull : c.umagic, c.uniform_rows};
 }
 
-// cache policy of the streamed loads (group records and key values): 0 = default, 2 = non-temporal
-#ifndef HJ_STREAM_AUX
-#define HJ_STREAM_AUX 0
-#endif
-
 // the record of group grp (on the lane that loads it): validity bits in .x/.y, location in .z/.w
 __device__ inline u32x4 idx_rec(const IdxCol& w, uint32_t grp) {
-    return __builtin_amdgcn_raw_buffer_load_b128(w.rec, grp * 16u, 0, HJ_STREAM_AUX);
+    return __builtin_amdgcn_raw_buffer_load_b128(w.rec, grp * 16u, 0, 0);
 }
 
 // idx_rec for a column that may be dense with pages of one row count (the build side): its
@@ -114,7 +109,7 @@ __device__ inline int32_t idx_load(const IdxCol& w, const u32x4& r, int u, uint6
     // the whole offset in the per-lane operand, which the buffer's range check covers: a row
     // past the last page reads 0
     const uint32_t off = A + 4u * mask_rank(vw) + (lane >= (E >> 16) ? (E & 0xffffu) : 0u);
-    return __builtin_amdgcn_raw_buffer_load_b32(w.val, off, 0, HJ_STREAM_AUX);
+    return __builtin_amdgcn_raw_buffer_load_b32(w.val, off, 0, 0);
 }
 
 // A column of the query's row-space arrays (a predicate's results, a decoded column): the value

@@ -1,0 +1,63 @@
+"""One-join pipelines over tables of more than 2^18 build rows run the sliced probe (a partition
+pass orders each 4096-row block's live (key, row) pairs by the top 3 bits of the key hash; probe
+block c * 8 + x, dispatched to XCD x, looks up slice x of a chunk of blocks): the same rows as
+with the option off and as the oracle's, with NULL keys, INT32_MIN / INT32_MAX keys, a source
+filter, VARCHAR output and the build on either side."""
+import numpy as np
+import pytest
+
+import hjtest
+
+pytestmark = pytest.mark.gpu
+I32, STR = hjtest.DataType.INT32, hjtest.DataType.VARCHAR
+
+
+def _plan(oracle, seed, nb, npr, build_left, src_filter, strings):
+    rng = np.random.default_rng(seed)
+    ext = np.array([-2**31, 2**31 - 1, 0, -1], np.int32)
+    body = rng.choice(np.arange(1, 4 * nb, dtype=np.int64), nb - 4, replace=False)
+    bk = np.concatenate([ext, body.astype(np.int32)])
+    bk = bk[rng.permutation(nb)]
+    dom = np.concatenate([bk, rng.integers(4 * nb, 8 * nb, nb).astype(np.int32)])
+    pk = dom[rng.integers(0, len(dom), npr)]
+    pv = (rng.random(npr) > 0.1).astype(np.uint8)
+    pp = rng.integers(0, 1000, npr).astype(np.int32)
+    bcols = [oracle.codec.encode_int32(bk), oracle.codec.encode_int32(np.arange(nb, dtype=np.int32))]
+    btypes = [(0, I32), (1, I32)]
+    if strings:
+        words = [f"s{i % 977}-{'x' * (i % 23)}" for i in range(nb)]
+        bcols.append(oracle.codec.encode_strings(words))
+        btypes.append((2, STR))
+    plan = hjtest.Plan()
+    plan.inputs.append(hjtest.ColumnarTable(nb, bcols))
+    plan.inputs.append(hjtest.ColumnarTable(npr, [oracle.codec.encode_int32(pk, pv),
+                                                  oracle.codec.encode_int32(pp)]))
+    b = plan.new_scan_node(0, btypes)
+    p = plan.new_scan_node(1, [(0, I32), (1, I32)], [(1, 0, 899)] if src_filter else ())
+    nbc = len(btypes)
+    if build_left:  # output: probe payload, build row, (build string)
+        out = [(nbc + 1, I32), (1, I32)] + ([(2, STR)] if strings else [])
+        plan.root = plan.new_join_node(True, b, p, 0, 0, out)
+    else:
+        out = [(1, I32), (3, I32)] + ([(4, STR)] if strings else [])
+        plan.root = plan.new_join_node(False, p, b, 0, 0, out)
+    return plan
+
+
+@pytest.mark.parametrize("case", [
+    dict(seed=1, nb=300_000, npr=1_500_000, build_left=True, src_filter=False, strings=False),
+    dict(seed=2, nb=600_000, npr=3_000_000, build_left=False, src_filter=True, strings=False),
+    dict(seed=3, nb=400_000, npr=1_000_000, build_left=True, src_filter=True, strings=True),
+], ids=["plain", "filtered-probe-left", "strings"])
+def test_xcd_slices_match_oracle(gpu_ctx, oracle, case):
+    plan = _plan(oracle, **case)
+    want = oracle.rows(oracle.execute(plan))
+    assert len(want) > 100_000
+    print("xcd slices case", case, flush=True)
+    for on in (1, 0):
+        gpu_ctx.set_option("xcd_slices", on)
+        try:
+            got = oracle.rows(gpu_ctx.execute(plan))
+        finally:
+            gpu_ctx.set_option("xcd_slices", 1)
+        assert got == want, on
