@@ -1086,9 +1086,11 @@ class Executor {
             hp.probe_blocks = (uint32_t)std::max<uint64_t>(1, (groups + gpb - 1) / gpb);
             uint64_t cap = (uint64_t)hp.probe_blocks * gpb * 64;
             // one join over a table larger than an XCD's L2 (more than 2^18 build rows: at
-            // least 2^20 key slots, 8 MiB): the sliced probe, one table slice per XCD
+            // least 2^20 key slots, 8 MiB; a cached table or an unfiltered build, whose tuple
+            // count is known here): the sliced probe, one table slice per XCD
             const uint64_t nbp = (nsrc + kPartRows - 1) / kPartRows;
-            if (ctx_->xcd_slices && hp.nstream == 1 && hp.ndep == 0 && emit &&
+            const bool exact_rows = hp.nstream == 1 && (cached[hp.order[0]] || !any_filter);
+            if (ctx_->xcd_slices && hp.nstream == 1 && hp.ndep == 0 && emit && exact_rows &&
                 stage_rows[hp.order[0]] > (1u << 18) && nbp >= 8 &&
                 nbp <= (uint64_t)(kEmitMaxBlocks / 8) * kSliceMaxK) {
                 const uint64_t chunks0 = std::min<uint64_t>(kEmitMaxBlocks / 8, nbp);
@@ -1647,6 +1649,21 @@ void to_host_pages(hj_context* ctx, const hj_dresult* r, hj_result* out) {
                 pc.cap = col.type == HJ_INT32 ? int32_page_bound(n)
                                               : n + col.num_chars / kLongPiece + 2;
                 pc.refs = (PageRef*)A.scratch(pc.cap * sizeof(PageRef));
+                if (col.type == HJ_VARCHAR && n > kJumpMinRows && n < (1ull << 31)) {
+                    // planned by jumps (pagewrite.hip): the planner's scratch
+                    const uint64_t coarse = (n + 1) / 1024 + 3;
+                    pc.jump = 1;
+                    pc.jlp = (uint32_t*)A.scratch((n + 1) * sizeof(uint32_t));
+                    pc.jbp = (uint64_t*)A.scratch((n / 4096 + 2) * sizeof(uint64_t));
+                    pc.jnx = (uint32_t*)A.scratch((n + 1) * sizeof(uint32_t));
+                    pc.ja = (uint32_t*)A.scratch((n + 1) * sizeof(uint32_t));
+                    pc.jb = (uint32_t*)A.scratch((n + 1) * sizeof(uint32_t));
+                    pc.jc = (uint32_t*)A.scratch(coarse * sizeof(uint32_t));
+                    pc.jm = (uint32_t*)A.scratch(coarse * 32 * sizeof(uint32_t));
+                    pc.jnode = (uint32_t*)A.scratch(coarse * 1024 * sizeof(uint32_t));
+                    pc.jcnt = (uint32_t*)A.scratch(64);
+                    pb.jump_rows = std::max<uint64_t>(pb.jump_rows, n);
+                }
             }
             {
                 Prof p_(ctx, "page_plan");

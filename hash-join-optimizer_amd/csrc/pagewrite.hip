@@ -120,8 +120,10 @@ __device__ inline uint32_t pick8(const uint32_t (&v)[8], uint32_t k) {
 // Row positions inside the window are 32-bit and relative to b (the open page may have started
 // before the window: a negative start, at least -65536 since a page holds at most 65535 rows),
 // and byte counts are 32-bit (a window holds at most 512 * 8187 bytes of short strings).
+// below counts the refs this call emits whose first row lies below cut_lim (the segmented
+// planner's refs of its own segment)
 __device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t b, uint32_t vbyte,
-                                      const uint64_t (&o)[9]) {
+                                      const uint64_t (&o)[9], uint64_t cut_lim, uint32_t& below) {
     const uint32_t lane = threadIdx.x;
     const uint64_t rem = c.n - b;
     const int32_t nw = rem < 512 ? (int32_t)rem : 512;  // rows of the window (>= 1)
@@ -177,6 +179,7 @@ __device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t 
             const int32_t xb = rl + (int32_t)t;
             if (lane == 0) put_ref(c, st.np, (uint64_t)((int64_t)b + s), (uint32_t)(xb - s), 0);
             ++st.np;
+            below += (uint64_t)((int64_t)b + s) < cut_lim ? 1u : 0u;
             // bytes of L's rows before row xb: p[t - 1] of lane L, which lane t - 1 holds in pj
             const uint32_t before = t ? lane_u32(pj, t - 1) : 0u;
             g = -(aexL + (int32_t)before);
@@ -188,6 +191,7 @@ __device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t 
         if (lim > s) {
             if (lane == 0) put_ref(c, st.np, (uint64_t)((int64_t)b + s), (uint32_t)(lim - s), 0);
             ++st.np;
+            below += (uint64_t)((int64_t)b + s) < cut_lim ? 1u : 0u;
         }
         const uint64_t ll = lane_u32(pick8(lens, kl), LL);
         const uint64_t pieces = (ll + kLongPiece - 1) / kLongPiece;
@@ -197,6 +201,7 @@ __device__ inline void varchar_window(const PageCol& c, PlanState& st, uint64_t 
             if (q < pieces) put_ref(c, st.np + q, rowl, (uint32_t)q, q == 0 ? 1u : 2u);
         }
         st.np += pieces;
+        below += rowl < cut_lim ? (uint32_t)pieces : 0u;
         g = -(int32_t)(lane_u32(aex, LL) + lane_u32(pick8(p, kl), LL));
         s = lim + 1;
     }
@@ -261,7 +266,8 @@ __device__ uint64_t plan_varchar(const PageCol& c) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) co[k] = no[k];
         if (b + 512 < n) load(b + 512);
-        varchar_window(c, st, b, cb, co);
+        uint32_t below = 0;
+        varchar_window(c, st, b, cb, co, 0, below);
     }
     if (n > st.s) {
         if (lane == 0) put_ref(c, st.np, st.s, (uint32_t)(n - st.s), 0);
@@ -292,6 +298,7 @@ __global__ __launch_bounds__(256) void k_page_valid(PageBatch b) {
 
 __global__ __launch_bounds__(64) void k_page_plan(PageBatch b) {
     const PageCol c = b.c[blockIdx.x];
+    if (c.jump) return;  // planned by jumps (k_pj_*)
     uint64_t np;
     if (c.type == HJ_INT32 && gp(b.valid_rows)[blockIdx.x] == c.n) {
         // no NULL: every page but the last holds exactly 1984 rows
@@ -304,6 +311,237 @@ __global__ __launch_bounds__(64) void k_page_plan(PageBatch b) {
         np = c.type == HJ_INT32 ? plan_int32(c) : plan_varchar(c);
     }
     if (threadIdx.x == 0) gp(b.counts)[blockIdx.x] = np;
+}
+
+// ---------------------------------------------------------------- VARCHAR planning by jumps
+// The pages of a long VARCHAR column form a chain: where a page starts decides where it ends,
+// and the next page starts there. One wave walks that chain window by window (plan_varchar);
+// for a long column the chain is found in parallel instead, with the same cuts:
+//   k_pj_cost  per row its page bytes (2 + length for a string of at most 8185 bytes, 0 for a
+//              NULL row, kLongCost for a longer string, which no page of rows can hold), summed
+//              within blocks of 4096 rows; k_pj_scan adds the blocks' totals up
+//   k_pj_next  next(i) = the first row of the page after a page starting at row i: the first j
+//              whose row does not fit, 5 + bytes(i .. j) + (j - i) / 8 > 8192 (a galloping then
+//              binary search over the prefix sums); a long string at i is a page chain node of
+//              its own, next(i) = i + 1
+//   k_pj_sq    ten squarings: next^32 (D) and next^1024 (E), every row at once
+//   k_pj_walk  one lane follows E from row 0 (a step per 1024 pages), k_pj_mid expands each of
+//              those nodes by D into 32 nodes, k_pj_fine each of these by next into 32 pages
+//   k_pj_refs  the pages in chain order as refs (a long string's pieces counted by a scan)
+constexpr uint32_t kJBlk = 4096;          // rows per cost block
+constexpr uint32_t kLongCost = 1u << 18;  // page bytes of a long string: more than a page
+
+__device__ inline bool pj_valid(const PageCol& c, uint64_t r) {
+    return (gp(c.validity)[r >> 3] >> (r & 7)) & 1u;
+}
+__device__ inline uint32_t pj_cost(const PageCol& c, uint64_t r) {
+    if (!pj_valid(c, r)) return 0u;
+    const uint64_t len = gp(c.offsets)[r + 1] - gp(c.offsets)[r];
+    return len > kLongStr ? kLongCost : 2u + (uint32_t)len;
+}
+// bytes of rows [0, j)
+__device__ inline uint64_t pj_prefix(const PageCol& c, uint64_t j) {
+    return j ? gp(c.jbp)[(j - 1) / kJBlk] + gp(c.jlp)[j - 1] : 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_pj_cost(PageBatch b) {
+    __shared__ uint32_t lds[4];
+    const PageCol& c = b.c[blockIdx.y];
+    if (!c.jump) return;
+    const uint64_t n = c.n;
+    const uint64_t r0 = (uint64_t)blockIdx.x * kJBlk;
+    if (r0 >= n) return;  // uniform for the block
+    constexpr int K = (int)(kJBlk / 256);
+    const uint64_t rt = r0 + (uint64_t)threadIdx.x * K;
+    uint32_t v[K], acc = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        acc += rt + k < n ? pj_cost(c, rt + k) : 0u;
+        v[k] = acc;
+    }
+    uint32_t tot = 0;
+    const uint32_t ex = block_excl_scan<256>(acc, lds, &tot);
+    HJ_GLOBAL uint32_t* lp = gp(c.jlp);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (rt + k < n) lp[rt + k] = ex + v[k];
+    if (threadIdx.x == 0) gp(c.jbp)[blockIdx.x] = tot;  // the block's total, scanned next
+}
+
+// exclusive scan of the blocks' totals, one block per column
+__global__ __launch_bounds__(1024) void k_pj_scan(PageBatch b) {
+    __shared__ uint64_t lds[16];
+    const PageCol& c = b.c[blockIdx.x];
+    if (!c.jump) return;
+    const uint64_t nb = (c.n + kJBlk - 1) / kJBlk;
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t i0 = (uint64_t)threadIdx.x * per;
+    HJ_GLOBAL uint64_t* bp = gp(c.jbp);
+    uint64_t acc = 0;
+    for (uint64_t i = i0; i < i0 + per && i < nb; ++i) acc += bp[i];
+    uint64_t tot = 0;
+    uint64_t run = block_excl_scan<1024>(acc, lds, &tot);
+    __syncthreads();  // every thread has read its totals before any is overwritten
+    for (uint64_t i = i0; i < i0 + per && i < nb; ++i) {
+        const uint64_t x = bp[i];
+        bp[i] = run;
+        run += x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pj_next(PageBatch b) {
+    const PageCol& c = b.c[blockIdx.y];
+    if (!c.jump) return;
+    const uint64_t n = c.n;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i > n) return;
+    uint32_t nx = (uint32_t)n;
+    if (i < n) {
+        const uint64_t pi = pj_prefix(c, i);
+        // does row j still fit in a page that starts at row i?
+        auto fits = [&](uint64_t j) {
+            return 5ull + (pj_prefix(c, j + 1) - pi) + ((j - i) >> 3) <= (uint64_t)kPageBytes;
+        };
+        if (!fits(i)) {
+            nx = (uint32_t)(i + 1);  // a long string: its own pieces
+        } else {
+            uint64_t lo = i, hi = n;  // row lo fits; rows from hi on are past the page or the end
+            for (uint64_t step = 64;; step <<= 1) {
+                const uint64_t j = i + step;
+                if (j >= hi) break;
+                if (fits(j)) {
+                    lo = j;
+                } else {
+                    hi = j;
+                    break;
+                }
+            }
+            while (hi - lo > 1) {
+                const uint64_t mid = lo + (hi - lo) / 2;
+                if (fits(mid))
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            nx = (uint32_t)hi;
+        }
+    }
+    gp(c.jnx)[i] = nx;
+}
+
+// squaring s of the jump tables: dst = src o src over rows [0, n] (src[n] = n)
+__global__ __launch_bounds__(256) void k_pj_sq(PageBatch b, int step) {
+    const PageCol& c = b.c[blockIdx.y];
+    if (!c.jump) return;
+    const uint64_t n = c.n;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i > n) return;
+    // steps 0-4: next -> next^32 in ja; steps 5-9: ja -> next^1024 in jb (jlp reused)
+    const uint32_t* src;
+    uint32_t* dst;
+    switch (step) {
+        case 0: src = c.jnx; dst = c.ja; break;
+        case 1: src = c.ja; dst = c.jb; break;
+        case 2: src = c.jb; dst = c.ja; break;
+        case 3: src = c.ja; dst = c.jb; break;
+        case 4: src = c.jb; dst = c.ja; break;
+        case 5: src = c.ja; dst = c.jb; break;
+        case 6: src = c.jb; dst = c.jlp; break;
+        case 7: src = c.jlp; dst = c.jb; break;
+        case 8: src = c.jb; dst = c.jlp; break;
+        default: src = c.jlp; dst = c.jb; break;
+    }
+    const uint32_t x = gp(src)[i];
+    gp(dst)[i] = gp(src)[x];
+}
+
+// the chain's nodes at every 1024th page (one lane per column)
+__global__ __launch_bounds__(64) void k_pj_walk(PageBatch b) {
+    const PageCol& c = b.c[blockIdx.x];
+    if (!c.jump || threadIdx.x) return;
+    const uint32_t n = (uint32_t)c.n;
+    const HJ_GLOBAL uint32_t* E = gp(c.jb);
+    HJ_GLOBAL uint32_t* C = gp(c.jc);
+    uint32_t t = 0, x = 0;
+    while (x < n) {
+        C[t++] = x;
+        x = E[x];
+    }
+    C[t] = n;
+    gp(c.jcnt)[0] = t;
+}
+
+// each of those nodes expanded by next^32 into 32 nodes, then each of these by next into 32
+__global__ __launch_bounds__(64) void k_pj_mid(PageBatch b) {
+    const PageCol& c = b.c[blockIdx.y];
+    if (!c.jump) return;
+    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= gp(c.jcnt)[0]) return;
+    const uint32_t n = (uint32_t)c.n;
+    const HJ_GLOBAL uint32_t* D = gp(c.ja);
+    HJ_GLOBAL uint32_t* M = gp(c.jm) + (uint64_t)t * 32;
+    uint32_t x = gp(c.jc)[t];
+    for (int k = 0; k < 32; ++k) {
+        M[k] = x;
+        x = x < n ? D[x] : n;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pj_fine(PageBatch b) {
+    const PageCol& c = b.c[blockIdx.y];
+    if (!c.jump) return;
+    const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= (uint64_t)gp(c.jcnt)[0] * 32) return;
+    const uint32_t n = (uint32_t)c.n;
+    const HJ_GLOBAL uint32_t* X = gp(c.jnx);
+    HJ_GLOBAL uint32_t* P = gp(c.jnode) + q * 32;
+    uint32_t x = gp(c.jm)[q];
+    for (int u = 0; u < 32; ++u) {
+        P[u] = x;
+        x = x < n ? X[x] : n;
+    }
+}
+
+// the chain's pages in order as refs: one per node, a long string's pieces more (a scan of the
+// nodes' ref counts), one block per column
+__global__ __launch_bounds__(1024) void k_pj_refs(PageBatch b) {
+    __shared__ uint64_t lds[16];
+    const PageCol& c = b.c[blockIdx.x];
+    if (!c.jump) return;
+    const uint32_t n = (uint32_t)c.n;
+    const uint64_t total_nodes = (uint64_t)gp(c.jcnt)[0] * 1024;
+    const HJ_GLOBAL uint32_t* P = gp(c.jnode);
+    const HJ_GLOBAL uint32_t* X = gp(c.jnx);
+    auto refs_of = [&](uint32_t x, uint64_t* len) -> uint64_t {
+        *len = 0;
+        if (x >= n) return 0;
+        if (pj_valid(c, x)) {
+            const uint64_t l = gp(c.offsets)[(uint64_t)x + 1] - gp(c.offsets)[x];
+            if (l > kLongStr) {
+                *len = l;
+                return (l + kLongPiece - 1) / kLongPiece;
+            }
+        }
+        return 1;
+    };
+    uint64_t base = 0;
+    for (uint64_t p0 = 0; p0 < total_nodes; p0 += 1024) {
+        const uint64_t p = p0 + threadIdx.x;
+        const uint32_t x = p < total_nodes ? P[p] : n;
+        uint64_t len = 0;
+        const uint64_t k = refs_of(x, &len);
+        uint64_t tot = 0;
+        const uint64_t ex = block_excl_scan<1024>(k, lds, &tot);
+        const uint64_t at = base + ex;
+        if (k == 1 && !len) {
+            put_ref(c, at, x, X[x] - x, 0);
+        } else if (k) {
+            for (uint64_t q = 0; q < k; ++q) put_ref(c, at + q, x, (uint32_t)q, q == 0 ? 1u : 2u);
+        }
+        base += tot;
+        if (!tot) break;  // past the chain's last node (uniform)
+    }
+    if (threadIdx.x == 0) gp(b.counts)[blockIdx.x] = base;
 }
 
 // 8 validity bits starting at bit q (bit i of byte i/8); nbytes = bytes holding the n rows
@@ -454,6 +692,22 @@ void page_plan(const PageBatch& b, uint64_t max_rows, hipStream_t s) {
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, words / 2048));
     hipLaunchKernelGGL(k_page_valid, dim3(blocks, b.ncol), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_page_plan, dim3(b.ncol), dim3(64), 0, s, b);
+    if (b.jump_rows) {  // VARCHAR columns planned by jumps; jump_rows = the longest's rows
+        const uint64_t jn = b.jump_rows;
+        const dim3 g1((uint32_t)((jn + kJBlk - 1) / kJBlk), b.ncol);
+        const dim3 gr((uint32_t)((jn + 1 + 255) / 256), b.ncol);
+        const uint64_t coarse = (jn + 1) / 1024 + 2;  // chain nodes at every 1024th page
+        hipLaunchKernelGGL(k_pj_cost, g1, dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_pj_scan, dim3(b.ncol), dim3(1024), 0, s, b);
+        hipLaunchKernelGGL(k_pj_next, gr, dim3(256), 0, s, b);
+        for (int step = 0; step < 10; ++step)
+            hipLaunchKernelGGL(k_pj_sq, gr, dim3(256), 0, s, b, step);
+        hipLaunchKernelGGL(k_pj_walk, dim3(b.ncol), dim3(64), 0, s, b);
+        hipLaunchKernelGGL(k_pj_mid, dim3((uint32_t)((coarse + 63) / 64), b.ncol), dim3(64), 0, s, b);
+        hipLaunchKernelGGL(k_pj_fine, dim3((uint32_t)((coarse * 32 + 255) / 256), b.ncol), dim3(256),
+                           0, s, b);
+        hipLaunchKernelGGL(k_pj_refs, dim3(b.ncol), dim3(1024), 0, s, b);
+    }
 }
 
 void page_write(const PageBatch& b, uint64_t p0, uint64_t count, hipStream_t s) {

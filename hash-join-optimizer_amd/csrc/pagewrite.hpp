@@ -24,6 +24,7 @@ constexpr uint32_t kPageBytes = 8192;
 constexpr uint32_t kLongStr = kPageBytes - 7;   // longer strings take pages of their own
 constexpr uint32_t kLongPiece = kPageBytes - 4; // chars per 0xffff / 0xfffe page
 constexpr int kPageCols = 16;
+constexpr uint64_t kJumpMinRows = 65536;  // VARCHAR columns longer than this: planned by jumps
 
 // one output page: rows [row0, row0 + rows) of the column, or piece `rows` of the long string
 // in row row0 (kind 1: first piece, 0xffff page; kind 2: later pieces, 0xfffe pages)
@@ -44,10 +45,23 @@ struct PageCol {
     PageRef* refs;            // planner output
     uint64_t cap;             // refs capacity
     uint64_t first;           // writer: index of the column's first page in the page buffer
+    // VARCHAR columns of more than kJumpMinRows rows: planned by jumps (scratch of the planner)
+    uint32_t jump;            // 1: planned by jumps, 0: one wave walks the column
+    uint32_t pad2;
+    uint32_t* jlp;            // n: bytes within the row's 4096-row block, up to the row
+    uint64_t* jbp;            // per 4096-row block: bytes of the rows before it
+    uint32_t* jnx;            // n + 1: next page start after a page starting at the row
+    uint32_t* ja;             // n + 1: next^32, then scratch
+    uint32_t* jb;             // n + 1: next^1024
+    uint32_t* jc;             // chain nodes at every 1024th page (n / 1024 + 3)
+    uint32_t* jm;             // ... at every 32nd page (32 x as many)
+    uint32_t* jnode;          // ... every page (1024 x as many)
+    uint32_t* jcnt;           // nodes in jc
 };
 
 struct PageBatch {
     PageCol c[kPageCols];
+    uint64_t jump_rows;  // the rows of the longest column planned by jumps (0: none)
     uint32_t ncol;
     uint32_t pad;
     uint64_t* counts;   // planner: pages per column
