@@ -531,6 +531,8 @@ def main():
     ap.add_argument("--reps", type=int, default=3, help="batch: timed repetitions (median)")
     ap.add_argument("--untuned", action="store_true",
                     help="batch: the engine's default parameters instead of the tuned table")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="engine option for the measured context (hj_set_option), repeatable")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 3 times (a plain copy kernel of "
                          "known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -571,6 +573,9 @@ def main():
     ctx = hjo.Context(arena_bytes=(64 << 30) if args.workload == "m2m" else 0)
     if args.workload == "m2m":
         ctx.set_option("pipeline", 0)
+    for kv in args.set:
+        name, _, value = kv.partition("=")
+        ctx.set_option(name, int(value))
     t = time.time()
     dts = [ctx.upload(tb) for tb in tables]
     log(f"uploaded {len(dts)} tables in {time.time() - t:.1f}s")

@@ -48,7 +48,10 @@ def _plan(oracle, seed, nb, npr, build_left, src_filter, strings):
     dict(seed=1, nb=300_000, npr=1_500_000, build_left=True, src_filter=False, strings=False),
     dict(seed=2, nb=600_000, npr=3_000_000, build_left=False, src_filter=True, strings=False),
     dict(seed=3, nb=400_000, npr=1_000_000, build_left=True, src_filter=True, strings=True),
-], ids=["plain", "filtered-probe-left", "strings"])
+    # 512 partition blocks of 2048 rows: 256 chunks of 2, so 2048 probe blocks, exactly as
+    # many as the output pass's prefix sums cover
+    dict(seed=4, nb=300_000, npr=1 << 20, build_left=False, src_filter=False, strings=True),
+], ids=["plain", "filtered-probe-left", "strings", "output-block-cap"])
 def test_xcd_slices_match_oracle(gpu_ctx, oracle, case):
     plan = _plan(oracle, **case)
     want = oracle.rows(oracle.execute(plan))
