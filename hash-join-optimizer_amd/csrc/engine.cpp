@@ -756,6 +756,19 @@ class Executor {
                 out.slots.push_back({src.slots[used[i].second].table, dst});
             }
         }
+        if (o.values) {
+            // build-side output values in the directory's row order: the matches of a probe
+            // row then read them consecutively instead of one scattered read each (worth it
+            // when there are more matches than build rows)
+            for (int i = 0; i < o.n; ++i) {
+                if (!o.side[i] || total < nb) continue;
+                int32_t* bv = scratch<int32_t>(nb);
+                uint8_t* bvv = o.valid8[i] ? scratch<uint8_t>(nb) : nullptr;
+                { Prof p_(ctx_, "dir_gather"); dir_gather_values(dv, o, i, bv, bvv, s_); }
+                o.bval[i] = bv;
+                o.bvalid[i] = bvv;
+            }
+        }
         out.n = total;
         HeavyEntry* heavy = scratch<HeavyEntry>(total / kLightMax + 1);
         uint32_t* hcount = scratch<uint32_t>(1);

@@ -565,8 +565,21 @@ __device__ inline bool rel_row_key(const RelSrc& s, uint64_t i, uint32_t* key) {
     return true;
 }
 
-__device__ inline void write_out(const OutSlots& o, uint64_t pos, uint32_t prow, uint32_t brow) {
+// output row pos of probe row prow and the build row at directory position dpos
+__device__ inline void write_out(const OutSlots& o, const DirView& d, uint64_t pos, uint32_t prow,
+                                 uint32_t dpos) {
+    uint32_t brow = 0;
+    bool have_brow = false;
     for (int s = 0; s < o.n; ++s) {
+        if (o.values && o.side[s] && o.bval[s]) {  // in directory order: read in place
+            o.dst[s][pos] = (uint32_t)o.bval[s][dpos];
+            if (o.valid8[s]) o.valid8[s][pos] = o.bvalid[s][dpos];
+            continue;
+        }
+        if (o.side[s] && !have_brow) {
+            brow = d.srow[dpos];
+            have_brow = true;
+        }
         const uint32_t src_row = o.side[s] ? brow : prow;
         const uint32_t* m = o.src[s];
         const uint32_t row = m ? m[src_row] : src_row;
@@ -585,7 +598,7 @@ __device__ inline void write_out(const OutSlots& o, uint64_t pos, uint32_t prow,
 __device__ inline void emit(const OutSlots& o, const DirView& d, uint32_t prow, uint32_t lo,
                             uint32_t cnt, uint64_t off, HeavyEntry* heavy, uint32_t* heavy_count) {
     if (cnt <= kLightMax) {
-        for (uint32_t j = 0; j < cnt; ++j) write_out(o, off + j, prow, d.srow[lo + j]);
+        for (uint32_t j = 0; j < cnt; ++j) write_out(o, d, off + j, prow, lo + j);
     } else {
         const uint32_t nch = (cnt + kHeavyChunk - 1) / kHeavyChunk;
         const uint32_t idx = atomicAdd(heavy_count, nch);
@@ -633,7 +646,7 @@ __device__ inline void emit_wave(const OutSlots& o, const DirView& d, uint32_t p
             if (wr->incl[i + step - 1] <= e) i += step;
         const uint32_t before = i ? wr->incl[i - 1] : 0u;
         const uint32_t j = e - before;
-        write_out(o, wr->off[i] + j, wr->prow[i], d.srow[wr->lo[i] + j]);
+        write_out(o, d, wr->off[i] + j, wr->prow[i], wr->lo[i] + j);
     }
     __syncthreads();
 }
@@ -834,10 +847,34 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(DirView d, OutSlots o,
     for (uint32_t e = blockIdx.x; e < nh; e += gridDim.x) {
         const HeavyEntry h = heavy[e];
         for (uint32_t j = threadIdx.x; j < h.cnt; j += kBlock)
-            write_out(o, h.out + j, h.prow, d.srow[h.lo + j]);
+            write_out(o, d, h.out + j, h.prow, h.lo + j);
     }
 }
 }  // namespace
+
+namespace {
+// a value-mode output column of the build side in the directory's row order
+__global__ __launch_bounds__(kBlock) void k_dir_gather(DirView d, OutSlots o, int c, int32_t* bval,
+                                                       uint8_t* bvalid) {
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < d.n;
+         p += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t brow = d.srow[p];
+        const uint32_t row = o.src[c] ? o.src[c][brow] : brow;
+        int32_t v = 0;
+        const bool ok = get_int32(o.col[c], row, &v);
+        bval[p] = ok ? v : 0;
+        if (bvalid) bvalid[p] = ok ? 1 : 0;
+    }
+}
+}  // namespace
+
+void dir_gather_values(const DirView& d, const OutSlots& o, int c, int32_t* bval, uint8_t* bvalid,
+                       hipStream_t s) {
+    if (!d.n) return;
+    const uint64_t blocks = std::min<uint64_t>((d.n + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_dir_gather, dim3((uint32_t)blocks), dim3(kBlock), 0, s, d, o, c, bval,
+                       bvalid);
+}
 
 void probe_scan_count(const ScanSrc& src, const DirView& d, uint64_t* counts, hipStream_t s) {
     if (!src.key.num_pages) return;

@@ -55,7 +55,15 @@ struct OutSlots {
     int32_t values;
     uint8_t* valid8[kMaxSlots];
     DCol col[kMaxSlots];
+    // value mode, build-side columns: the values (and validity bytes, where valid8 is set) in
+    // the directory's row order (bval[c][p] = column c at the build row of directory position
+    // p), so the matches of a probe row read them consecutively; null: gathered per match
+    const int32_t* bval[kMaxSlots];
+    const uint8_t* bvalid[kMaxSlots];
 };
+// bval / bvalid of output column c of a value-mode join: one pass over the directory's rows
+void dir_gather_values(const DirView& d, const OutSlots& o, int c, int32_t* bval, uint8_t* bvalid,
+                       hipStream_t s);
 
 struct HeavyEntry {
     uint32_t prow;
