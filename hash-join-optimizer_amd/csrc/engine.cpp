@@ -687,15 +687,21 @@ class Executor {
         }
         uint64_t total = 0;
         uint64_t* poffs = nullptr;
+        uint64_t* match = nullptr;
         if (pblk) {
             uint64_t* counts = scratch<uint64_t>(pblk);
             poffs = scratch<uint64_t>(pblk);
             uint64_t* part = scratch<uint64_t>(scan_partials_needed(pblk));
             uint64_t* tot = scratch<uint64_t>(1);
+            // each probe row's lookup kept for the write pass (8 bytes a row), when the arena
+            // has room for it with the output to spare
+            const uint64_t prows = psc ? pss.key.num_rows : prs.n;
+            if (ctx_->match_stash && A_.free_bytes() / 8 >= prows * sizeof(uint64_t))
+                match = scratch<uint64_t>(prows);
             if (psc)
-                { Prof p_(ctx_, "probe_scan_count"); probe_scan_count(pss, dv, counts, s_); }
+                { Prof p_(ctx_, "probe_scan_count"); probe_scan_count(pss, dv, counts, match, s_); }
             else
-                { Prof p_(ctx_, "probe_rel_count"); probe_rel_count(prs, dv, counts, s_); }
+                { Prof p_(ctx_, "probe_rel_count"); probe_rel_count(prs, dv, counts, match, s_); }
             { Prof p_(ctx_, "scan"); scan_excl_sum_u64(counts, poffs, pblk, part, tot, s_); }
             total = fetch(ctx_, tot);
         }
@@ -774,9 +780,9 @@ class Executor {
         uint32_t* hcount = scratch<uint32_t>(1);
         HJ_HIP(hipMemsetAsync(hcount, 0, sizeof(uint32_t), s_));
         if (psc)
-            { Prof p_(ctx_, "probe_scan_write"); probe_scan_write(pss, dv, poffs, o, heavy, hcount, s_); }
+            { Prof p_(ctx_, "probe_scan_write"); probe_scan_write(pss, dv, poffs, o, heavy, hcount, match, s_); }
         else
-            { Prof p_(ctx_, "probe_rel_write"); probe_rel_write(prs, dv, poffs, o, heavy, hcount, s_); }
+            { Prof p_(ctx_, "probe_rel_write"); probe_rel_write(prs, dv, poffs, o, heavy, hcount, match, s_); }
         { Prof p_(ctx_, "expand_heavy"); expand_heavy(dv, o, heavy, hcount, s_); }
         if (dres) {
             for (int i = 0; i < o.n; ++i) {
@@ -1898,6 +1904,10 @@ int hj_set_option(hj_context* ctx, const char* name, int64_t value) {
     }
     if (std::strcmp(name, "xcd_slices") == 0) {
         ctx->xcd_slices = value != 0;
+        return HJ_OK;
+    }
+    if (std::strcmp(name, "match_stash") == 0) {
+        ctx->match_stash = value != 0;
         return HJ_OK;
     }
     if (std::strcmp(name, "dir_cache_bytes") == 0) {

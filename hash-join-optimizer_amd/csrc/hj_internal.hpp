@@ -202,6 +202,7 @@ struct hj_context {
     bool part_build = true;    // pipeline builds partitioned in LDS (hj_set_option)
     bool gpu_pages = true;     // result pages written on the device (hj_set_option)
     bool xcd_slices = true;    // one-join pipelines over large tables: per-XCD table slices
+    bool match_stash = true;   // join-at-a-time: the count pass keeps each row's lookup
     // pinned double buffer for result pages on their way to host memory (allocated on first use)
     uint8_t* page_stage = nullptr;
     hipEvent_t page_ev[2] = {nullptr, nullptr};

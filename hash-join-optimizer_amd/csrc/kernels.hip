@@ -779,7 +779,8 @@ namespace {
 template <bool WRITE>
 __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, uint64_t* counts,
                                                        const uint64_t* offsets, OutSlots o,
-                                                       HeavyEntry* heavy, uint32_t* heavy_count) {
+                                                       HeavyEntry* heavy, uint32_t* heavy_count,
+                                                       uint64_t* match) {
     __shared__ uint64_t lds[kBlock / HJ_WAVE];
     const uint32_t p = blockIdx.x;
     const HJ_GLOBAL uint8_t* pg = page_ptr(src.key, p);
@@ -789,8 +790,10 @@ __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, u
     if (!WRITE) {
         uint64_t acc = 0;
         for (uint32_t l = threadIdx.x; l < nr; l += kBlock) {
-            uint32_t key, lo;
-            if (scan_row_key(src, p, pg, hdr, l, r0 + l, &key)) acc += dir_lookup(d, key, &lo);
+            uint32_t key, lo = 0, cnt = 0;
+            if (scan_row_key(src, p, pg, hdr, l, r0 + l, &key)) cnt = dir_lookup(d, key, &lo);
+            acc += cnt;
+            if (match) match[r0 + l] = cnt ? ((uint64_t)lo << 32 | cnt) : 0ull;
         }
         const uint64_t tot = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) counts[p] = tot;
@@ -801,7 +804,15 @@ __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, u
     for (uint32_t l0 = 0; l0 < nr; l0 += kBlock) {
         const uint32_t l = l0 + threadIdx.x;
         uint32_t key, lo = 0, cnt = 0;
-        if (l < nr && scan_row_key(src, p, pg, hdr, l, r0 + l, &key)) cnt = dir_lookup(d, key, &lo);
+        if (l < nr) {
+            if (match) {
+                const uint64_t m = match[r0 + l];
+                lo = (uint32_t)(m >> 32);
+                cnt = (uint32_t)m;
+            } else if (scan_row_key(src, p, pg, hdr, l, r0 + l, &key)) {
+                cnt = dir_lookup(d, key, &lo);
+            }
+        }
         uint64_t tot;
         const uint64_t off = run + block_excl_op<OpSum>((uint64_t)cnt, lds, &tot);
         emit_wave(o, d, r0 + l, lo, cnt, off, heavy, heavy_count, &wr[threadIdx.x / HJ_WAVE]);
@@ -812,15 +823,19 @@ __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, u
 template <bool WRITE>
 __global__ __launch_bounds__(kBlock) void k_probe_rel(RelSrc src, DirView d, uint64_t* counts,
                                                       const uint64_t* offsets, OutSlots o,
-                                                      HeavyEntry* heavy, uint32_t* heavy_count) {
+                                                      HeavyEntry* heavy, uint32_t* heavy_count,
+                                                      uint64_t* match) {
     __shared__ uint64_t lds[kBlock / HJ_WAVE];
     const uint64_t base = (uint64_t)blockIdx.x * kRelTile;
     if (!WRITE) {
         uint64_t acc = 0;
         for (uint32_t l = threadIdx.x; l < kRelTile; l += kBlock) {
             const uint64_t i = base + l;
-            uint32_t key, lo;
-            if (i < src.n && rel_row_key(src, i, &key)) acc += dir_lookup(d, key, &lo);
+            if (i >= src.n) break;
+            uint32_t key, lo = 0, cnt = 0;
+            if (rel_row_key(src, i, &key)) cnt = dir_lookup(d, key, &lo);
+            acc += cnt;
+            if (match) match[i] = cnt ? ((uint64_t)lo << 32 | cnt) : 0ull;
         }
         const uint64_t tot = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) counts[blockIdx.x] = tot;
@@ -831,7 +846,15 @@ __global__ __launch_bounds__(kBlock) void k_probe_rel(RelSrc src, DirView d, uin
     for (uint32_t l0 = 0; l0 < kRelTile; l0 += kBlock) {
         const uint64_t i = base + l0 + threadIdx.x;
         uint32_t key, lo = 0, cnt = 0;
-        if (i < src.n && rel_row_key(src, i, &key)) cnt = dir_lookup(d, key, &lo);
+        if (i < src.n) {
+            if (match) {
+                const uint64_t m = match[i];
+                lo = (uint32_t)(m >> 32);
+                cnt = (uint32_t)m;
+            } else if (rel_row_key(src, i, &key)) {
+                cnt = dir_lookup(d, key, &lo);
+            }
+        }
         uint64_t tot;
         const uint64_t off = run + block_excl_op<OpSum>((uint64_t)cnt, lds, &tot);
         emit_wave(o, d, (uint32_t)i, lo, cnt, off, heavy, heavy_count, &wr[threadIdx.x / HJ_WAVE]);
@@ -876,28 +899,31 @@ void dir_gather_values(const DirView& d, const OutSlots& o, int c, int32_t* bval
                        bvalid);
 }
 
-void probe_scan_count(const ScanSrc& src, const DirView& d, uint64_t* counts, hipStream_t s) {
+void probe_scan_count(const ScanSrc& src, const DirView& d, uint64_t* counts, uint64_t* match,
+                      hipStream_t s) {
     if (!src.key.num_pages) return;
     hipLaunchKernelGGL(k_probe_scan<false>, dim3(src.key.num_pages), dim3(kBlock), 0, s, src, d,
-                       counts, nullptr, OutSlots{}, nullptr, nullptr);
+                       counts, nullptr, OutSlots{}, nullptr, nullptr, match);
 }
 void probe_scan_write(const ScanSrc& src, const DirView& d, const uint64_t* offsets,
                       const OutSlots& o, HeavyEntry* heavy, uint32_t* heavy_count,
-                      hipStream_t s) {
+                      uint64_t* match, hipStream_t s) {
     if (!src.key.num_pages) return;
     hipLaunchKernelGGL(k_probe_scan<true>, dim3(src.key.num_pages), dim3(kBlock), 0, s, src, d,
-                       nullptr, offsets, o, heavy, heavy_count);
+                       nullptr, offsets, o, heavy, heavy_count, match);
 }
-void probe_rel_count(const RelSrc& src, const DirView& d, uint64_t* counts, hipStream_t s) {
+void probe_rel_count(const RelSrc& src, const DirView& d, uint64_t* counts, uint64_t* match,
+                     hipStream_t s) {
     if (!src.n) return;
     hipLaunchKernelGGL(k_probe_rel<false>, dim3(rel_blocks(src.n)), dim3(kBlock), 0, s, src, d,
-                       counts, nullptr, OutSlots{}, nullptr, nullptr);
+                       counts, nullptr, OutSlots{}, nullptr, nullptr, match);
 }
 void probe_rel_write(const RelSrc& src, const DirView& d, const uint64_t* offsets,
-                     const OutSlots& o, HeavyEntry* heavy, uint32_t* heavy_count, hipStream_t s) {
+                     const OutSlots& o, HeavyEntry* heavy, uint32_t* heavy_count, uint64_t* match,
+                     hipStream_t s) {
     if (!src.n) return;
     hipLaunchKernelGGL(k_probe_rel<true>, dim3(rel_blocks(src.n)), dim3(kBlock), 0, s, src, d,
-                       nullptr, offsets, o, heavy, heavy_count);
+                       nullptr, offsets, o, heavy, heavy_count, match);
 }
 namespace {
 // thread t: rows 8t .. 8t + 7, read as one 8-byte word, written as bitmap byte t; without

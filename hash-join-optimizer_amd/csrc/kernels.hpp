@@ -125,14 +125,19 @@ void extract_rel_count(const RelSrc& src, uint32_t* counts, hipStream_t s);
 void extract_rel_write(const RelSrc& src, const uint32_t* offsets, uint32_t* out_h,
                        uint32_t* out_row, hipStream_t s);
 
-// ---- probe (two passes: per-block output counts, then writes)
-void probe_scan_count(const ScanSrc& src, const DirView& d, uint64_t* counts, hipStream_t s);
+// ---- probe (two passes: per-block output counts, then writes). With a match array (one
+// word per probe row: first directory position << 32 | match count, 0 for none) the count pass
+// stores each row's lookup and the write pass reads it back instead of looking the key up again.
+void probe_scan_count(const ScanSrc& src, const DirView& d, uint64_t* counts, uint64_t* match,
+                      hipStream_t s);
 void probe_scan_write(const ScanSrc& src, const DirView& d, const uint64_t* offsets,
                       const OutSlots& o, HeavyEntry* heavy, uint32_t* heavy_count,
-                      hipStream_t s);
-void probe_rel_count(const RelSrc& src, const DirView& d, uint64_t* counts, hipStream_t s);
+                      uint64_t* match, hipStream_t s);
+void probe_rel_count(const RelSrc& src, const DirView& d, uint64_t* counts, uint64_t* match,
+                     hipStream_t s);
 void probe_rel_write(const RelSrc& src, const DirView& d, const uint64_t* offsets,
-                     const OutSlots& o, HeavyEntry* heavy, uint32_t* heavy_count, hipStream_t s);
+                     const OutSlots& o, HeavyEntry* heavy, uint32_t* heavy_count, uint64_t* match,
+                     hipStream_t s);
 // validity bitmap (whole 64-bit words) of n rows from their validity bytes (nullptr: all set)
 void pack_valid8(const uint8_t* valid8, uint64_t n, uint8_t* bits, hipStream_t s);
 void expand_heavy(const DirView& d, const OutSlots& o, const HeavyEntry* heavy,

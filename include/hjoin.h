@@ -323,6 +323,11 @@ int hj_batch_free(hj_context* ctx, hj_batch* b);
  *   "gpu_pages" 1 (default) / 0: hj_execute and hj_dresult_to_host write the result pages on the
  *              device and copy them out; 0 copies the flat columns and pages them on the host.
  *              Both give the same bytes.
+ *   "xcd_slices" 1 (default) / 0: one-join pipelines over large tables partition the probe rows
+ *              by hash slice and probe each slice on its own XCD; 0 probes every row in place.
+ *   "match_stash" 1 (default) / 0: the join-at-a-time path keeps each probe row's lookup from
+ *              the count pass (8 bytes a row of scratch) for the write pass; 0 looks up again.
+ *              Both give the same rows.
  *   "dir_cache_bytes" N > 0: keep the tables the fused pipeline builds for joins keyed by the
  *              source scan (a dimension's key column with its scan's filters and predicate
  *              program) in a directory cache of N bytes of HBM, so that a later query over the

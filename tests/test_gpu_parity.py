@@ -11,15 +11,19 @@ import hjtest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["pipeline", "pipeline_cas", "join_at_a_time"])
+@pytest.fixture(params=["pipeline", "pipeline_cas", "join_at_a_time", "jat_lookup_twice"])
 def mode(request, gpu_ctx):
     """run each test with the fused probe pipeline (tables built segment by segment in LDS, or
-    by compare-and-swap inserts into cleared tables) and with join-at-a-time"""
-    gpu_ctx.set_option("pipeline", 0 if request.param == "join_at_a_time" else 1)
+    by compare-and-swap inserts into cleared tables) and with join-at-a-time (its write pass
+    reading the count pass's stored lookups, or looking every key up again)"""
+    jat = request.param in ("join_at_a_time", "jat_lookup_twice")
+    gpu_ctx.set_option("pipeline", 0 if jat else 1)
     gpu_ctx.set_option("partitioned_build", 0 if request.param == "pipeline_cas" else 1)
+    gpu_ctx.set_option("match_stash", 0 if request.param == "jat_lookup_twice" else 1)
     yield request.param
     gpu_ctx.set_option("pipeline", 1)
     gpu_ctx.set_option("partitioned_build", 1)
+    gpu_ctx.set_option("match_stash", 1)
 
 
 @pytest.mark.parametrize("case", hjtest.load_kat_cases(), ids=lambda c: c["name"])
