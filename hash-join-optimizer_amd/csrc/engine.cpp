@@ -810,6 +810,8 @@ class Executor {
         uint32_t col;
     };
     std::vector<AttrSlot> pipe_keys_;  // the pipeline's probe keys (slot, column)
+    int pay_out_ = -1;   // the output column whose values the pipeline's table carries
+    int ppay_out_ = -1;  // the output column whose values the partition pass carries
     std::vector<std::string> cache_refs_;  // directory cache entries this query holds
     std::vector<uint32_t> job_tuples_;     // tuples of each built pipeline job, after the pass
 
@@ -875,6 +877,8 @@ class Executor {
 
     Pipe try_pipeline(Rel& out, hj_dresult** dres, bool allow_emit) {
         pipe_keys_.clear();
+        pay_out_ = -1;
+        ppay_out_ = -1;
         uint64_t source = 0;
         std::vector<uint64_t> joins;
         if (!chain(plan_->root, source, joins, 0) || joins.empty() ||
@@ -1124,6 +1128,41 @@ class Executor {
                 hp.ptr = scratch<uint32_t>(cap);
                 hp.pcnt = scratch<uint32_t>(nbp * 8);
                 hp.blk_origin = scratch<uint32_t>(hp.probe_blocks);
+                // The join's only build-side output, an INT32 column without NULL, rides in
+                // the table's row words (a table built by this query, not published to the
+                // directory cache): the sliced probe stages the value from the matching
+                // bucket, and the output copies it instead of reading it at a random row.
+                const int st0 = hp.order[0], j0 = hp.st[st0].job;
+                int nb = 0, bi = -1;
+                for (size_t i = 0; i < cur.size(); ++i)
+                    if (cur[i].slot == st0 + 1) {
+                        ++nb;
+                        bi = (int)i;
+                    }
+                if (j0 >= 0 && nb == 1) {
+                    const DevColumn& pc = column(slot_table[st0 + 1], cur[bi].col);
+                    if (pc.type == HJ_INT32 && pc.all_dense) {
+                        hb.job[j0].pay = qview(slot_table[st0 + 1], cur[bi].col);
+                        hb.job[j0].pay_on = 1;
+                        pay_out_ = bi;
+                    }
+                }
+                // likewise the source's only output: the partition pass carries its value
+                // with the key, and the probe stages that value instead of the row
+                int ns = 0, si = -1;
+                for (size_t i = 0; i < cur.size(); ++i)
+                    if (cur[i].slot == 0) {
+                        ++ns;
+                        si = (int)i;
+                    }
+                if (ns == 1) {
+                    const DevColumn& sc = column(slot_table[0], cur[si].col);
+                    if (sc.type == HJ_INT32 && sc.all_dense) {
+                        hp.ppay = qview(slot_table[0], cur[si].col);
+                        hp.ppay_on = 1;
+                        ppay_out_ = si;
+                    }
+                }
             }
             for (int q = 0; q <= hp.nstream; ++q) hp.stage[q] = scratch<uint32_t>(cap);
             hp.blk_cnt = scratch<uint32_t>(hp.probe_blocks);
@@ -1212,6 +1251,7 @@ class Executor {
             const int j = plan_stage_job(i);
             if (j < 0 || j >= (int)job_tuples_.size()) continue;
             const BuildJob& jb = hb.job[j];
+            if (jb.pay_on) continue;  // row words hold values, not rows
             ctx_->dcache->insert(ckeys[i], jb.table, table_bits(job_tuples_[j]), jb.meta, jb.bf,
                                  job_tuples_[j], s_);
         }
@@ -1246,6 +1286,7 @@ class Executor {
             for (const AttrSlot& k : pipe_keys_)
                 if (k.slot == cur[i].slot && k.col == cur[i].col) ec.nonull = 1;
             ec.slot = ps ? 1 + hp.qpos[ps - 1] : 0;
+            ec.staged = ((int)i == pay_out_ || (int)i == ppay_out_) ? 1 : 0;
             ec.type = dc.type;
             if (dc.type == HJ_VARCHAR) ec.sidx = nstr++;
             ec.valid8 = scratch<uint8_t>(cap);

@@ -240,8 +240,8 @@ __global__ __launch_bounds__(kB) void k_emit(EmitDesc d) {
             for (int u = 0; u < kIntRows; ++u) {
                 const uint32_t f = f0 + u * kB + t;
                 if (f >= R) continue;
-                int32_t v = 0;
-                const bool ok = get_int32(ec.col, rid[u], &v);
+                int32_t v = (int32_t)rid[u];
+                const bool ok = ec.staged || get_int32(ec.col, rid[u], &v);
                 gp(ec.values)[off + f] = ok ? v : 0;
                 if (!ec.nonull) gp(ec.valid8)[off + f] = ok ? 1 : 0;
             }

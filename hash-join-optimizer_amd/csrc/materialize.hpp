@@ -67,7 +67,7 @@ struct EmitCol {
     uint8_t* valid8;      // scratch: validity byte per output row (unused when nonull)
     uint8_t* validity;    // bitmap, 64-bit words
     int32_t nonull;       // no output row is NULL: the bitmap is set whole, no validity bytes
-    int32_t pad;
+    int32_t staged;       // INT32: the stage array holds the values themselves (no gather)
 };
 
 struct EmitDesc {

@@ -328,6 +328,40 @@ __global__ __launch_bounds__(kB) void k_sb_filter_part(const BuildBatch* __restr
     }
 }
 
+// Payload pass (jobs with pay_on, after either filter pass): the kPaySplit blocks of filter
+// block x replace the rows staged in x's range (rows of x's own kBuildRows-row range, so the
+// value reads stay in a few pages) by their values of the job's payload column, which the table
+// then holds in place of rows; each thread takes kPayRows rows, their loads in flight together.
+// A pass of its own keeps the filter kernels' registers as they are.
+constexpr int kPayRows = 8;
+constexpr uint32_t kPaySplit = kBuildRows / (kB * kPayRows);
+__global__ __launch_bounds__(kB) void k_sb_pay(const BuildBatch* __restrict__ bb) {
+    const int job = build_job_of(bb, blockIdx.x / kPaySplit);
+    const BuildJob& jb = bb->job[job];
+    if (!jb.pay_on) return;
+    const uint32_t x = blockIdx.x / kPaySplit - jb.blk0;
+    const uint32_t n = gp(jb.blk)[x];
+    const uint32_t j0 = (blockIdx.x % kPaySplit) * (kB * kPayRows);
+    HJ_GLOBAL uint32_t* sr = gp(jb.st_row) + (uint64_t)x * kBuildRows;
+    uint32_t row[kPayRows];
+    int32_t v[kPayRows];
+#pragma unroll
+    for (int k = 0; k < kPayRows; ++k) {
+        const uint32_t j = j0 + k * kB + threadIdx.x;
+        row[k] = j < n ? sr[j] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPayRows; ++k) {
+        v[k] = 0;
+        if (j0 + k * kB + threadIdx.x < n) (void)get_int32(jb.pay, row[k], &v[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kPayRows; ++k) {
+        const uint32_t j = j0 + k * kB + threadIdx.x;
+        if (j < n) sr[j] = (uint32_t)v[k];
+    }
+}
+
 // Segment pass: the table segments of every job (2^fine_bits(k) per job, k from the tuple
 // count), a grid-stride loop over them. A block gathers a segment's tuples from every filter
 // block's staged range (its partitions are contiguous there; a partition that holds several
@@ -943,9 +977,32 @@ __global__ __launch_bounds__(kB) void k_sp_part(const PipeDesc* __restrict__ pdp
     HJ_GLOBAL uint32_t* tk = gp(pd.ptk) + r0;
     HJ_GLOBAL uint32_t* tr = gp(pd.ptr) + r0;
     const uint32_t tot = s_tot;
-    for (uint32_t j = t; j < tot; j += kB) {
-        tk[j] = o_key[j];
-        tr[j] = r0 + o_row[j];
+    if (!pd.ppay_on) {
+        for (uint32_t j = t; j < tot; j += kB) {
+            tk[j] = o_key[j];
+            tr[j] = r0 + o_row[j];
+        }
+        return;
+    }
+    // the source's only output, an INT32 column without NULL, travels with the key in place of
+    // the row: every value of the block's range loaded first (one per thread and round), then
+    // the stores
+    constexpr int kPR = (int)(kPartRows / kB);
+    uint32_t pv[kPR];
+#pragma unroll
+    for (int k = 0; k < kPR; ++k) {
+        const uint32_t j = t + k * kB;
+        int32_t v = 0;
+        if (j < tot) (void)get_int32(pd.ppay, r0 + o_row[j], &v);
+        pv[k] = (uint32_t)v;
+    }
+#pragma unroll
+    for (int k = 0; k < kPR; ++k) {
+        const uint32_t j = t + k * kB;
+        if (j < tot) {
+            tk[j] = o_key[j];
+            tr[j] = pv[k];
+        }
     }
 }
 
@@ -1286,12 +1343,16 @@ uint32_t probe_resident_blocks(int ns, uint32_t num_cus) {
 
 void pipe_build(const BuildBatch& hb, const BuildBatch* db, hipStream_t s) {
     if (hb.njobs == 0) return;
+    bool pay = false;
+    for (int j = 0; j < hb.njobs; ++j) pay = pay || hb.job[j].pay_on;
     if (hb.part) {
         hipLaunchKernelGGL(k_sb_filter_part, dim3(hb.grid + hb.igrid), dim3(kB), 0, s, db);
+        if (pay) hipLaunchKernelGGL(k_sb_pay, dim3(hb.grid * kPaySplit), dim3(kB), 0, s, db);
         hipLaunchKernelGGL(k_sb_segbuild, dim3(hb.sgrid), dim3(kB), 0, s, db);
         return;
     }
     hipLaunchKernelGGL(k_sb_filter, dim3(hb.grid + hb.igrid), dim3(kB), 0, s, db);
+    if (pay) hipLaunchKernelGGL(k_sb_pay, dim3(hb.grid * kPaySplit), dim3(kB), 0, s, db);
     hipLaunchKernelGGL(k_sb_clear, dim3(hb.cgrid, hb.njobs), dim3(kB), 0, s, db);
     hipLaunchKernelGGL(k_sb_insert, dim3(hb.grid), dim3(kB), 0, s, db);
 }

@@ -85,6 +85,11 @@ struct BuildJob {
     uint16_t* poff;     // partitioned build: per filter block, the start of each of the
                         // 2^pbits partitions in its staged range, then its total
     uint32_t* bf;       // blocked Bloom filter, kBloomWords words (small build sides only)
+    // pay_on: the table's row words hold the value of INT32 column `pay` (no NULL) at each
+    // tuple's row instead of the row (the join's only build-side output, read by the probe
+    // straight from the matching bucket)
+    DCol pay;
+    int32_t pay_on;
 };
 
 struct BuildBatch {
@@ -120,6 +125,10 @@ struct StreamJoin {
 };
 
 struct PipeDesc {
+    // sliced probe: ppay_on, the partition pass stores the source row's value of INT32 column
+    // ppay (no NULL; the source's only output) in place of the row, and the survivors carry it
+    DCol ppay;
+    int32_t ppay_on;
     DCol fcol[kMaxFilters];       // filters of the source scan
     int32_t flo[kMaxFilters];
     int32_t fhi[kMaxFilters];
