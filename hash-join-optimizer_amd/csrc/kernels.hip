@@ -548,7 +548,10 @@ __device__ inline uint32_t dir_lookup(const DirView& d, uint32_t key, uint32_t* 
 __device__ inline bool scan_row_key(const ScanSrc& s, uint32_t p, const HJ_GLOBAL uint8_t* pg, uint32_t hdr,
                                     uint32_t local, uint32_t r, uint32_t* key) {
     int32_t k;
-    if (!get_int32_in_page(s.key, p, pg, hdr, local, &k)) return false;
+    // through the query's group index where the column has one: one record and one value load
+    if (!(s.key.gidx ? gidx_int32(s.key.pages, s.key.gidx, r, &k)
+                     : get_int32_in_page(s.key, p, pg, hdr, local, &k)))
+        return false;
     for (int f = 0; f < s.nf; ++f) {
         int32_t v;
         if (!get_int32(s.fcol[f], r, &v)) return false;
@@ -619,12 +622,54 @@ __device__ inline void emit(const OutSlots& o, const DirView& d, uint32_t prow, 
 // inclusive match counts in LDS. Output positions of one row are consecutive and rows follow
 // each other, so the writes are coalesced whatever the rows' match counts; heavy rows go to the
 // heavy list as before. Called by every thread of the block (it holds block barriers).
+// probe-side INT32 output columns whose values a wave reads once per probe row (value mode)
+constexpr int kWaveVals = 4;
 struct WaveRows {
     uint32_t incl[HJ_WAVE];
     uint32_t lo[HJ_WAVE];
     uint32_t prow[HJ_WAVE];
     uint64_t off[HJ_WAVE];
+    int32_t pv[kWaveVals][HJ_WAVE];  // the row's value of each of the first kWaveVals probe-side
+    uint8_t pok[kWaveVals][HJ_WAVE];  // output columns, and its validity
 };
+
+// write_out for match dpos of the wave's row i, whose probe-side values are in wr
+__device__ inline void write_out_wave(const OutSlots& o, const DirView& d, uint64_t pos,
+                                      uint32_t prow, uint32_t dpos, const WaveRows* wr,
+                                      uint32_t i) {
+    uint32_t brow = 0;
+    bool have_brow = false;
+    int pc = 0;  // probe-side value columns so far
+    for (int s = 0; s < o.n; ++s) {
+        if (o.values && !o.side[s] && pc < kWaveVals) {
+            o.dst[s][pos] = (uint32_t)wr->pv[pc][i];
+            if (o.valid8[s]) o.valid8[s][pos] = wr->pok[pc][i];
+            ++pc;
+            continue;
+        }
+        if (o.values && !o.side[s]) ++pc;
+        if (o.values && o.side[s] && o.bval[s]) {
+            o.dst[s][pos] = (uint32_t)o.bval[s][dpos];
+            if (o.valid8[s]) o.valid8[s][pos] = o.bvalid[s][dpos];
+            continue;
+        }
+        if (o.side[s] && !have_brow) {
+            brow = d.srow[dpos];
+            have_brow = true;
+        }
+        const uint32_t src_row = o.side[s] ? brow : prow;
+        const uint32_t* m = o.src[s];
+        const uint32_t row = m ? m[src_row] : src_row;
+        if (!o.values) {
+            o.dst[s][pos] = row;
+            continue;
+        }
+        int32_t v = 0;
+        const bool ok = get_int32(o.col[s], row, &v);
+        o.dst[s][pos] = ok ? (uint32_t)v : 0u;
+        if (o.valid8[s]) o.valid8[s][pos] = ok ? 1 : 0;
+    }
+}
 
 __device__ inline void emit_wave(const OutSlots& o, const DirView& d, uint32_t prow, uint32_t lo,
                                  uint32_t cnt, uint64_t off, HeavyEntry* heavy,
@@ -638,6 +683,21 @@ __device__ inline void emit_wave(const OutSlots& o, const DirView& d, uint32_t p
     wr->lo[lane] = lo;
     wr->prow[lane] = prow;
     wr->off[lane] = off;
+    if (o.values) {  // each light row's probe-side values, read once by its own lane
+        int pc = 0;
+        for (int s = 0; s < o.n && pc < kWaveVals; ++s) {
+            if (o.side[s]) continue;
+            int32_t v = 0;
+            bool ok = false;
+            if (cl) {
+                const uint32_t* m = o.src[s];
+                ok = get_int32(o.col[s], m ? m[prow] : prow, &v);
+            }
+            wr->pv[pc][lane] = ok ? v : 0;
+            wr->pok[pc][lane] = ok ? 1 : 0;
+            ++pc;
+        }
+    }
     __syncthreads();
     for (uint32_t e = lane; e < total; e += HJ_WAVE) {
         uint32_t i = 0;  // rows whose matches all come before match e
@@ -646,7 +706,7 @@ __device__ inline void emit_wave(const OutSlots& o, const DirView& d, uint32_t p
             if (wr->incl[i + step - 1] <= e) i += step;
         const uint32_t before = i ? wr->incl[i - 1] : 0u;
         const uint32_t j = e - before;
-        write_out(o, d, wr->off[i] + j, wr->prow[i], wr->lo[i] + j);
+        write_out_wave(o, d, wr->off[i] + j, wr->prow[i], wr->lo[i] + j, wr, i);
     }
     __syncthreads();
 }
@@ -788,12 +848,30 @@ __global__ __launch_bounds__(kBlock) void k_probe_scan(ScanSrc src, DirView d, u
     const uint32_t nr = hdr & 0xffffu;
     const uint32_t r0 = src.key.row_start[p];
     if (!WRITE) {
+        // kCU rows per thread and round: their keys first, then their lookups, then the
+        // stores, so that the rows' load chains are in flight together
+        constexpr int kCU = 4;
         uint64_t acc = 0;
-        for (uint32_t l = threadIdx.x; l < nr; l += kBlock) {
-            uint32_t key, lo = 0, cnt = 0;
-            if (scan_row_key(src, p, pg, hdr, l, r0 + l, &key)) cnt = dir_lookup(d, key, &lo);
-            acc += cnt;
-            if (match) match[r0 + l] = cnt ? ((uint64_t)lo << 32 | cnt) : 0ull;
+        for (uint32_t l0 = threadIdx.x; l0 < nr; l0 += kBlock * kCU) {
+            uint32_t key[kCU], lo[kCU], cnt[kCU];
+            bool ok[kCU];
+#pragma unroll
+            for (int u = 0; u < kCU; ++u) {
+                const uint32_t l = l0 + u * kBlock;
+                key[u] = 0;
+                ok[u] = l < nr && scan_row_key(src, p, pg, hdr, l, r0 + l, &key[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kCU; ++u) {
+                lo[u] = 0;
+                cnt[u] = ok[u] ? dir_lookup(d, key[u], &lo[u]) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kCU; ++u) {
+                const uint32_t l = l0 + u * kBlock;
+                acc += cnt[u];
+                if (match && l < nr) match[r0 + l] = cnt[u] ? ((uint64_t)lo[u] << 32 | cnt[u]) : 0ull;
+            }
         }
         const uint64_t tot = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) counts[p] = tot;
