@@ -940,15 +940,37 @@ __global__ __launch_bounds__(kBlock) void k_probe_rel(RelSrc src, DirView d, uin
     }
 }
 
-// heavy chunks: one block per chunk at a time, grid-stride over the chunk list
+// heavy chunks: one block per chunk at a time, grid-stride over the chunk list. In value mode
+// the chunk's probe row's values are read once per chunk (every thread reads the same words)
+// and each output takes them from registers.
 __global__ __launch_bounds__(kBlock) void k_expand_heavy(DirView d, OutSlots o,
                                                          const HeavyEntry* heavy,
                                                          const uint32_t* heavy_count) {
+    __shared__ WaveRows wr;  // entry 0: the chunk's probe row
     const uint32_t nh = *heavy_count;
     for (uint32_t e = blockIdx.x; e < nh; e += gridDim.x) {
         const HeavyEntry h = heavy[e];
+        if (!o.values) {
+            for (uint32_t j = threadIdx.x; j < h.cnt; j += kBlock)
+                write_out(o, d, h.out + j, h.prow, h.lo + j);
+            continue;
+        }
+        if (threadIdx.x == 0) {
+            int pc = 0;
+            for (int s = 0; s < o.n && pc < kWaveVals; ++s) {
+                if (o.side[s]) continue;
+                int32_t v = 0;
+                const uint32_t* m = o.src[s];
+                const bool ok = get_int32(o.col[s], m ? m[h.prow] : h.prow, &v);
+                wr.pv[pc][0] = ok ? v : 0;
+                wr.pok[pc][0] = ok ? 1 : 0;
+                ++pc;
+            }
+        }
+        __syncthreads();
         for (uint32_t j = threadIdx.x; j < h.cnt; j += kBlock)
-            write_out(o, d, h.out + j, h.prow, h.lo + j);
+            write_out_wave(o, d, h.out + j, h.prow, h.lo + j, &wr, 0);
+        __syncthreads();
     }
 }
 }  // namespace
