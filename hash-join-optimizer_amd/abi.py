@@ -79,7 +79,8 @@ class hj_batch_params(C.Structure):
                 ("radix_bits", C.c_uint32), ("load_shift", C.c_uint32),
                 ("large_build_rows", C.c_uint32), ("large_load_shift", C.c_uint32),
                 ("unroll", C.c_uint32), ("coop", C.c_uint32), ("use_graph", C.c_uint32),
-                ("plain_probe_rows", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("plain_probe_rows", C.c_uint32), ("plain_build_rows", C.c_uint32),
+                ("reserved", C.c_uint32 * 1)]
 
 
 P = C.POINTER

@@ -14,7 +14,9 @@ each parameter keeping the best value found so far for the others):
                             same for builds of at least 256K rows only
   radix bits                0, 4, 8, 12: the insert rounds ordered by that many top hash bits
   plain route               sources of at least 1M / 4M / 16M rows (or none) run through the
-                            fused pipeline one at a time, their results copied into the batch
+                            fused pipeline one at a time, their results written into the batch
+  plain build route         likewise queries whose builds hold at least 256K / 1M / 4M rows (or
+                            none): the pipeline builds tables in LDS segments
 Every timing is the median of `reps` runs of the same graph; every configuration's results are
 checked against the first configuration's row counts. The chosen parameters and all timings are
 written as JSON; tuned_params() reads the committed tables (tuning/batch_mi355x.json, one
@@ -33,7 +35,7 @@ TABLE = os.path.join(HERE, "tuning", "batch_mi355x.json")
 
 DEFAULTS = {"coop": 0, "unroll": 4, "probe_item_groups": 256, "load_shift": 1,
             "large_build_rows": 0, "large_load_shift": 0, "radix_bits": 0, "insert_item_rows": 8192,
-            "plain_probe_rows": 0}
+            "plain_probe_rows": 0, "plain_build_rows": 0}
 
 STEPS = [
     ("lane mapping", [{"coop": c, "unroll": u} for c in (0, 1) for u in (2, 4)]),
@@ -44,6 +46,7 @@ STEPS = [
                           {"load_shift": 2, "large_build_rows": 1 << 18, "large_load_shift": 1}]),
     ("radix bits", [{"radix_bits": r} for r in (0, 4, 8, 12)]),
     ("plain route", [{"plain_probe_rows": r} for r in (0, 1 << 20, 1 << 22, 1 << 24)]),
+    ("plain build route", [{"plain_build_rows": r} for r in (0, 1 << 18, 1 << 20, 1 << 22)]),
 ]
 
 

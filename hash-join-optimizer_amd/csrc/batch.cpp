@@ -269,7 +269,11 @@ void batch_build(hj_batch* b) {
     for (uint32_t i = 0; i < n; ++i) {
         if (plan_query(b->plans[i], b->tables[i].data(), b->qp[i])) {
             b->qidx[i] = (int32_t)b->nbatched++;
-            b->qp[i].launch = !b->prm.plain_probe_rows || b->qp[i].src_rows < b->prm.plain_probe_rows;
+            uint64_t brows = 0;
+            for (int j = 0; j < b->qp[i].q.nj; ++j) brows += b->qp[i].brows[j];
+            b->qp[i].launch =
+                (!b->prm.plain_probe_rows || b->qp[i].src_rows < b->prm.plain_probe_rows) &&
+                (!b->prm.plain_build_rows || brows < b->prm.plain_build_rows);
             order.push_back(i);
         }
     }
@@ -595,19 +599,29 @@ int hj_batch_run(hj_context* ctx, hj_batch* b) {
                 b->views[qi].num_rows = c[2];
                 if (c[1] == 0) continue;  // the launches' result is complete
             }
-            // on its own: not a star join, a source at or above plain_probe_rows, or a repeated
+            // on its own: not a star join, a source at or above plain_probe_rows, builds at or
+            // above plain_build_rows, or a repeated
             // build key
             b->fallback[i] = 1;
             hj_dresult* r = nullptr;
+            if (qi >= 0) {  // a star join: the fused pass may write into the batch's outputs
+                const hj_dresult& v = b->views[qi];
+                for (const auto& c : v.cols) ctx->emit_into.push_back({c.values, c.validity});
+            }
             int rc = hj_execute_device(ctx, b->plans[i], b->tables[i].data(), &r);
+            ctx->emit_into.clear();
             if (rc == HJ_OK && qi >= 0) {
-                // a star join: its columns copied into the batch's outputs (stream order)
+                // a star join: its columns copied into the batch's outputs (stream order) unless
+                // the fused pass wrote them there
                 rc = batch_guard(ctx, [&] {
                     hj_dresult& v = b->views[qi];
                     const uint64_t rows = r->num_rows;
                     if (r->cols.size() != v.cols.size() || rows > b->qp[i].src_rows)
                         throw Error(HJ_E_INVALID, "batch result shape differs from the plan's");
                     for (size_t c = 0; c < v.cols.size(); ++c) {
+                        if (r->cols[c].values == v.cols[c].values &&
+                            r->cols[c].validity == v.cols[c].validity)
+                            continue;  // written in place
                         if (rows)
                             HJ_HIP(hipMemcpyAsync(v.cols[c].values, r->cols[c].values, rows * 4,
                                                   hipMemcpyDeviceToDevice, ctx->stream));

@@ -1271,6 +1271,9 @@ class Executor {
         EmitDesc ed;
         std::memset(&ed, 0, sizeof ed);
         ed.ncol = (int32_t)cur.size();
+        bool into = ctx_->emit_into.size() == cur.size();
+        for (size_t i = 0; into && i < cur.size(); ++i)
+            into = column(slot_table[cur[i].slot], cur[i].col).type == HJ_INT32;
         int nstr = 0;  // VARCHAR columns in output order: the fused pass's byte count entries
         for (size_t i = 0; i < cur.size(); ++i) {
             const int ps = cur[i].slot;
@@ -1292,6 +1295,14 @@ class Executor {
             ec.valid8 = scratch<uint8_t>(cap);
             hj_dresult::Col c;
             c.type = dc.type;
+            if (into) {  // the caller's buffers (every output INT32)
+                c.validity = ctx_->emit_into[i].second;
+                c.values = ctx_->emit_into[i].first;
+                ec.validity = c.validity;
+                ec.values = c.values;
+                res->cols.push_back(c);
+                continue;
+            }
             c.validity = result<uint8_t>(vbytes);
             ec.validity = c.validity;
             if (dc.type == HJ_INT32) {

@@ -189,6 +189,10 @@ struct DirCache {
 struct hj_context {
     int device = 0;
     std::shared_ptr<hj::DirCache> dcache;  // built tables kept across queries (nullptr: none)
+    // set by a batch around one query it runs on its own: when that query's result comes
+    // straight from the fused pass and every output is INT32, column c's values and validity
+    // bitmap are written to emit_into[c] (the batch's buffers, sized for the source's rows)
+    std::vector<std::pair<int32_t*, uint8_t*>> emit_into;
     hipStream_t stream = nullptr;
     hj::Arena arena;
     uint8_t* arena_mem = nullptr;

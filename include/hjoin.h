@@ -289,9 +289,13 @@ typedef struct hj_batch_params {
     uint32_t use_graph;         /* 1: the launch is a HIP graph instantiated on the first run */
     uint32_t plain_probe_rows;  /* star joins whose source has at least this many rows run one at
                                    a time through the fused pipeline after the launches, their
-                                   results copied into the batch (0: every star join in the
-                                   launches) */
-    uint32_t reserved[2];
+                                   results written into the batch's outputs (0: every star join
+                                   in the launches) */
+    uint32_t plain_build_rows;  /* likewise star joins whose build sides hold at least this many
+                                   rows in all (0: no such route): the pipeline builds their
+                                   tables in LDS segments, the launches by one compare-and-swap
+                                   per tuple */
+    uint32_t reserved[1];
 } hj_batch_params;
 /* plans[i] runs over tables[i][0 .. plans[i]->num_inputs); plans and tables stay owned by the
  * caller and must outlive the batch. params NULL: the defaults above with use_graph = 1. */
