@@ -329,6 +329,7 @@ class Executor {
                     b.col[b.n] = dc.view();
                     b.out[b.n] = q.gidx;
                     b.blk0[b.n + 1] = b.blk0[b.n] + (uint32_t)((G + 255) / 256);
+                    if (&b == &pending_) pending_src_[b.n] = (int64_t)w.t == src_table;
                     ++b.n;
                 }
             } else if (dc.type == HJ_INT32) {
@@ -348,8 +349,10 @@ class Executor {
         prep_mark_ = A_.mark();
     }
 
-    // the probe side's index work not yet run (a pipeline build takes it into its filter launch)
+    // the probe side's index work not yet run (a pipeline takes the source's columns into its
+    // fused probe and the rest into its build's filter launch)
     GroupIndexBatch pending_;
+    bool pending_src_[kIndexCols] = {};  // entry i is a column of the pipeline's source
     void flush_pending() {
         if (!pending_.n) return;
         { Prof p_(ctx_, "group_index"); launch_group_index(pending_, s_); }
@@ -1015,11 +1018,6 @@ class Executor {
                 jb.poff = scratch<uint16_t>((uint64_t)jb.nblk * (P + 1));
             }
         hb.sgrid = (uint32_t)ctx_->num_cus * 4;  // segment pass: a grid-stride loop
-        if (pending_.n) {  // the probe side's index, in extra blocks of the filter launch
-            hb.gib = pending_;
-            hb.igrid = pending_.blk0[pending_.n];
-            std::memset(&pending_, 0, sizeof pending_);
-        }
         hb.pcnt = metas + kMetaWords + 2 * kEmitOut;
 
         hp.nf = (int32_t)srel.filters.size();
@@ -1187,6 +1185,31 @@ class Executor {
             hp.tile_off = scratch<uint32_t>(ncap / 4096 + 2);
         }
         hp.grid = (uint32_t)ctx_->num_cus * 8;
+        // The probe side's index, taken here, after the last return without a launch: the
+        // plain fused probe's blocks index the source's columns (up to kProbeIdxCols) for their
+        // own row ranges as they start; every other column (all of them before a sliced probe)
+        // goes to extra blocks of the build's filter launch, or a launch of its own when every
+        // table is cached.
+        if (pending_.n) {
+            GroupIndexBatch rest;
+            std::memset(&rest, 0, sizeof rest);
+            for (int i = 0; i < pending_.n; ++i) {
+                if (!hp.xslices && pending_src_[i] && hp.nsidx < kProbeIdxCols) {
+                    hp.sidx_table = ctx_->index_tables ? ~0u : 0u;  // ~0: the whole table
+                    hp.sidx_col[hp.nsidx] = pending_.col[i];
+                    hp.sidx_out[hp.nsidx] = pending_.out[i];
+                    ++hp.nsidx;
+                    continue;
+                }
+                rest.col[rest.n] = pending_.col[i];
+                rest.out[rest.n] = pending_.out[i];
+                rest.blk0[rest.n + 1] = rest.blk0[rest.n] + (pending_.blk0[i + 1] - pending_.blk0[i]);
+                ++rest.n;
+            }
+            hb.gib = rest;
+            hb.igrid = rest.blk0[rest.n];
+            std::memset(&pending_, 0, sizeof pending_);
+        }
 
         // descriptors and zeroed counters -> pinned staging -> device
         uint8_t* pin = (uint8_t*)ctx_->pinned + hj_context::kDescOffset;
@@ -1956,6 +1979,10 @@ int hj_set_option(hj_context* ctx, const char* name, int64_t value) {
     }
     if (std::strcmp(name, "xcd_slices") == 0) {
         ctx->xcd_slices = value != 0;
+        return HJ_OK;
+    }
+    if (std::strcmp(name, "index_tables") == 0) {
+        ctx->index_tables = value != 0;
         return HJ_OK;
     }
     if (std::strcmp(name, "match_stash") == 0) {

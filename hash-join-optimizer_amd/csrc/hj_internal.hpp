@@ -207,6 +207,8 @@ struct hj_context {
     bool gpu_pages = true;     // result pages written on the device (hj_set_option)
     bool xcd_slices = true;    // one-join pipelines over large tables: per-XCD table slices
     bool match_stash = true;   // join-at-a-time: the count pass keeps each row's lookup
+    bool index_tables = true;  // fused probe: blocks index their source rows through LDS page
+                               // tables (false: a page search per group)
     // pinned double buffer for result pages on their way to host memory (allocated on first use)
     uint8_t* page_stage = nullptr;
     hipEvent_t page_ev[2] = {nullptr, nullptr};

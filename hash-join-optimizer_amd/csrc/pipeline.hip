@@ -29,6 +29,8 @@ namespace {
 constexpr int kB = 256;
 constexpr uint32_t kNW = kB / HJ_WAVE;
 static_assert(kGroupIndexBlock == (uint32_t)kB, "index blocks of the filter launch: one group per thread");
+static_assert(kProbeIdxCols <= (int)kNW, "the fused probe's page tables: one wave per column");
+static_assert(kPageTable == HJ_WAVE, "a page table: one entry per lane");
 
 __device__ inline uint32_t log2ceil(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
@@ -672,6 +674,30 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
     uint32_t bad = 0;
 #pragma unroll
     for (int j = 0; j < kMaxStages; ++j) bad |= gp(pd.meta)[4 + 4 * j + 1];
+    const uint32_t n0 = gp(pd.sj[0].meta)[0];
+    const bool use_bf = bf_enabled(n0);
+    const uint32_t lw = bf_log2w(n0);
+    if (pd.nsidx) {
+        // the source columns' group index for this block's groups (also when the pass is
+        // voided below: the query's other paths read it), written before the waves read it;
+        // the page tables live in the queue's LDS, unused until the stream
+        static_assert(sizeof(qbuf) >= kProbeIdxCols * kPageTableWords * 4, "page tables");
+        uint32_t* pts = &qbuf[0][0][0];
+        const uint32_t gpb = pd.probe_gpb, gb0 = blockIdx.x * gpb;
+        const uint32_t r_lo = gb0 * 64u;
+        if ((int)w < pd.nsidx) {
+            const DCol& c = pd.sidx_col[w];
+            const uint64_t end = (uint64_t)(gb0 + gpb) * 64u;
+            const uint32_t r_hi = (uint32_t)(end < c.num_rows ? end : c.num_rows);
+            if (r_lo < r_hi)
+                page_table_load(c, r_lo, r_hi, pts + w * kPageTableWords, lane, pd.sidx_table);
+        }
+        __syncthreads();
+        for (int c = 0; c < pd.nsidx; ++c)
+            page_table_groups(pd.sidx_col[c], pd.sidx_out[c], gb0, gpb, t, kB,
+                              pts + c * kPageTableWords);
+        __syncthreads();
+    }
     if (bad) {
         if (t == 0) {
             if (blockIdx.x == 0) gp(pd.meta)[0] = 1;
@@ -679,9 +705,6 @@ __global__ __launch_bounds__(kB) void k_sp_probe(const PipeDesc* __restrict__ pd
         }
         return;
     }
-    const uint32_t n0 = gp(pd.sj[0].meta)[0];
-    const bool use_bf = bf_enabled(n0);
-    const uint32_t lw = bf_log2w(n0);
     {
         // the whole Bloom word array (kBloomWords, used or not): independent 16-byte loads
         constexpr int kC = kBloomWords / 4 / kB;

@@ -124,7 +124,15 @@ struct StreamJoin {
     const uint32_t* bf;     // ... and Bloom words
 };
 
+constexpr int kProbeIdxCols = 4;  // source columns the fused probe's blocks index themselves
+
 struct PipeDesc {
+    // the source's columns whose group index the fused probe's blocks build, each block for
+    // its own groups, as it starts
+    DCol sidx_col[kProbeIdxCols];
+    uint64_t* sidx_out[kProbeIdxCols];
+    int32_t nsidx;
+    uint32_t sidx_table;  // page-table entries a block may use (0: none, per-group page search)
     // sliced probe: ppay_on, the partition pass stores the source row's value of INT32 column
     // ppay (no NULL; the source's only output) in place of the row, and the survivors carry it
     DCol ppay;

@@ -329,6 +329,10 @@ int hj_batch_free(hj_context* ctx, hj_batch* b);
  *              Both give the same bytes.
  *   "xcd_slices" 1 (default) / 0: one-join pipelines over large tables partition the probe rows
  *              by hash slice and probe each slice on its own XCD; 0 probes every row in place.
+ *   "index_tables" 1 (default) / 0: the fused probe's blocks index their own source rows in
+ *              64-row groups through a table of the pages those rows span, held in LDS; 0 finds
+ *              every group's page by its own search (what a block does when its rows span more
+ *              pages than a table holds). Both give the same rows.
  *   "match_stash" 1 (default) / 0: the join-at-a-time path keeps each probe row's lookup from
  *              the count pass (8 bytes a row of scratch) for the write pass; 0 looks up again.
  *              Both give the same rows.

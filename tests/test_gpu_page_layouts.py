@@ -79,6 +79,23 @@ def test_page_layouts_vs_oracle(gpu_ctx, oracle, layout, pipeline):
     assert got == want
 
 
+@pytest.mark.parametrize("layout", ["regular", "uniform64", "uniform100", "uniform1000"])
+def test_page_layouts_probe_index_without_tables(gpu_ctx, oracle, layout):
+    """the fused probe's blocks index their source rows through LDS page tables; with the
+    tables off (option index_tables 0) every group is indexed by its own page search, the
+    path a block takes when its rows span more pages than a table holds"""
+    rng = np.random.default_rng(sum(map(ord, layout)) + 7)
+    plan = layout_plan(rng, layout)
+    want = oracle.rows(oracle.execute(writer_pages(plan, oracle.codec)))
+    for tables in (0, 1):
+        gpu_ctx.set_option("index_tables", tables)
+        try:
+            got = oracle.rows(gpu_ctx.execute(plan))
+        finally:
+            gpu_ctx.set_option("index_tables", 1)
+        assert got == want, tables
+
+
 @pytest.mark.parametrize("layout", ["tiny", "regular", "uniform100"])
 def test_page_layouts_root_scan_filtered(gpu_ctx, oracle, layout):
     """a filtered root scan reads its filter column and outputs through the same access paths"""
