@@ -70,6 +70,8 @@ def test_batch_matches_oracle(hjo, gpu_ctx, oracle, small_batch):
     {"probe_item_groups": 1},
     {"probe_item_groups": 3},
     {"plain_probe_rows": 5000},
+    {"plain_build_rows": 3000},
+    {"plain_probe_rows": 20000, "plain_build_rows": 6000},
 ], ids=lambda p: ",".join(f"{k}={v}" for k, v in p.items()))
 def test_batch_parameters(hjo, gpu_ctx, oracle, small_batch, params):
     plans, want = small_batch
@@ -77,8 +79,12 @@ def test_batch_parameters(hjo, gpu_ctx, oracle, small_batch, params):
     print(f"params {params}", flush=True)
     [(info, res)] = _run(hjo, gpu_ctx, plans, dict(params, use_graph=0))
     for i in range(len(plans)):
-        # sources of at least plain_probe_rows rows run through the fused pipeline
-        assert info[i][0] == (plans[i].inputs[0].num_rows < params.get("plain_probe_rows", 1 << 31)), i
+        # sources of at least plain_probe_rows rows, and builds of at least plain_build_rows
+        # rows in all, run through the fused pipeline (their outputs written into the batch's)
+        src_in = plans[i].inputs[0].num_rows < params.get("plain_probe_rows", 1 << 31)
+        builds = sum(t.num_rows for t in plans[i].inputs[1:])
+        build_in = builds < params.get("plain_build_rows", 1 << 31)
+        assert info[i][0] == (src_in and build_in), i
         assert oracle.rows(res[i]) == want[i], i
 
 
