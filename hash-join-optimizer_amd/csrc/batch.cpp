@@ -12,11 +12,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hjoin.h"
@@ -210,12 +213,16 @@ struct hj_batch {
     std::vector<std::string> own_err;
     std::vector<uint8_t> fallback;   // per query: ran on its own in the last run
     bool ran = false;
+    // contexts (own stream, arena and host thread during a run) that run the star joins routed
+    // past the launches beside them; created by the first run that has such queries
+    std::vector<hj_context*> helpers;
 };
 
 namespace {
 
 void batch_release(hj_batch* b) {
     if (!b) return;
+    for (auto* h : b->helpers) hj_context_destroy(h);
     for (auto& r : b->own) hj_result_free(&r);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -578,6 +585,75 @@ int hj_batch_create(hj_context* ctx, uint32_t n, const hj_plan* const* plans,
     return HJ_OK;
 }
 
+namespace {
+constexpr uint32_t kBatchHelpers = 4;           // helper contexts of a batch
+constexpr uint64_t kHelperArena = 8ull << 30;   // at most this much arena each
+
+// the helper contexts (at most kBatchHelpers, at most one per routed query), with the batch
+// context's engine options and directory cache; none where creating one fails
+void ensure_helpers(hj_batch* b, size_t routed) {
+    const size_t want = std::min<size_t>(kBatchHelpers, routed);
+    const uint64_t arena = std::min<uint64_t>(b->ctx->arena.cap, kHelperArena);
+    while (b->helpers.size() < want) {
+        hj_context* h = nullptr;
+        if (hj_context_create(arena, &h) != HJ_OK) break;
+        h->use_pipeline = b->ctx->use_pipeline;
+        h->part_build = b->ctx->part_build;
+        h->gpu_pages = b->ctx->gpu_pages;
+        h->xcd_slices = b->ctx->xcd_slices;
+        h->match_stash = b->ctx->match_stash;
+        h->index_tables = b->ctx->index_tables;
+        h->dcache = b->ctx->dcache;
+        b->helpers.push_back(h);
+    }
+}
+
+// query i on its own on context c: a star join's columns go into the batch's outputs (written
+// there by the fused pass, else copied in stream order), any other query's result to host pages
+int run_own(hj_batch* b, hj_context* c, uint32_t i) {
+    const int32_t qi = b->qidx[i];
+    b->fallback[i] = 1;
+    hj_dresult* r = nullptr;
+    if (qi >= 0) {
+        const hj_dresult& v = b->views[qi];
+        for (const auto& col : v.cols) c->emit_into.push_back({col.values, col.validity});
+    }
+    int rc = hj_execute_device(c, b->plans[i], b->tables[i].data(), &r);
+    c->emit_into.clear();
+    if (rc == HJ_OK && qi >= 0) {
+        rc = batch_guard(c, [&] {
+            hj_dresult& v = b->views[qi];
+            const uint64_t rows = r->num_rows;
+            if (r->cols.size() != v.cols.size() || rows > b->qp[i].src_rows)
+                throw Error(HJ_E_INVALID, "batch result shape differs from the plan's");
+            for (size_t k = 0; k < v.cols.size(); ++k) {
+                if (r->cols[k].values == v.cols[k].values &&
+                    r->cols[k].validity == v.cols[k].validity)
+                    continue;  // written in place
+                if (rows)
+                    HJ_HIP(hipMemcpyAsync(v.cols[k].values, r->cols[k].values, rows * 4,
+                                          hipMemcpyDeviceToDevice, c->stream));
+                const uint64_t vb = (rows + 63) / 64 * 8;
+                if (vb && r->cols[k].validity)
+                    HJ_HIP(hipMemcpyAsync(v.cols[k].validity, r->cols[k].validity, vb,
+                                          hipMemcpyDeviceToDevice, c->stream));
+                else if (vb)
+                    HJ_HIP(hipMemsetAsync(v.cols[k].validity, 0xff, vb, c->stream));
+            }
+            HJ_HIP(hipStreamSynchronize(c->stream));
+            v.num_rows = rows;
+        });
+        hj_dresult_free(c, r);
+    } else if (rc == HJ_OK) {
+        rc = hj_dresult_to_host(c, r, &b->own[i]);
+        hj_dresult_free(c, r);
+    }
+    b->own_rc[i] = rc;
+    b->own_err[i] = rc ? c->last_error : std::string();
+    return rc;
+}
+}  // namespace
+
 int hj_batch_run(hj_context* ctx, hj_batch* b) {
     if (!ctx || !b || ctx != b->ctx) return HJ_E_INVALID;
     return batch_guard(ctx, [&] {
@@ -587,6 +663,34 @@ int hj_batch_run(hj_context* ctx, hj_batch* b) {
         std::fill(b->fallback.begin(), b->fallback.end(), 0);
         std::fill(b->own_rc.begin(), b->own_rc.end(), HJ_OK);
         b->ran = false;
+        // star joins routed past the launches run on the helper contexts, one host thread each,
+        // pulling queries from a shared counter, while this thread runs the launches and the
+        // queries that run on their own here
+        std::vector<uint32_t> routed;
+        for (uint32_t i = 0; i < b->n; ++i)
+            if (b->qidx[i] >= 0 && !b->qp[i].launch) routed.push_back(i);
+        ensure_helpers(b, routed.size());
+        std::atomic<size_t> next{0};
+        std::mutex mu;
+        std::vector<uint32_t> retry;
+        std::vector<std::thread> th;
+        struct Joiner {  // the helpers are joined however this scope ends
+            std::vector<std::thread>& t;
+            ~Joiner() {
+                for (auto& x : t)
+                    if (x.joinable()) x.join();
+            }
+        } joiner{th};
+        for (hj_context* h : b->helpers)
+            th.emplace_back([&, h] {
+                for (size_t k; (k = next.fetch_add(1)) < routed.size();)
+                    if (run_own(b, h, routed[k]) == HJ_E_OOM) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        retry.push_back(routed[k]);
+                    }
+            });
+        if (b->helpers.empty())
+            for (uint32_t i : routed) run_own(b, ctx, i);
         batch_enqueue(b);
         HJ_HIP(hipMemcpyAsync(b->h_ctr, b->d_ctr, b->ctr_words * 4, hipMemcpyDeviceToHost,
                               ctx->stream));
@@ -599,49 +703,12 @@ int hj_batch_run(hj_context* ctx, hj_batch* b) {
                 b->views[qi].num_rows = c[2];
                 if (c[1] == 0) continue;  // the launches' result is complete
             }
-            // on its own: not a star join, a source at or above plain_probe_rows, builds at or
-            // above plain_build_rows, or a repeated
-            // build key
-            b->fallback[i] = 1;
-            hj_dresult* r = nullptr;
-            if (qi >= 0) {  // a star join: the fused pass may write into the batch's outputs
-                const hj_dresult& v = b->views[qi];
-                for (const auto& c : v.cols) ctx->emit_into.push_back({c.values, c.validity});
-            }
-            int rc = hj_execute_device(ctx, b->plans[i], b->tables[i].data(), &r);
-            ctx->emit_into.clear();
-            if (rc == HJ_OK && qi >= 0) {
-                // a star join: its columns copied into the batch's outputs (stream order) unless
-                // the fused pass wrote them there
-                rc = batch_guard(ctx, [&] {
-                    hj_dresult& v = b->views[qi];
-                    const uint64_t rows = r->num_rows;
-                    if (r->cols.size() != v.cols.size() || rows > b->qp[i].src_rows)
-                        throw Error(HJ_E_INVALID, "batch result shape differs from the plan's");
-                    for (size_t c = 0; c < v.cols.size(); ++c) {
-                        if (r->cols[c].values == v.cols[c].values &&
-                            r->cols[c].validity == v.cols[c].validity)
-                            continue;  // written in place
-                        if (rows)
-                            HJ_HIP(hipMemcpyAsync(v.cols[c].values, r->cols[c].values, rows * 4,
-                                                  hipMemcpyDeviceToDevice, ctx->stream));
-                        const uint64_t vb = (rows + 63) / 64 * 8;
-                        if (vb && r->cols[c].validity)
-                            HJ_HIP(hipMemcpyAsync(v.cols[c].validity, r->cols[c].validity, vb,
-                                                  hipMemcpyDeviceToDevice, ctx->stream));
-                        else if (vb)
-                            HJ_HIP(hipMemsetAsync(v.cols[c].validity, 0xff, vb, ctx->stream));
-                    }
-                    v.num_rows = rows;
-                });
-                hj_dresult_free(ctx, r);
-            } else if (rc == HJ_OK) {
-                rc = hj_dresult_to_host(ctx, r, &b->own[i]);
-                hj_dresult_free(ctx, r);
-            }
-            b->own_rc[i] = rc;
-            b->own_err[i] = rc ? ctx->last_error : std::string();
+            if (qi >= 0 && !b->qp[i].launch) continue;  // routed: run by the helpers
+            // on its own: not a star join, or a repeated build key
+            run_own(b, ctx, i);
         }
+        for (auto& x : th) x.join();
+        for (uint32_t i : retry) run_own(b, ctx, i);  // a helper's arena was too small
         HJ_HIP(hipStreamSynchronize(ctx->stream));
         b->ran = true;
     });
