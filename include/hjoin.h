@@ -302,7 +302,10 @@ typedef struct hj_batch_params {
 int hj_batch_create(hj_context* ctx, uint32_t n, const hj_plan* const* plans,
                     const hj_dtable* const* const* tables, const hj_batch_params* params,
                     hj_batch** out);
-/* run every query of the batch (the four launches, then the queries that run on their own);
+/* run every query of the batch: the four launches, the star joins routed past them on up to four
+ * helper contexts beside them (created by the first run that has such queries: own stream, host
+ * thread and an arena of at most 8 GiB each, freed with the batch), then the queries that run on
+ * their own on ctx;
  * returns when every result is complete. Results of the previous run are replaced. */
 int hj_batch_run(hj_context* ctx, hj_batch* b);
 uint32_t hj_batch_size(const hj_batch* b);
