@@ -667,7 +667,9 @@ def main():
         "execute_ms": execute_ms,
         "step_reads": "every key, filter and output value and validity bit is read from the "
                       "resident input pages inside the step; the per-query 64-row group index "
-                      "is built inside the step (extra blocks of the build's filter launch)",
+                      "is built inside the step (the source's columns by the fused probe's "
+                      "blocks, each for its row range; other columns by extra blocks of the "
+                      "build's filter launch)",
     }
     for t_ in dts:
         t_.free()
