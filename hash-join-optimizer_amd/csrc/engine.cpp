@@ -504,12 +504,15 @@ class Executor {
                         throw Error(HJ_E_INVALID, "LIKE patterns longer than 63 characters are not supported");
                     o.llen = (uint32_t)t.size();
                     uint32_t run = 0;
+                    bool bytes = t.size() <= (size_t)kMaxByteTokens;
                     for (size_t j = 0; j < t.size(); ++j) {
                         if (t[j] == kLikeAnyRun) o.pct |= 1ull << (j + 1);
                         if (t[j] == kLikeAnyOne) o.any |= 1ull << (j + 1);
                         run = t[j] == kLikeAnyRun ? run + 1 : 0;
                         o.pad = std::max(o.pad, run);
+                        bytes = bytes && (t[j] == kLikeAnyRun || t[j] < 128);
                     }
+                    o.ival = bytes ? 1 : 0;
                     const uint8_t* tb8 = (const uint8_t*)t.data();
                     lit.insert(lit.end(), tb8, tb8 + t.size() * 4);
                 } else {

@@ -120,6 +120,48 @@ __device__ bool like_match(const DCol& c, StrCursor cur, const HJ_GLOBAL uint32_
     return (d >> m) & 1ull;
 }
 
+// The same automaton over bytes, for patterns of '%' and ASCII characters only (at most
+// kMaxByteTokens tokens, so the state fits 32 bits). No literal matches a byte >= 0x80 and '%'
+// takes every character but '\n', so the bytes of a multi-byte character (or an invalid byte)
+// leave the state one step over the whole character would: closure(d & pct), which further
+// such bytes do not change. tab[b]: low word the pattern positions byte b matches, high word
+// pct, or 0 for '\n'. Eight bytes per aligned 8-byte load; their table entries are read before
+// the state steps through them.
+__device__ inline uint32_t like_closure32(uint32_t d, uint32_t pct, uint32_t runs) {
+    for (uint32_t i = 0; i < runs; ++i) d |= (d << 1) & pct;
+    return d;
+}
+
+__device__ bool like_match_bytes(const DCol& c, StrCursor cur, uint32_t m, uint32_t pct,
+                                 uint32_t runs, const uint64_t* tab) {
+    uint32_t d = like_closure32(1u, pct, runs);
+    while (cur.rem && d) {
+        if (!cur.seg) {
+            ++cur.page;
+            const HJ_GLOBAL uint8_t* pg = page_ptr(c, cur.page);
+            cur.p = pg + 4;
+            cur.seg = *(const HJ_GLOBAL uint32_t*)pg >> 16;
+        }
+        const uintptr_t a = (uintptr_t)cur.p;
+        const uint32_t sh = (uint32_t)(a & 7u);
+        const uint64_t w = *(const HJ_GLOBAL uint64_t*)(a - sh) >> (8u * sh);
+        uint32_t nb = 8u - sh < cur.seg ? 8u - sh : cur.seg;
+        if ((uint64_t)nb > cur.rem) nb = (uint32_t)cur.rem;
+        cur.p += nb;
+        cur.seg -= nb;
+        cur.rem -= nb;
+        uint64_t e[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) e[k] = tab[(uint32_t)(w >> (8 * k)) & 0xffu];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if ((uint32_t)k < nb)
+                d = like_closure32(((d << 1) & (uint32_t)e[k]) | (d & (uint32_t)(e[k] >> 32)),
+                                   pct, runs);
+    }
+    return (d >> m) & 1u;
+}
+
 // -1 / 0 / 1: the value against lit[0, ll), bytes as unsigned, a proper prefix first
 __device__ int compare_string(const DCol& c, StrCursor cur, const HJ_GLOBAL uint8_t* lit, uint32_t ll) {
     uint32_t i = 0;
@@ -131,22 +173,24 @@ __device__ int compare_string(const DCol& c, StrCursor cur, const HJ_GLOBAL uint
     return i < ll ? -1 : 0;
 }
 
-constexpr int kLikeTables = 4;  // LIKE entries with an ASCII table in LDS
+constexpr int kLikeTables = 4;  // LIKE entries with a table in LDS
 
 __global__ __launch_bounds__(256) void k_predicate(PredProg p) {
-    // per LIKE entry (the first kLikeTables): the pattern positions each ASCII character matches
-    __shared__ uint64_t ascii[kLikeTables][128];
+    // per LIKE entry (the first kLikeTables): byte patterns, the byte automaton's table over
+    // all 256 bytes; other patterns, the positions each ASCII character matches (first 128)
+    __shared__ uint64_t ascii[kLikeTables][256];
     {
         int nt = 0;  // LIKE entries seen so far: the i-th one uses table i
         for (int k = 0; k < p.nops && nt < kLikeTables; ++k) {
-            const int op = p.op[k].op;
-            if (op != HJ_P_LIKE && op != HJ_P_NOT_LIKE) continue;
-            if (threadIdx.x < 128) {
-                const HJ_GLOBAL uint32_t* tok = (const HJ_GLOBAL uint32_t*)gp(p.lit + p.op[k].loff);
+            const PredOp& o = p.op[k];
+            if (o.op != HJ_P_LIKE && o.op != HJ_P_NOT_LIKE) continue;
+            const uint32_t ch = threadIdx.x;
+            if (o.ival || ch < 128) {
+                const HJ_GLOBAL uint32_t* tok = (const HJ_GLOBAL uint32_t*)gp(p.lit + o.loff);
                 uint64_t lit = 0;
-                for (uint32_t j = 0; j < p.op[k].llen; ++j)
-                    lit |= (uint64_t)(tok[j] == threadIdx.x) << (j + 1);
-                ascii[nt][threadIdx.x] = lit;
+                for (uint32_t j = 0; j < o.llen; ++j) lit |= (uint64_t)(tok[j] == ch) << (j + 1);
+                if (o.ival) lit |= (uint64_t)(ch == '\n' ? 0u : (uint32_t)o.pct) << 32;
+                ascii[nt][ch] = lit;
             }
             ++nt;
         }
@@ -200,9 +244,11 @@ __global__ __launch_bounds__(256) void k_predicate(PredProg p) {
             else if (o.op == HJ_P_IS_NOT_NULL) b = ok;
             else if (!ok) b = false;
             else if (like) {
-                const bool mt = like_match(c, cur, (const HJ_GLOBAL uint32_t*)gp(p.lit + o.loff),
-                                           o.llen, o.pct, o.any, o.pad,
-                                           li < kLikeTables ? ascii[li] : nullptr);
+                const bool mt =
+                    o.ival && li < kLikeTables
+                        ? like_match_bytes(c, cur, o.llen, (uint32_t)o.pct, o.pad, ascii[li])
+                        : like_match(c, cur, (const HJ_GLOBAL uint32_t*)gp(p.lit + o.loff), o.llen,
+                                     o.pct, o.any, o.pad, li < kLikeTables ? ascii[li] : nullptr);
                 b = o.op == HJ_P_LIKE ? mt : !mt;
             } else {
                 const int cmp = compare_string(c, cur, gp(p.lit + o.loff), o.llen);

@@ -18,11 +18,13 @@ namespace hj {
 constexpr int kMaxPredOps = 32;
 constexpr int kMaxPredCols = 8;   // distinct columns a program reads
 constexpr int kMaxLikeTokens = 63;
+constexpr int kMaxByteTokens = 31;  // byte automaton: bit kMaxByteTokens is its last state bit
 
 struct PredOp {
     int32_t op;      // HJ_P_*
     int32_t col;     // comparisons: index into PredProg::col
-    int32_t ival;    // INT32 literal
+    int32_t ival;    // INT32 literal; LIKE: 1 when the pattern is '%' and ASCII characters only
+                     //   and at most kMaxByteTokens tokens (the automaton then runs over bytes)
     uint32_t loff;   // VARCHAR: literal bytes at lit + loff, llen of them; LIKE: tokens at
     uint32_t llen;   //   (const uint32_t*)(lit + loff), llen of them
     uint32_t pad;    // LIKE: the longest run of consecutive '%' tokens

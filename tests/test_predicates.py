@@ -210,6 +210,31 @@ def test_join_with_predicates_vs_oracle(gpu_ctx, oracle, mode, seed):
     assert oracle.rows(gpu_ctx.execute(plan)) == oracle.rows(oracle.execute(plan))
 
 
+# patterns of '%' and ASCII characters only, at most 31 tokens: the kernel runs these over
+# bytes; the last ones (a non-ASCII literal, '_', 32 tokens) take the character automaton
+BYTE_PATTERNS = ["%ab%", "a%", "%b", "ab", "%a%b%", "%%c", "", "%", "%%", "a.c%", "%\n%", "%.%",
+                 "%e%d", "ab%%", "%c%c%c", "%" + "a" * 30, "a%b%c%d%e%a%b%c%d%e%a%b%c%d%e%",
+                 "%é%", "%a_%", "%" + "b" * 31]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(4))
+def test_scan_like_byte_patterns_vs_oracle(gpu_ctx, oracle, seed):
+    # texts with multi-byte characters, '\n' and (odd seeds) long strings over several pages
+    rng = np.random.default_rng(1100 + seed)
+    t = predicate_table(rng, 3000, unicode=True, long_every=211 if seed % 2 else 0)
+    for pat in BYTE_PATTERNS:
+        for op in ("like", "not_like"):
+            plan = scan_plan(oracle, t, [(op, 1, pat)])
+            assert oracle.rows(gpu_ctx.execute(plan)) == oracle.rows(oracle.execute(plan)), (op, pat)
+    # six LIKE entries in one program: the fifth and sixth have no table in LDS
+    prog = [("like", 1, BYTE_PATTERNS[0])]
+    for pat in BYTE_PATTERNS[1:6]:
+        prog += [("like", 1, pat), ("or",)]
+    plan = scan_plan(oracle, t, prog)
+    assert oracle.rows(gpu_ctx.execute(plan)) == oracle.rows(oracle.execute(plan))
+
+
 @pytest.mark.gpu
 def test_predicate_errors(gpu_ctx, hjo, oracle):
     t = predicate_table(np.random.default_rng(1), 50)
